@@ -1,0 +1,478 @@
+/*
+ * CPU oracle (C/OpenMP restatement of the PyHermes Numba kernels).
+ * TEST INFRASTRUCTURE ONLY: used by tests/ as the checker and by bench.py's
+ * cpu_baseline leg as the timed CPU baseline.  Never linked into the product.
+ *
+ * Each function names the reference file:line it restates (paths relative to
+ * the PyHermes source tree).  Build: oracle/Makefile -> oracle/_build/liboracle.so
+ * with -ffp-contract=off so products and sums round like the Numba kernels.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+typedef struct { double re, im; } cplx;
+
+static inline cplx c_make(double re, double im) { cplx c = {re, im}; return c; }
+static inline cplx c_add(cplx a, cplx b) { return c_make(a.re + b.re, a.im + b.im); }
+static inline cplx c_mul(cplx a, cplx b) {
+    return c_make(a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re);
+}
+static inline cplx c_scale(double s, cplx a) { return c_make(s * a.re, s * a.im); }
+
+int oracle_num_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+/* ---------------------------------------------------------------------------
+ * Projection: wavelet_grid.py:17-70 (_project_scaling_grid_impl, full grid).
+ * Serial loop over particles, 27 FP64 adds per particle.  Python negative
+ * indexing wraps cells -1..-L; the oracle wraps every index modulo L so that
+ * in-range inputs behave exactly like the reference (cell == L is undefined
+ * behaviour there).  Optionally returns per-particle cell and fraction indices.
+ * ------------------------------------------------------------------------- */
+void oracle_project(const double *pos, const double *w, int64_t n,
+                    const double *phi, int phi_len, int phi_res,
+                    int J, double box, double *eps_out,
+                    int32_t *cell_out, int32_t *frac_out) {
+    const int64_t L = (int64_t)1 << J;
+    const int support = phi_len / phi_res;
+    const double sf = (double)L / box;
+    memset(eps_out, 0, sizeof(double) * L * L * L);
+    for (int64_t p = 0; p < n; ++p) {
+        int32_t c[3], f[3];
+        for (int d = 0; d < 3; ++d) {
+            double s = pos[3 * p + d] * sf;
+            c[d] = (int32_t)floor(s);
+            f[d] = (int32_t)((s - (double)c[d]) * (double)phi_res);
+        }
+        if (cell_out) { cell_out[3 * p] = c[0]; cell_out[3 * p + 1] = c[1]; cell_out[3 * p + 2] = c[2]; }
+        if (frac_out) { frac_out[3 * p] = f[0]; frac_out[3 * p + 1] = f[1]; frac_out[3 * p + 2] = f[2]; }
+        double wt = w[p];
+        for (int sx = 0; sx < support; ++sx) {
+            double phx = wt * phi[f[0] + phi_res * sx];
+            int64_t ix = (((int64_t)c[0] - sx) % L + L) % L;
+            for (int sy = 0; sy < support; ++sy) {
+                double phxy = phx * phi[f[1] + phi_res * sy];
+                int64_t iy = (((int64_t)c[1] - sy) % L + L) % L;
+                for (int sz = 0; sz < support; ++sz) {
+                    int64_t iz = (((int64_t)c[2] - sz) % L + L) % L;
+                    eps_out[(ix * L + iy) * L + iz] += phxy * phi[f[2] + phi_res * sz];
+                }
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Isotropic window transfer functions: window_functions.py:29-129, :426-438.
+ * code: 0 sphere(R) 1 gaussian(R) 2 shell(R) 3 thick_shell(R, dR)
+ *       4 gaussian_shell(R_shell, R_smooth) 5 inverse_laplacian()
+ * ------------------------------------------------------------------------- */
+static double win_sphere(double ki, double kj, double kk, double R) {
+    double k = sqrt(ki * ki + kj * kj + kk * kk);
+    double q = 2 * M_PI * k * R;
+    if (q == 0.0) return 1.0;
+    return 3 * (sin(q) - q * cos(q)) / (q * q * q);
+}
+
+double oracle_window_eval(int code, double ki, double kj, double kk, double a0, double a1) {
+    switch (code) {
+    case 0: return win_sphere(ki, kj, kk, a0);
+    case 1: {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        double q = 2 * M_PI * k * a0;
+        return exp(-(q * q) / 2);
+    }
+    case 2: {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        double q = 2 * M_PI * k * a0;
+        if (q == 0.0) return 1.0;
+        return sin(q) / q;
+    }
+    case 3: {
+        double ri = a0 - 0.5 * a1;
+        if (ri < 0.0) ri = 0.0;
+        double ro = a0 + 0.5 * a1;
+        double den = ro * ro * ro - ri * ri * ri;
+        if (den == 0.0) return 1.0;
+        return (ro * ro * ro * win_sphere(ki, kj, kk, ro) - ri * ri * ri * win_sphere(ki, kj, kk, ri)) / den;
+    }
+    case 4: {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        if (k == 0) return 1.0;
+        double qs = 2 * M_PI * k * a0;
+        double qm = 2 * M_PI * k * a1;
+        double den = a0 * a0 + a1 * a1;
+        if (den == 0.0) return 1.0;
+        double sinc = (qs == 0.0) ? 1.0 : sin(qs) / qs;
+        return (a1 * a1 * cos(qs) + a0 * a0 * sinc) / den * exp(-(qm * qm) / 2);
+    }
+    case 5: {
+        double k2 = ki * ki + kj * kj + kk * kk;
+        if (k2 == 0.0) return 0.0;
+        return -1.0 / (((2.0 * M_PI) * (2.0 * M_PI)) * k2);
+    }
+    }
+    return NAN;
+}
+
+/* convolution.py:14-38 (bandwidth 1) then convolution.py:263-281 fold. */
+void oracle_octant_rfft_kernel(int L, const double *P, int code, double a0, double a1,
+                               double *out /* (L, L, L/2+1) */) {
+    const int L1 = L + 1;
+    double *wa = (double *)malloc(sizeof(double) * (size_t)L1 * L1 * L1);
+    const double inv_L = 1.0 / L;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < L1; ++i)
+        for (int j = 0; j < L1; ++j)
+            for (int k = 0; k < L1; ++k) {
+                double t = 0.0;
+                t += P[i] * P[j] * P[k] * oracle_window_eval(code, i * inv_L, j * inv_L, k * inv_L, a0, a1);
+                wa[((size_t)i * L1 + j) * L1 + k] = t;
+            }
+    const int Lh = L / 2 + 1;
+#define WA(a, b, c) wa[((size_t)(a) * L1 + (b)) * L1 + (c)]
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < L; ++x)
+        for (int y = 0; y < L; ++y)
+            for (int z = 0; z < Lh; ++z)
+                out[((size_t)x * L + y) * Lh + z] =
+                    WA(x, y, z) + WA(L - x, y, z) + WA(x, L - y, z) + WA(x, y, L - z) +
+                    WA(L - x, L - y, z) + WA(L - x, y, L - z) + WA(x, L - y, L - z) +
+                    WA(L - x, L - y, L - z);
+#undef WA
+    free(wa);
+}
+
+/* convolution.py:73-157, bandwidth 1: the 8 explicit aliases, no octant symmetry. */
+void oracle_full_rfft_kernel(int L, const double *P, int code, double a0, double a1, double *out) {
+    const double inv_L = 1.0 / L;
+    const int Lh = L / 2 + 1;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < L; ++x) {
+        int xm = L - x;
+        for (int y = 0; y < L; ++y) {
+            int ym = L - y;
+            for (int z = 0; z < Lh; ++z) {
+                int zm = L - z;
+                double xp = x * inv_L, xn = -xm * inv_L;
+                double yp = y * inv_L, yn = -ym * inv_L;
+                double zp = z * inv_L, zn = -zm * inv_L;
+                double t = 0.0;
+                t += P[x] * P[y] * P[z] * oracle_window_eval(code, xp, yp, zp, a0, a1);
+                t += P[xm] * P[y] * P[z] * oracle_window_eval(code, xn, yp, zp, a0, a1);
+                t += P[x] * P[ym] * P[z] * oracle_window_eval(code, xp, yn, zp, a0, a1);
+                t += P[x] * P[y] * P[zm] * oracle_window_eval(code, xp, yp, zn, a0, a1);
+                t += P[xm] * P[ym] * P[z] * oracle_window_eval(code, xn, yn, zp, a0, a1);
+                t += P[xm] * P[y] * P[zm] * oracle_window_eval(code, xn, yp, zn, a0, a1);
+                t += P[x] * P[ym] * P[zm] * oracle_window_eval(code, xp, yn, zn, a0, a1);
+                t += P[xm] * P[ym] * P[zm] * oracle_window_eval(code, xn, yn, zn, a0, a1);
+                out[((size_t)x * L + y) * Lh + z] = t;
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * Special functions: special_functions.py:49-128 (spherical_jn_numba),
+ * :273-304 (assoc_legendre_numba), :307-334 (spherical_harmonic_numba).
+ * ------------------------------------------------------------------------- */
+double oracle_spherical_jn(int l, double x) {
+    if (l < 0) return NAN;
+    if (x == 0.0) return l == 0 ? 1.0 : 0.0;
+    double ax = fabs(x);
+    if (ax <= 5.0e-2) {
+        if (l == 0) {
+            double x2 = x * x;
+            return 1.0 - x2 / 6.0 + x2 * x2 / 120.0;
+        }
+        double denom = 1.0;
+        for (int n = 1; n <= l; ++n) denom *= 2.0 * n + 1.0;
+        double x2 = x * x;
+        return (pow(x, l) / denom) *
+               (1.0 - x2 / (2.0 * (2.0 * l + 3.0)) + x2 * x2 / (8.0 * (2.0 * l + 3.0) * (2.0 * l + 5.0)));
+    }
+    double sx = sin(x), cx = cos(x);
+    double j0 = sx / x;
+    if (l == 0) return j0;
+    double j1 = sx / (x * x) - cx / x;
+    if (l == 1) return j1;
+    if (ax < l + 1.0) {
+        int n_stop = l + 40;
+        double jp1 = 0.0, jcur = 1.0, jt = 0.0, jd0 = 0.0, jd1 = 0.0;
+        for (int n = n_stop; n > 0; --n) {
+            double jm1 = ((2.0 * n + 1.0) / x) * jcur - jp1;
+            if (n - 1 == l) jt = jm1;
+            if (n - 1 == 1) jd1 = jm1;
+            if (n - 1 == 0) jd0 = jm1;
+            jp1 = jcur;
+            jcur = jm1;
+            if (fabs(jcur) > 1.0e200) {
+                jp1 *= 1.0e-200; jcur *= 1.0e-200; jt *= 1.0e-200; jd0 *= 1.0e-200; jd1 *= 1.0e-200;
+            } else if (fabs(jcur) < 1.0e-200 && jcur != 0.0) {
+                jp1 *= 1.0e200; jcur *= 1.0e200; jt *= 1.0e200; jd0 *= 1.0e200; jd1 *= 1.0e200;
+            }
+        }
+        if (fabs(jd0) >= fabs(jd1) || jd1 == 0.0) return jt * (j0 / jd0);
+        return jt * (j1 / jd1);
+    }
+    double jm1 = j0, jcur = j1;
+    for (int ell = 1; ell < l; ++ell) {
+        double jn = ((2.0 * ell + 1.0) / x) * jcur - jm1;
+        jm1 = jcur;
+        jcur = jn;
+    }
+    return jcur;
+}
+
+double oracle_assoc_legendre(int l, int m, double x) {
+    if (l < 0 || m < 0 || m > l) return NAN;
+    double pmm = 1.0;
+    if (m > 0) {
+        double t = 1.0 - x * x;
+        double somx2 = sqrt(t > 0.0 ? t : 0.0);
+        double fact = 1.0;
+        for (int i = 0; i < m; ++i) { pmm *= -fact * somx2; fact += 2.0; }
+    }
+    if (l == m) return pmm;
+    double pmmp1 = x * (2.0 * m + 1.0) * pmm;
+    if (l == m + 1) return pmmp1;
+    double a = pmm, b = pmmp1, pll = 0.0;
+    for (int ell = m + 2; ell <= l; ++ell) {
+        pll = ((2.0 * ell - 1.0) * x * b - (ell + m - 1.0) * a) / (ell - m);
+        a = b;
+        b = pll;
+    }
+    return pll;
+}
+
+static double factorial_small(int n) {
+    double out = 1.0;
+    for (int i = 2; i <= n; ++i) out *= i;
+    return out;
+}
+
+cplx oracle_sph_harm_c(int l, int m, double ki, double kj, double kk) {
+    if (l < 0) return c_make(NAN, 0.0);
+    if (abs(m) > l) return c_make(0.0, 0.0);
+    double k = sqrt(ki * ki + kj * kj + kk * kk);
+    if (k == 0.0) {
+        if (l == 0 && m == 0) return c_make(1.0 / sqrt(4.0 * M_PI), 0.0);
+        return c_make(0.0, 0.0);
+    }
+    double mu = kk / k;
+    if (mu > 1.0) mu = 1.0; else if (mu < -1.0) mu = -1.0;
+    double theta = acos(mu);
+    double ph = atan2(kj, ki);
+    double x = cos(theta);
+    if (m >= 0) {
+        double P = oracle_assoc_legendre(l, m, x);
+        double norm = sqrt((2.0 * l + 1.0) / (4.0 * M_PI) * factorial_small(l - m) / factorial_small(l + m));
+        double s = norm * P;
+        return c_make(s * cos(m * ph), s * sin(m * ph));
+    }
+    cplx yp = oracle_sph_harm_c(l, -m, ki, kj, kk);
+    double sign = ((-m) % 2 == 1) ? -1.0 : 1.0;
+    return c_make(sign * yp.re, -sign * yp.im);
+}
+
+void oracle_sph_harm(int l, int m, double ki, double kj, double kk, double *re, double *im) {
+    cplx y = oracle_sph_harm_c(l, m, ki, kj, kk);
+    *re = y.re;
+    *im = y.im;
+}
+
+/* radial_multipole_windows.py:128-144, shell profile (code 0) only. */
+static cplx radial_multipole_eval(double ki, double kj, double kk, double R, int l, int m) {
+    if (l < 0) return c_make(NAN, 0.0);
+    if (abs(m) > l) return c_make(0.0, 0.0);
+    double k = sqrt(ki * ki + kj * kj + kk * kk);
+    double radial = oracle_spherical_jn(l, 2.0 * M_PI * k * R);
+    cplx ang = oracle_sph_harm_c(l, m, ki, kj, kk);
+    return c_scale(radial, ang);
+}
+
+/* radial_multipole_windows.py:147-183 -> (L, L, L) complex128, interleaved re/im. */
+void oracle_radial_multipole_window(int L, const double *P, double R, int l, int m, double *out) {
+    const double inv_L = 1.0 / L;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < L; ++x) {
+        double pi0 = P[L - x], pi1 = P[x];
+        double ki0 = (x - L) * inv_L, ki1 = x * inv_L;
+        for (int y = 0; y < L; ++y) {
+            double pj0 = P[L - y], pj1 = P[y];
+            double kj0 = (y - L) * inv_L, kj1 = y * inv_L;
+            for (int z = 0; z < L; ++z) {
+                double pk0 = P[L - z], pk1 = P[z];
+                double kk0 = (z - L) * inv_L, kk1 = z * inv_L;
+                cplx t = c_scale(pi0 * pj0 * pk0, radial_multipole_eval(ki0, kj0, kk0, R, l, m));
+                t = c_add(t, c_scale(pi0 * pj0 * pk1, radial_multipole_eval(ki0, kj0, kk1, R, l, m)));
+                t = c_add(t, c_scale(pi0 * pj1 * pk0, radial_multipole_eval(ki0, kj1, kk0, R, l, m)));
+                t = c_add(t, c_scale(pi0 * pj1 * pk1, radial_multipole_eval(ki0, kj1, kk1, R, l, m)));
+                t = c_add(t, c_scale(pi1 * pj0 * pk0, radial_multipole_eval(ki1, kj0, kk0, R, l, m)));
+                t = c_add(t, c_scale(pi1 * pj0 * pk1, radial_multipole_eval(ki1, kj0, kk1, R, l, m)));
+                t = c_add(t, c_scale(pi1 * pj1 * pk0, radial_multipole_eval(ki1, kj1, kk0, R, l, m)));
+                t = c_add(t, c_scale(pi1 * pj1 * pk1, radial_multipole_eval(ki1, kj1, kk1, R, l, m)));
+                size_t o = ((size_t)x * L + y) * L + z;
+                out[2 * o] = t.re;
+                out[2 * o + 1] = t.im;
+            }
+        }
+    }
+}
+
+/* ---------------------------------------------------------------------------
+ * corr3pcf_multipoles.py:134-170 (compute_3d_summand_cpu): returns the raw
+ * complex sum (without the 4*pi/N factor).  F1, F2 interleaved complex128.
+ * Per-thread partials are combined in thread order (deterministic for a
+ * fixed thread count).
+ * ------------------------------------------------------------------------- */
+void oracle_summand(const double *data, const double *F1, const double *F2, const double *gamma,
+                    int L, int support, double *out_re, double *out_im) {
+    const int64_t N = (int64_t)L * L * L;
+    const int64_t plane = (int64_t)L * L;
+    int nt = oracle_num_threads();
+    double *part = (double *)calloc(2 * (size_t)nt, sizeof(double));
+#pragma omp parallel
+    {
+        int tid = 0;
+#ifdef _OPENMP
+        tid = omp_get_thread_num();
+#endif
+        double tr = 0.0, ti = 0.0;
+#pragma omp for schedule(static)
+        for (int64_t idx = 0; idx < N; ++idx) {
+            int64_t lx = idx / plane;
+            int64_t rem = idx - lx * plane;
+            int64_t ly = rem / L;
+            int64_t lz = rem - ly * L;
+            cplx s1 = c_make(0.0, 0.0);
+            for (int l1x = 0; l1x < support; ++l1x) {
+                int64_t i1x = ((lx - l1x) % L + L) % L;
+                for (int l1y = 0; l1y < support; ++l1y) {
+                    int64_t i1y = ((ly - l1y) % L + L) % L;
+                    for (int l1z = 0; l1z < support; ++l1z) {
+                        int64_t i1z = ((lz - l1z) % L + L) % L;
+                        cplx s2 = c_make(0.0, 0.0);
+                        for (int l2x = 0; l2x < support; ++l2x) {
+                            int64_t i2x = ((lx - l2x) % L + L) % L;
+                            double gx = gamma[l1x * support + l2x];
+                            for (int l2y = 0; l2y < support; ++l2y) {
+                                int64_t i2y = ((ly - l2y) % L + L) % L;
+                                double gxy = gx * gamma[l1y * support + l2y];
+                                for (int l2z = 0; l2z < support; ++l2z) {
+                                    int64_t i2z = ((lz - l2z) % L + L) % L;
+                                    double wgt = gxy * gamma[l1z * support + l2z];
+                                    int64_t o = (i2x * L + i2y) * L + i2z;
+                                    s2 = c_add(s2, c_make(wgt * F2[2 * o], wgt * F2[2 * o + 1]));
+                                }
+                            }
+                        }
+                        int64_t o1 = (i1x * L + i1y) * L + i1z;
+                        s1 = c_add(s1, c_mul(c_make(F1[2 * o1], F1[2 * o1 + 1]), s2));
+                    }
+                }
+            }
+            cplx v = c_scale(data[idx], s1);
+            tr += v.re;
+            ti += v.im;
+        }
+        part[2 * tid] = tr;
+        part[2 * tid + 1] = ti;
+    }
+    double sr = 0.0, si = 0.0;
+    for (int t = 0; t < nt; ++t) { sr += part[2 * t]; si += part[2 * t + 1]; }
+    free(part);
+    *out_re = sr;
+    *out_im = si;
+}
+
+/* ---------------------------------------------------------------------------
+ * Monte-Carlo triplet estimator.
+ * wavelet_grid.py:143-186 (interpolate_grid_at_pos_numba) and
+ * corr3pcf_kernels.py:21-63 (generate_triangle_offsets), :100-140 (box-random
+ * centres).  The per-rotation uniforms (a, b, c) are drawn by the caller.
+ * ------------------------------------------------------------------------- */
+void oracle_interpolate(double *out, const double *spos, int64_t n, const double *grid,
+                        const double *phi, int L, int phi_res, int support,
+                        double dx, double dy, double dz) {
+    const int64_t mask = L - 1;
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; ++p) {
+        double sx = spos[3 * p] + dx, sy = spos[3 * p + 1] + dy, sz = spos[3 * p + 2] + dz;
+        int64_t xc = (int64_t)floor(sx), yc = (int64_t)floor(sy), zc = (int64_t)floor(sz);
+        int64_t xf = (int64_t)((sx - (double)xc) * phi_res);
+        int64_t yf = (int64_t)((sy - (double)yc) * phi_res);
+        int64_t zf = (int64_t)((sz - (double)zc) * phi_res);
+        double acc = 0.0;
+        for (int a = 0; a < support; ++a) {
+            int64_t xg = (xc - a) & mask;
+            double px = phi[xf + a * phi_res];
+            for (int b = 0; b < support; ++b) {
+                int64_t yg = (yc - b) & mask;
+                double py = phi[yf + b * phi_res];
+                for (int c = 0; c < support; ++c) {
+                    int64_t zg = (zc - c) & mask;
+                    double pz = phi[zf + c * phi_res];
+                    acc += grid[(xg * L + yg) * L + zg] * px * py * pz;
+                }
+            }
+        }
+        out[p] = acc;
+    }
+}
+
+void oracle_triangle_offsets(double R1, double R2, double mu, double phi, double ct1, double alpha,
+                             double *o /* x2 y2 z2 x3 y3 z3 */) {
+    double st1 = sqrt(1.0 - ct1 * ct1);
+    double dx1 = st1 * cos(phi), dy1 = st1 * sin(phi), dz1 = ct1;
+    o[0] = R1 * dx1; o[1] = R1 * dy1; o[2] = R1 * dz1;
+    double ox1, oy1, oz1;
+    if (fabs(dx1) > 1e-10 || fabs(dy1) > 1e-10) { ox1 = -dy1; oy1 = dx1; oz1 = 0.0; }
+    else { ox1 = 0.0; oy1 = -dz1; oz1 = dy1; }
+    double nrm = sqrt(ox1 * ox1 + oy1 * oy1 + oz1 * oz1);
+    ox1 /= nrm; oy1 /= nrm; oz1 /= nrm;
+    double ox2 = dy1 * oz1 - dz1 * oy1;
+    double oy2 = dz1 * ox1 - dx1 * oz1;
+    double oz2 = dx1 * oy1 - dy1 * ox1;
+    double ca = cos(alpha), sa = sin(alpha);
+    double st2 = 1.0 - mu * mu;
+    if (st2 < 0.0) st2 = 0.0;
+    double st = sqrt(st2);
+    double dx2 = mu * dx1 + st * (ca * ox1 + sa * ox2);
+    double dy2 = mu * dy1 + st * (ca * oy1 + sa * oy2);
+    double dz2 = mu * dz1 + st * (ca * oz1 + sa * oz2);
+    o[3] = R2 * dx2; o[4] = R2 * dy2; o[5] = R2 * dz2;
+}
+
+/* corr3pcf_kernels.py:100-140 with caller-supplied uniforms abc (n_rot x 3). */
+double oracle_triplet_box_random(double R1, double R2, double mu, const double *centers, int64_t n,
+                                 int n_rot, const double *abc, const double *e1, const double *e2,
+                                 const double *e3, const double *phi, int L, int phi_res, int support) {
+    double *n1 = (double *)malloc(sizeof(double) * n);
+    double *n2 = (double *)malloc(sizeof(double) * n);
+    double *n3 = (double *)malloc(sizeof(double) * n);
+    oracle_interpolate(n1, centers, n, e1, phi, L, phi_res, support, 0.0, 0.0, 0.0);
+    double total = 0.0;
+    for (int r = 0; r < n_rot; ++r) {
+        double o[6];
+        oracle_triangle_offsets(R1, R2, mu, 2.0 * M_PI * abc[3 * r], 2.0 * abc[3 * r + 1] - 1.0,
+                                2.0 * M_PI * abc[3 * r + 2], o);
+        oracle_interpolate(n2, centers, n, e2, phi, L, phi_res, support, o[0], o[1], o[2]);
+        oracle_interpolate(n3, centers, n, e3, phi, L, phi_res, support, o[3], o[4], o[5]);
+        double s = 0.0;
+        for (int64_t i = 0; i < n; ++i) s += n1[i] * n2[i] * n3[i];
+        total += s;
+    }
+    free(n1); free(n2); free(n3);
+    return total / ((double)n_rot * (double)n);
+}

@@ -1,0 +1,115 @@
+"""Oracle: window kernels, special functions, FFT convolutions and pair products.
+
+Test infrastructure only.
+"""
+
+import numpy as np
+from scipy.fft import fftn, ifftn, irfftn, rfftn
+
+from . import native
+
+# Window type codes shared with oracle/c/oracle.c (window_functions.py:441-458 names).
+WINDOW_CODES = {"sphere": 0, "gaussian": 1, "shell": 2, "thick_shell": 3, "gaussian_shell": 4,
+                "inverse_laplacian": 5}
+
+
+def window_args(win_type, rescaled_len_args):
+    """Positional arguments of the built-in transfer functions (window_functions.py:29-129)."""
+    if win_type in ("sphere", "gaussian", "shell"):
+        return float(rescaled_len_args["R"]), 0.0
+    if win_type == "thick_shell":
+        return float(rescaled_len_args["R"]), float(rescaled_len_args["delta_R"])
+    if win_type == "gaussian_shell":
+        return float(rescaled_len_args["R_shell"]), float(rescaled_len_args["R_smooth"])
+    if win_type == "inverse_laplacian":
+        return 0.0, 0.0
+    raise ValueError(win_type)
+
+
+def octant_rfft_kernel(L, P, win_type, a0, a1=0.0):
+    """convolution.py:14-38 + :263-281 (bandwidth 1) -> (L, L, L//2+1) float64."""
+    out = np.empty((L, L, L // 2 + 1), dtype=np.float64)
+    P = native.f64(P)
+    native.lib().oracle_octant_rfft_kernel(L, native.ptr(P), WINDOW_CODES[win_type], a0, a1, native.ptr(out))
+    if win_type == "inverse_laplacian":  # window.py:266-272 zero-mode convention
+        out[0, 0, 0] = 0.0
+    return out
+
+
+def full_rfft_kernel(L, P, win_type, a0, a1=0.0):
+    """convolution.py:73-157 (explicit 8 aliases, bandwidth 1) -> (L, L, L//2+1) float64."""
+    out = np.empty((L, L, L // 2 + 1), dtype=np.float64)
+    P = native.f64(P)
+    native.lib().oracle_full_rfft_kernel(L, native.ptr(P), WINDOW_CODES[win_type], a0, a1, native.ptr(out))
+    if win_type == "inverse_laplacian":
+        out[0, 0, 0] = 0.0
+    return out
+
+
+def radial_multipole_window(L, P, R_grid, l, m):
+    """radial_multipole_windows.py:147-183 (thin shell) -> (L, L, L) complex128."""
+    out = np.empty((L, L, L), dtype=np.complex128)
+    P = native.f64(P)
+    native.lib().oracle_radial_multipole_window(L, native.ptr(P), float(R_grid), int(l), int(m),
+                                                native.ptr(out.view(np.float64)))
+    return out
+
+
+def spherical_jn(l, x):
+    """special_functions.py:49-128 (scalar)."""
+    return native.lib().oracle_spherical_jn(int(l), float(x))
+
+
+def sph_harm(l, m, ki, kj, kk):
+    """special_functions.py:307-334 (scalar) -> complex."""
+    re = native.ctypes.c_double()
+    im = native.ctypes.c_double()
+    native.lib().oracle_sph_harm(int(l), int(m), float(ki), float(kj), float(kk),
+                                 native.ctypes.byref(re), native.ctypes.byref(im))
+    return complex(re.value, im.value)
+
+
+def assoc_legendre(l, m, x):
+    """special_functions.py:273-304 (scalar)."""
+    return native.lib().oracle_assoc_legendre(int(l), int(m), float(x))
+
+
+def convolve_rfft(field, kernel, workers=1):
+    """convolution.py:284-288."""
+    sc = rfftn(field, workers=workers)
+    sc *= kernel
+    return irfftn(sc, s=field.shape, workers=workers)
+
+
+def convolve_complex(field, kernel, workers=1):
+    """convolution.py:291-295."""
+    sc = fftn(field, workers=workers)
+    sc *= kernel
+    return ifftn(sc, workers=workers)
+
+
+def pair_product(field1, field2, kernel, workers=1):
+    """corr2pcf.py:366-372: sum_x (f1 @ W)(x) f2(x) / N."""
+    conv = convolve_rfft(field1, kernel, workers)
+    return float(np.einsum("ijk,ijk->", conv, field2, optimize=True) / conv.size)
+
+
+def pair_product_parseval(field1, field2, kernel):
+    """Independent Parseval form: (1/N^2) sum_k W(k) F1(k) conj(F2(k)) over the full spectrum."""
+    L = field1.shape[0]
+    F1 = np.fft.fftn(field1)
+    F2 = np.fft.fftn(field2)
+    # expand the rFFT kernel to the full grid by Hermitian symmetry of a real kernel
+    Wf = np.empty((L, L, L))
+    h = L // 2 + 1
+    Wf[:, :, :h] = kernel
+    ix = (-np.arange(L)) % L
+    Wf[:, :, h:] = kernel[ix][:, ix][:, :, (L - np.arange(h, L)) % L]
+    return float(np.real(np.sum(Wf * F1 * np.conj(F2))) / L ** 6)
+
+
+def pair_product_realspace(field1, field2, win_real):
+    """Brute force sum_x sum_y w(x - y) f1(y) f2(x) / N with a real-space window array."""
+    L = field1.shape[0]
+    conv = np.real(np.fft.ifftn(np.fft.fftn(win_real) * np.fft.fftn(field1)))
+    return float(np.sum(conv * field2) / L ** 3)

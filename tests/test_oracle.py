@@ -1,0 +1,276 @@
+"""Pins the CPU oracle with invariants, closed forms, library routines and brute force.
+
+These run without a GPU.  A plausible mistake in the oracle (a dropped term, a
+wrong sign or index, a transposed operand) fails at least one of them.
+"""
+
+import numpy as np
+import pytest
+import scipy.special as sps
+
+from oracle import mc, multipole, projection, scaling, windows
+
+
+# ---------------------------------------------------------------- phi / P / Gamma
+def test_phi_invariants(phi):
+    assert phi.shape == (3072,)
+    assert phi[0] == 0.0 and phi[-1] == 0.0
+    assert abs(phi.sum() - 1024.0) < 1e-11
+    pu = phi.reshape(3, 1024).sum(axis=0)
+    np.testing.assert_allclose(pu, 1.0, atol=1e-14)
+    # db2 scaling function peak value (Daubechies 1988 tables give ~1.3660)
+    assert abs(phi.max() - 1.3660) < 1e-3
+
+
+def test_power_spectrum_matches_cascade_closed_form(phi):
+    L = 64
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    closed = scaling.cascade_power_closed_form(np.arange(L + 1) / L)
+    np.testing.assert_allclose(P, closed, rtol=1e-11, atol=1e-15)
+    assert abs(P[0] - 1.0) < 1e-12
+    assert P[L] < 1e-25
+
+
+def test_gamma_bruteforce_and_partition(phi):
+    g = scaling.cal_gamma(phi, 3, 1024)
+    n = len(phi)
+    i = np.arange(n)
+    brute = np.zeros((3, 3))
+    for l1 in range(3):
+        for l2 in range(3):
+            brute[l1, l2] = np.sum(phi * phi[(i - 1024 * l1) % n] * phi[(i - 1024 * l2) % n]) / 1024
+    np.testing.assert_allclose(g, brute, rtol=0, atol=1e-15)
+    np.testing.assert_allclose(g, g.T, atol=1e-16)
+    assert abs(g.sum() - 1.0) < 1e-12
+
+
+# ---------------------------------------------------------------- projection
+def _positions(rng, n, box):
+    pos = rng.uniform(0, box, size=(n, 3))
+    pos[:5] = rng.uniform(0, box * 1e-3, size=(5, 3))  # exercise the negative-index wrap
+    return pos
+
+
+@pytest.mark.parametrize("J", [3, 4])
+def test_projection_c_matches_bruteforce(phi, J):
+    rng = np.random.default_rng(11 + J)
+    box = 100.0
+    pos = _positions(rng, 300, box)
+    w = rng.lognormal(0, 0.3, 300)
+    eps_c, cell, frac = projection.project_c(pos, w, phi, J, box)
+    eps_b = projection.project_bruteforce(pos, w, phi, J, box)
+    np.testing.assert_allclose(eps_c, eps_b, rtol=0, atol=1e-13 * np.abs(eps_b).max())
+    assert abs(eps_c.sum() - w.sum()) < 1e-11 * w.sum()
+    c2, f2 = projection.cell_and_frac(pos, J, box)
+    np.testing.assert_array_equal(cell, c2)
+    np.testing.assert_array_equal(frac, f2)
+
+
+def test_projection_cell_frac_closed_form():
+    # positions on exact binary fractions: with box == L, scale_factor == 1
+    J = 4
+    L = 1 << J
+    rng = np.random.default_rng(3)
+    c = rng.integers(0, L, size=(200, 3))
+    f = rng.integers(0, 1024, size=(200, 3))
+    pos = c + f / 1024.0 + 2.0 ** -20
+    cell, frac = projection.cell_and_frac(pos, J, float(L))
+    np.testing.assert_array_equal(cell, c)
+    np.testing.assert_array_equal(frac, f)
+
+
+def test_projection_whole_cell_shift_is_roll(phi):
+    J, box = 4, 64.0
+    L = 1 << J
+    rng = np.random.default_rng(5)
+    pos = rng.uniform(4 * box / L, box - 4 * box / L, size=(200, 3))
+    w = rng.uniform(0.5, 1.5, 200)
+    e0 = projection.project_bruteforce(pos, w, phi, J, box)
+    e1 = projection.project_bruteforce(pos + np.array([1, 0, 2]) * (box / L), w, phi, J, box)
+    np.testing.assert_allclose(e1, np.roll(e0, (1, 0, 2), axis=(0, 1, 2)), atol=1e-13)
+
+
+def test_projection_weights_modes():
+    cw = np.array([1.0, 2.0, 3.0])
+    fv = np.array([2.0, 1.0, 0.5])
+    pw, s = projection.projection_weights(cw, fv, "catalog")
+    np.testing.assert_allclose(pw, cw * fv / 6.0)
+    assert s["field_integral"] == pytest.approx(5.5 / 6.0)
+    pw, s = projection.projection_weights(cw, fv, "unit")
+    assert s["field_integral"] == pytest.approx(1.0)
+    pw, s = projection.projection_weights(cw, fv, "raw")
+    np.testing.assert_allclose(pw, cw * fv)
+
+
+# ---------------------------------------------------------------- special functions
+@pytest.mark.parametrize("l", list(range(0, 13)))
+def test_spherical_jn_against_scipy(l):
+    xs = np.concatenate([[1e-3, 0.02, 0.05, 0.06], np.linspace(0.1, l + 1.5, 17), [20.0, 77.7, 300.0, 650.0]])
+    for x in xs:
+        ours = windows.spherical_jn(l, x)
+        ref = sps.spherical_jn(l, x)
+        assert abs(ours - ref) <= 1e-9 * max(abs(ref), 1e-12) + 1e-15, (l, x, ours, ref)
+
+
+def test_sph_harm_against_scipy():
+    rng = np.random.default_rng(0)
+    for _ in range(40):
+        k = rng.normal(size=3)
+        theta = np.arccos(k[2] / np.linalg.norm(k))
+        ph = np.arctan2(k[1], k[0])
+        for l in range(0, 11):
+            for m in range(-l, l + 1):
+                ours = windows.sph_harm(l, m, *k)
+                ref = sps.sph_harm_y(l, m, theta, ph)
+                assert abs(ours - ref) < 1e-12 * max(1.0, abs(ref)), (l, m)
+    assert windows.sph_harm(0, 0, 0, 0, 0) == pytest.approx(1 / np.sqrt(4 * np.pi))
+    assert windows.sph_harm(2, 1, 0, 0, 0) == 0
+
+
+# ---------------------------------------------------------------- window kernels
+@pytest.mark.parametrize("wt,args", [("sphere", (2.5, 0)), ("gaussian", (1.5, 0)), ("shell", (3.0, 0)),
+                                      ("thick_shell", (3.0, 1.5)), ("gaussian_shell", (3.0, 1.0)),
+                                      ("inverse_laplacian", (0, 0))])
+def test_octant_fold_equals_explicit_alias_builder(phi, wt, args):
+    L = 8
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    a = windows.octant_rfft_kernel(L, P, wt, *args)
+    b = windows.full_rfft_kernel(L, P, wt, *args)
+    np.testing.assert_allclose(a, b, rtol=1e-12, atol=1e-15)
+    if wt != "inverse_laplacian":
+        assert windows.native.lib().oracle_window_eval(windows.WINDOW_CODES[wt], 0, 0, 0, *args) == 1.0
+
+
+def test_multipole_l0_equals_shell_kernel(phi):
+    L = 8
+    R = 2.3
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    K00 = windows.radial_multipole_window(L, P, R, 0, 0)
+    Ks = windows.full_rfft_kernel(L, P, "shell", R)
+    np.testing.assert_allclose(K00[:, :, :L // 2 + 1].real * np.sqrt(4 * np.pi), Ks, rtol=1e-12, atol=1e-15)
+    assert np.abs(K00.imag).max() < 1e-15
+
+
+def test_multipole_conjugation_symmetry(phi):
+    L = 8
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    neg = (-np.arange(L)) % L
+    for l, m in [(1, 1), (2, 1), (3, 2), (4, 0)]:
+        Kp = windows.radial_multipole_window(L, P, 2.7, l, m)
+        Km = windows.radial_multipole_window(L, P, 2.7, l, -m)
+        Kp_neg = Kp[neg][:, neg][:, :, neg]
+        np.testing.assert_allclose(Km, (-1) ** (l + m) * np.conj(Kp_neg), atol=1e-14)
+
+
+# ---------------------------------------------------------------- convolution / pair product
+def test_pair_product_parseval(phi):
+    L = 8
+    rng = np.random.default_rng(1)
+    f1 = rng.normal(size=(L, L, L))
+    f2 = rng.normal(size=(L, L, L))
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    K = windows.octant_rfft_kernel(L, P, "shell", 2.0)
+    a = windows.pair_product(f1, f2, K)
+    b = windows.pair_product_parseval(f1, f2, K)
+    assert abs(a - b) < 1e-12 * max(1.0, abs(b))
+
+
+def test_convolution_matches_realspace_circular(phi):
+    L = 8
+    rng = np.random.default_rng(2)
+    f = rng.normal(size=(L, L, L))
+    w = rng.normal(size=(L, L, L))
+    K = np.fft.rfftn(w)
+    conv = windows.convolve_rfft(f, K)
+    brute = np.zeros_like(f)
+    for s in np.ndindex(L, L, L):
+        brute += w[s] * np.roll(f, s, axis=(0, 1, 2))
+    np.testing.assert_allclose(conv, brute, atol=1e-10)
+
+
+# ---------------------------------------------------------------- multipole contraction
+def test_summand_matches_bruteforce(phi):
+    L = 8
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=(L, L, L))
+    F1 = rng.normal(size=(L, L, L)) + 1j * rng.normal(size=(L, L, L))
+    F2 = rng.normal(size=(L, L, L)) + 1j * rng.normal(size=(L, L, L))
+    g = scaling.cal_gamma(phi, 3, 1024)
+    a = multipole.summand_c(d, F1, F2, g)
+    b = multipole.summand_bruteforce(d, F1, F2, g)
+    assert abs(a - b) < 1e-11 * abs(b)
+
+
+def test_summand_constant_closed_form(phi):
+    L = 8
+    g = scaling.cal_gamma(phi, 3, 1024)
+    d = np.full((L, L, L), 0.7)
+    F1 = np.full((L, L, L), 1.5 - 0.5j)
+    F2 = np.full((L, L, L), -0.25 + 2.0j)
+    v = multipole.m_summand(d, F1, F2, g)
+    expect = 4 * np.pi * 0.7 * (1.5 - 0.5j) * (-0.25 + 2.0j)
+    assert abs(v - expect) < 1e-12 * abs(expect)
+
+
+def test_combine_m_terms_closed_form():
+    vals = np.array([1.0 + 2.0j, 0.5 - 1.0j, 0.25 + 0.75j])
+    got = multipole.combine_multipole_m_terms(vals, 2)
+    expect = (vals[0] - 2 * vals[1].real + 2 * vals[2].real).real
+    assert got == pytest.approx(expect)
+
+
+def test_ratio_solve_against_legendre_product():
+    from numpy.polynomial import legendre as leg
+    rng = np.random.default_rng(9)
+    lmax = 6
+    B = np.concatenate([[2.0], rng.normal(size=lmax) * 0.3])
+    C = rng.normal(size=lmax + 1)
+    A = leg.legmul(B, C)[:lmax + 1]
+    M = multipole.build_mixing_matrix(B, lmax)
+    np.testing.assert_allclose(M @ C, A, rtol=1e-12, atol=1e-12)
+    sol, _, _ = multipole.solve_multipoles_from_ratio(A, B, lmax)
+    np.testing.assert_allclose(sol, C, rtol=1e-10, atol=1e-12)
+
+
+# ---------------------------------------------------------------- Monte Carlo
+def test_interpolation_matches_bruteforce(phi):
+    L = 16
+    rng = np.random.default_rng(6)
+    grid = rng.normal(size=(L, L, L))
+    sp = rng.uniform(0, L, size=(500, 3))
+    a = mc.interpolate(grid, sp, phi, (1.3, -2.7, 0.4))
+    b = mc.interpolate_bruteforce(grid, sp, phi, (1.3, -2.7, 0.4))
+    np.testing.assert_allclose(a, b, atol=1e-13)
+    const = mc.interpolate(np.full((L, L, L), 2.5), sp, phi)
+    np.testing.assert_allclose(const, 2.5, atol=1e-12)
+
+
+def test_triangle_offsets_geometry():
+    rng = np.random.default_rng(8)
+    for _ in range(50):
+        R1, R2, mu = rng.uniform(1, 10), rng.uniform(1, 10), rng.uniform(-1, 1)
+        o = mc.triangle_offsets(R1, R2, mu, rng.uniform(0, 2 * np.pi), rng.uniform(-1, 1),
+                                rng.uniform(0, 2 * np.pi))
+        a, b = o[:3], o[3:]
+        assert np.linalg.norm(a) == pytest.approx(R1, rel=1e-12)
+        assert np.linalg.norm(b) == pytest.approx(R2, rel=1e-12)
+        assert a @ b / (R1 * R2) == pytest.approx(mu, abs=1e-12)
+
+
+def test_triplet_estimator_constant_and_bruteforce(phi):
+    L = 16
+    rng = np.random.default_rng(10)
+    centers = rng.uniform(0, L, size=(200, 3))
+    abc = mc.rotation_uniforms(43, 5, 2)
+    c = [np.full((L, L, L), v) for v in (1.5, 2.0, 0.5)]
+    est = mc.triplet_box_random(3.0, 4.0, 0.3, centers, abc, *c, phi)
+    assert est == pytest.approx(1.5, rel=1e-12)
+    e = [rng.normal(size=(L, L, L)) for _ in range(3)]
+    est = mc.triplet_box_random(3.0, 4.0, 0.3, centers, abc, *e, phi)
+    n1 = mc.interpolate_bruteforce(e[0], centers, phi)
+    tot = 0.0
+    for r in range(abc.shape[0]):
+        o = mc.triangle_offsets(3.0, 4.0, 0.3, 2 * np.pi * abc[r, 0], 2 * abc[r, 1] - 1, 2 * np.pi * abc[r, 2])
+        tot += np.sum(n1 * mc.interpolate_bruteforce(e[1], centers, phi, o[:3])
+                      * mc.interpolate_bruteforce(e[2], centers, phi, o[3:]))
+    assert est == pytest.approx(tot / (abc.shape[0] * centers.shape[0]), rel=1e-11)
