@@ -1,0 +1,96 @@
+"""ctypes binding of libhermes_amd.so (the HIP/gfx950 C ABI declared in include/hermes_amd.h).
+
+The product has no CPU fallback: if the library is missing or no HIP device is
+present, every compute entry point raises RuntimeError.
+"""
+
+import ctypes
+import os
+import threading
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libhermes_amd.so")
+
+_lib = None
+_ctx = {}
+_lock = threading.Lock()
+
+c_int, c_i64, c_double, c_void_p = ctypes.c_int, ctypes.c_int64, ctypes.c_double, ctypes.c_void_p
+
+# name -> argtypes (all return int status unless listed in _RESTYPES)
+_SIGNATURES = {
+    "hermes_version": [],
+    "hermes_last_error": [],
+    "hermes_ctx_create": [c_int, ctypes.POINTER(c_void_p)],
+    "hermes_ctx_destroy": [c_void_p],
+    "hermes_ctx_set_stream": [c_void_p, c_void_p],
+    "hermes_sync": [c_void_p],
+    "hermes_project": [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_double, c_void_p, c_int, c_int, c_double,
+                       c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
+    "hermes_fft_r2c": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p],
+    "hermes_fft_c2r": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p],
+    "hermes_fft_c2c": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
+}
+_RESTYPES = {"hermes_last_error": ctypes.c_char_p}
+
+
+def declared_symbols():
+    return list(_SIGNATURES)
+
+
+def load():
+    """Load the shared library (raises RuntimeError if it was not built)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(
+            f"pyhermes_amd: HIP library not found at {LIB_PATH}; build it with "
+            "`python -c 'import __graft_entry__ as g; g.build()'` (make -C pyhermes_amd/csrc).")
+    lib = ctypes.CDLL(LIB_PATH)
+    for name, argtypes in _SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.argtypes = argtypes
+        fn.restype = _RESTYPES.get(name, c_int)
+    _lib = lib
+    return lib
+
+
+def last_error():
+    return load().hermes_last_error().decode("utf-8", "replace")
+
+
+def check(rc, what):
+    if rc != 0:
+        raise RuntimeError(f"{what} failed (status {rc}): {last_error()}")
+
+
+def call(name, *args):
+    fn = getattr(load(), name)
+    check(fn(*args), name)
+
+
+def context(device=None):
+    """Per-device library context bound to torch's current HIP stream."""
+    import torch
+    if not torch.cuda.is_available():
+        raise RuntimeError("pyhermes_amd requires a HIP device (torch.cuda.is_available() is False)")
+    if device is None:
+        device = torch.cuda.current_device()
+    with _lock:
+        ctx = _ctx.get(device)
+        if ctx is None:
+            handle = c_void_p()
+            call("hermes_ctx_create", int(device), ctypes.byref(handle))
+            ctx = handle
+            _ctx[device] = ctx
+    stream = torch.cuda.current_stream(device).cuda_stream
+    call("hermes_ctx_set_stream", ctx, c_void_p(stream))
+    return ctx
+
+
+def ptr(t):
+    """Device pointer of a torch tensor (or None -> NULL)."""
+    if t is None:
+        return None
+    return c_void_p(t.data_ptr())

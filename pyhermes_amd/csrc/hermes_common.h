@@ -1,0 +1,72 @@
+// Shared definitions for the MI355X (gfx950) field-pipeline library.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <cstdio>
+#include <string>
+#include <vector>
+
+struct hermes_fft_cache;
+
+// One library context per device: stream, grow-only workspaces and FFT plans.
+struct hermes_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;  // default stream unless the caller installs one
+    // grow-only scratch buffers owned by the context
+    void* ws[8] = {nullptr};
+    size_t ws_size[8] = {0};
+    hermes_fft_cache* fft = nullptr;
+};
+
+// Workspace slots (each slot is an independent grow-only buffer).
+enum hermes_ws_slot {
+    WS_PROJ_TILE = 0,   // projection: per-tile counters / offsets
+    WS_PROJ_REC = 1,    // projection: bucketed particle records
+    WS_REDUCE = 2,      // reductions: per-block partial sums
+    WS_SPEC = 3,        // window / FFT staging
+    WS_MISC = 4,
+    WS_MC = 5,
+    WS_MLP = 6,
+    WS_CONTRACT = 7,
+};
+
+void hermes_set_error(const char* fmt, ...);
+void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes);
+
+#define HERMES_CHECK_HIP(expr)                                                            \
+    do {                                                                                   \
+        hipError_t _e = (expr);                                                            \
+        if (_e != hipSuccess) {                                                            \
+            hermes_set_error("%s failed: %s (%s:%d)", #expr, hipGetErrorString(_e), __FILE__, \
+                             __LINE__);                                                    \
+            return 1;                                                                      \
+        }                                                                                  \
+    } while (0)
+
+#define HERMES_REQUIRE(cond, ...)          \
+    do {                                   \
+        if (!(cond)) {                     \
+            hermes_set_error(__VA_ARGS__); \
+            return 2;                      \
+        }                                  \
+    } while (0)
+
+#define HERMES_LAUNCH_CHECK()                                                          \
+    do {                                                                               \
+        hipError_t _e = hipGetLastError();                                             \
+        if (_e != hipSuccess) {                                                        \
+            hermes_set_error("kernel launch failed: %s (%s:%d)", hipGetErrorString(_e), \
+                             __FILE__, __LINE__);                                      \
+            return 1;                                                                  \
+        }                                                                              \
+    } while (0)
+
+static inline unsigned int hermes_blocks(int64_t n, int threads, int64_t cap = 1 << 20) {
+    int64_t b = (n + threads - 1) / threads;
+    if (b < 1) b = 1;
+    if (b > cap) b = cap;
+    return (unsigned int)b;
+}
