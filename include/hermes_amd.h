@@ -51,6 +51,94 @@ int hermes_project(hermes_ctx *ctx, const double *pos, const double *w, int64_t 
                    const double *phi, int phi_len, int phi_res, double phi_abs_max, float *eps32,
                    double *eps64, int32_t *cell_out, int32_t *frac_out, int32_t *cell_counts);
 
+
+/* ---- window descriptors (window_functions.py:29-129, :426-438;
+ *      radial_multipole_windows.py:128-144 thin-shell profile).
+ * Length arguments a0, a1 are in grid units (len_args * L / box_size). */
+enum {
+    HERMES_WIN_SPHERE = 0,            /* a0 = R */
+    HERMES_WIN_GAUSSIAN = 1,          /* a0 = R */
+    HERMES_WIN_SHELL = 2,             /* a0 = R */
+    HERMES_WIN_THICK_SHELL = 3,       /* a0 = R, a1 = delta_R */
+    HERMES_WIN_GAUSSIAN_SHELL = 4,    /* a0 = R_shell, a1 = R_smooth */
+    HERMES_WIN_INVERSE_LAPLACIAN = 5, /* no arguments; zero mode -> 0 */
+    HERMES_WIN_SHELL_MULTIPOLE = 6    /* a0 = R: j_l(2 pi |k| R) Y_l^m(k-hat) */
+};
+
+typedef struct {
+    int32_t type, l, m, pad;
+    double a0, a1;
+} hermes_window_desc;
+
+/* Real rFFT-layout kernel (L, L, L/2+1) of one isotropic window, summed over the
+ * 8 aliases with |phi_hat|^2 weights P (L+1 values).  Replaces
+ * calculate_real_window_octant_array_numba + fold_octant_window_to_rfft_kernel,
+ * convolution.py:14-38, :263-281 (bandwidth 1), with the zero-mode rule of
+ * window.py:266-272.  dbl selects FP64 (faithful) or FP32 output. */
+int hermes_window_build_rfft(hermes_ctx *ctx, int J, const double *P, const hermes_window_desc *desc, int dbl,
+                             void *out);
+
+/* Complex full-grid kernel (L, L, L) of one window (any type; multipole shells
+ * use the 8 signed aliases {x-L, x}).  Replaces
+ * calculate_radial_multipole_window_array_numba, radial_multipole_windows.py:147-183. */
+int hermes_window_build_full(hermes_ctx *ctx, int J, const double *P, const hermes_window_desc *desc, int dbl,
+                             void *out);
+
+/* In-place spectrum *= kernel * scale.  half != 0: spectrum and kernel are
+ * (L, L, L/2+1); otherwise (L, L, L).  kernel_complex selects a complex kernel.
+ * Together with hermes_fft_* this replaces specialized_convolution_3d[_complex],
+ * convolution.py:284-295. */
+int hermes_spectrum_mul(hermes_ctx *ctx, int J, int half, int dbl, void *spec, const void *kernel,
+                        int kernel_complex, double scale);
+
+/* Fused multi-window pair product (Parseval form of pair_product_with_window,
+ * corr2pcf.py:366-372): for every descriptor b,
+ *   out[b] = sum_{k, full grid} K_b(k) Re(F1(k) conj F2(k))
+ * from the half spectra F1, F2 (F2 may equal F1) in one pass over the spectrum.
+ * The caller divides by N^2.  dbl: spectra are complex128 (FP64 evaluation)
+ * or complex64 (FP32 evaluation); accumulation is FP64 and deterministic. */
+int hermes_pair_reduce(hermes_ctx *ctx, int J, int dbl, const void *F1, const void *F2, const double *P,
+                       const hermes_window_desc *descs, int n_desc, double *out_host);
+
+/* Fused multipole window stage of the batched SH-3PCF path (FP32): from the half
+ * spectrum H (L, L, L/2+1) complex64 of a real field, writes for every shell
+ * radius R_s (grid units) the full spectrum
+ *   out[s] = H_full(k) * K_{R_s,l,m}(k) * scale,  out (n_shell, L, L, L) complex64,
+ * with K as in radial_multipole_windows.py:147-183.  One window evaluation serves
+ * the 8 octant-mirror modes.  Replaces the per-(l,m) window builds of
+ * corr3pcf_multipoles.py:267-321 for all shells at once. */
+int hermes_multipole_apply_f32(hermes_ctx *ctx, int J, const void *H, const double *P, const double *radii,
+                               int n_shell, int l, int m, double scale, void *out);
+
+/* Deterministic FP64 sum of n real (ncomp = 1) or complex (ncomp = 2) values,
+ * stored as float (dbl = 0) or double (dbl = 1); result to host.  Replaces the
+ * np.sum in SFCField._derived_field_integral, sfc_field.py:312-324. */
+int hermes_reduce_sum(hermes_ctx *ctx, const void *x, int64_t n, int ncomp, int dbl, double *out_host);
+
+/* ---- SH-3PCF contraction ----
+ * Plain FP64 path: replaces compute_3d_result_gpu + reduce_complex_sum_kernel
+ * (corr3pcf_multipoles.py:33-58, :99-131) as driven by compute_multipole_m_summand
+ * (:393-457).  d1 (L^3) f64, F1/F2 (L^3) complex128, gamma3x3_host (host, row major)
+ * -> out_host[2] = raw complex sum (re, im), without the 4*pi/N factor. */
+int hermes_multipole_summand_f64(hermes_ctx *ctx, int J, const double *d1, const void *F1, const void *F2,
+                                 const double *gamma3x3_host, double *out_host);
+
+/* Batched FP32 path (MFMA): for shells s1 < n_u, s2 < n_v (<= 16 each),
+ *   out_host[s1 * n_v + s2] = sum_x d1[x] sum_{l1,l2} G3(l1,l2) Re(U_s1[x-l1] * W_s2[x-l2])
+ * with W = V (v_conj = 0) or conj(V) (v_conj = 1); V = NULL means V = U and
+ * W = conj(U).  Gamma = Q diag(lam) Q^T is given by its eigen-decomposition
+ * (Q3x3_host row = tap, column = eigenvector; lam3_host).  U, V are
+ * (n, L, L, L) complex64, d1 (L^3) float.  FP32 products, FP64 accumulation.
+ * Replaces the per-(r12, r13) loop over compute_multipole_m_summand for all
+ * shell pairs of one (l, m) at once. */
+int hermes_multipole_gram_f32(hermes_ctx *ctx, int J, const float *d1, const void *U, int n_u, const void *V,
+                              int n_v, int v_conj, const double *Q3x3_host, const double *lam3_host,
+                              double *out_host);
+
+/* Checks the lane layout of the 16x16x4 FP32 matrix instruction used above;
+ * max_abs_err = 0 on a conforming device.  (No counterpart in the reference.) */
+int hermes_mfma_selftest(hermes_ctx *ctx, double *max_abs_err);
+
 #ifdef __cplusplus
 }
 #endif

@@ -31,6 +31,29 @@ _SIGNATURES = {
     "hermes_fft_c2r": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p],
     "hermes_fft_c2c": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
 }
+
+
+class WindowDesc(ctypes.Structure):
+    """hermes_window_desc (include/hermes_amd.h)."""
+    _fields_ = [("type", ctypes.c_int32), ("l", ctypes.c_int32), ("m", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("a0", ctypes.c_double), ("a1", ctypes.c_double)]
+
+
+_dpp = ctypes.POINTER(ctypes.c_double)
+_SIGNATURES.update({
+    "hermes_window_build_rfft": [c_void_p, c_int, c_void_p, ctypes.POINTER(WindowDesc), c_int, c_void_p],
+    "hermes_window_build_full": [c_void_p, c_int, c_void_p, ctypes.POINTER(WindowDesc), c_int, c_void_p],
+    "hermes_spectrum_mul": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_double],
+    "hermes_pair_reduce": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(WindowDesc), c_int,
+                           _dpp],
+    "hermes_multipole_apply_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_int, c_double,
+                                   c_void_p],
+    "hermes_reduce_sum": [c_void_p, c_void_p, c_i64, c_int, c_int, _dpp],
+    "hermes_multipole_summand_f64": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, _dpp, _dpp],
+    "hermes_multipole_gram_f32": [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, _dpp, _dpp,
+                                  _dpp],
+    "hermes_mfma_selftest": [c_void_p, _dpp],
+})
 _RESTYPES = {"hermes_last_error": ctypes.c_char_p}
 
 

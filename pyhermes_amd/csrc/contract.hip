@@ -1,0 +1,326 @@
+// Spherical-harmonic 3PCF triple-product contraction.
+//
+// Plain FP64 path (faithful to compute_3d_result_gpu + reduce_complex_sum_kernel,
+// corr3pcf_multipoles.py:33-58, :99-131):
+//   S = sum_x d1[x] sum_{l1} F1[x-l1] sum_{l2} G3(l1,l2) F2[x-l2]   (complex128)
+// fused with a deterministic FP64 reduction.
+//
+// Batched FP32 path (MFMA): with Gamma = Q diag(lam) Q^T,
+//   sum_{l1,l2} G3(l1,l2) U[x-l1] V[x-l2] = sum_{abc} lam_abc U~_abc[x] V~_abc[x],
+// U~_abc = separable 3-tap filter of U with Q's columns.  For 16 shells at once
+//   G[s1][s2] = sum_x d1[x] sum_{abc,comp} lam_abc U~_{s1,abc,comp}[x] V~_{s2,abc,comp}[x]
+// is a 16 x 16 Gram matrix accumulated with v_mfma_f32_16x16x4f32 (lane = shell
+// x cell), flushed to FP64 after every x plane; per-workgroup FP64 partials are
+// summed in fixed order.  comp runs over (re, im) so G = Re(U V) sums when V is
+// given as (V.re, -V.im).
+#include "../../include/hermes_amd.h"
+#include "hermes_common.h"
+
+namespace {
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------- plain FP64
+constexpr int SUM_THREADS = 256;
+
+__global__ __launch_bounds__(SUM_THREADS) void k_summand_f64(int L, const double* __restrict__ d1,
+                                                             const double2* __restrict__ F1,
+                                                             const double2* __restrict__ F2, const double* __restrict__ G,
+                                                             double* __restrict__ partial) {
+    __shared__ double sh[2][SUM_THREADS / 64];
+    const int64_t N = (int64_t)L * L * L;
+    double tr = 0.0, ti = 0.0;
+    for (int64_t idx = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; idx < N; idx += (int64_t)gridDim.x * blockDim.x) {
+        const int lz = (int)(idx % L), ly = (int)((idx / L) % L), lx = (int)(idx / ((int64_t)L * L));
+        double s1r = 0.0, s1i = 0.0;
+        for (int l1x = 0; l1x < 3; ++l1x) {
+            const int i1x = (lx - l1x + L) % L;
+            for (int l1y = 0; l1y < 3; ++l1y) {
+                const int i1y = (ly - l1y + L) % L;
+                for (int l1z = 0; l1z < 3; ++l1z) {
+                    const int i1z = (lz - l1z + L) % L;
+                    double s2r = 0.0, s2i = 0.0;
+                    for (int l2x = 0; l2x < 3; ++l2x) {
+                        const int i2x = (lx - l2x + L) % L;
+                        double ryr = 0.0, ryi = 0.0;
+                        for (int l2y = 0; l2y < 3; ++l2y) {
+                            const int i2y = (ly - l2y + L) % L;
+                            double rzr = 0.0, rzi = 0.0;
+                            for (int l2z = 0; l2z < 3; ++l2z) {
+                                const int i2z = (lz - l2z + L) % L;
+                                const double2 f = F2[((int64_t)i2x * L + i2y) * L + i2z];
+                                const double g = G[l1z * 3 + l2z];
+                                rzr += g * f.x;
+                                rzi += g * f.y;
+                            }
+                            const double g = G[l1y * 3 + l2y];
+                            ryr += g * rzr;
+                            ryi += g * rzi;
+                        }
+                        const double g = G[l1x * 3 + l2x];
+                        s2r += g * ryr;
+                        s2i += g * ryi;
+                    }
+                    const double2 a = F1[((int64_t)i1x * L + i1y) * L + i1z];
+                    s1r += a.x * s2r - a.y * s2i;
+                    s1i += a.x * s2i + a.y * s2r;
+                }
+            }
+        }
+        const double d = d1[idx];
+        tr += d * s1r;
+        ti += d * s1i;
+    }
+    tr = wave_sum_d(tr);
+    ti = wave_sum_d(ti);
+    if (threadIdx.x % 64 == 0) {
+        sh[0][threadIdx.x / 64] = tr;
+        sh[1][threadIdx.x / 64] = ti;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int w = 0; w < SUM_THREADS / 64; ++w) t += sh[threadIdx.x][w];
+        partial[2 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+__global__ void k_sum_pairs(const double* __restrict__ partial, int nb, int width, double* __restrict__ out) {
+    for (int j = threadIdx.x; j < width; j += blockDim.x) {
+        double t = 0.0;
+        for (int i = 0; i < nb; ++i) t += partial[(int64_t)i * width + j];
+        out[j] = t;
+    }
+}
+
+// ---------------------------------------------------------------- batched FP32 MFMA Gram
+struct GramArgs {
+    int L, S1, S2, cross, xseg;
+    float vsign;      // sign applied to V's imaginary part in the cross case (-1: Re(U V))
+    float Q[9];       // Q[tap * 3 + col]
+    float lam[27];    // lam_a lam_b lam_c, index (a * 3 + b) * 3 + c
+};
+
+constexpr int GRAM_WAVES = 4;
+
+// yz-filtered values Y[b][c] (complex) of field f at plane x for cell (y, z).
+__device__ __forceinline__ void yz_filter(const float2* __restrict__ f, int L, int x, int y, int z, const float* Q,
+                                          float2 Y[9]) {
+    const int m = L - 1;
+    float2 Z[3][3];  // Z[ly][c]
+#pragma unroll
+    for (int ly = 0; ly < 3; ++ly) {
+        const int yy = (y - ly) & m;
+        const float2* row = f + ((int64_t)x * L + yy) * L;
+        const float2 u0 = row[z & m], u1 = row[(z - 1) & m], u2 = row[(z - 2) & m];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const float q0 = Q[0 * 3 + c], q1 = Q[1 * 3 + c], q2 = Q[2 * 3 + c];
+            Z[ly][c] = make_float2(q0 * u0.x + q1 * u1.x + q2 * u2.x, q0 * u0.y + q1 * u1.y + q2 * u2.y);
+        }
+    }
+#pragma unroll
+    for (int b = 0; b < 3; ++b) {
+        const float q0 = Q[0 * 3 + b], q1 = Q[1 * 3 + b], q2 = Q[2 * 3 + b];
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            Y[b * 3 + c] = make_float2(q0 * Z[0][c].x + q1 * Z[1][c].x + q2 * Z[2][c].x,
+                                       q0 * Z[0][c].y + q1 * Z[1][c].y + q2 * Z[2][c].y);
+    }
+}
+
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+__global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __restrict__ d1,
+                                                              const float2* __restrict__ U,
+                                                              const float2* __restrict__ V, GramArgs g,
+                                                              double* __restrict__ partial) {
+    __shared__ double red[GRAM_WAVES][256];
+    const int L = g.L;
+    const int lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+    const int s = lane % 16, k = lane / 16;
+    const int nzg = L / 4, nyg = L / GRAM_WAVES;
+    const int zg = blockIdx.x % nzg;
+    const int yg = (blockIdx.x / nzg) % nyg;
+    const int xs = blockIdx.x / (nzg * nyg);
+    const int y = yg * GRAM_WAVES + wave;
+    const int z = zg * 4 + k;
+    const int x0 = xs * g.xseg;
+    const int64_t N = (int64_t)L * L * L;
+    const bool ua = s < g.S1;
+    const bool vb = s < g.S2;
+    const float2* fu = U + (int64_t)(ua ? s : 0) * N;
+    const float2* fv = (g.cross ? V : U) + (int64_t)(vb ? s : 0) * N;
+    float Q[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) Q[i] = g.Q[i];
+
+    float2 Yu[3][9], Yv[3][9];  // sliding window over x: [0] = x, [1] = x-1, [2] = x-2
+#pragma unroll
+    for (int w = 1; w <= 2; ++w) {
+        const int xx = (x0 - w) & (L - 1);
+        if (ua) yz_filter(fu, L, xx, y, z, Q, Yu[w]);
+        else
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Yu[w][i] = make_float2(0.f, 0.f);
+        if (g.cross) {
+            if (vb) yz_filter(fv, L, xx, y, z, Q, Yv[w]);
+            else
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Yv[w][i] = make_float2(0.f, 0.f);
+        }
+    }
+    double acc64[4] = {0.0, 0.0, 0.0, 0.0};
+    for (int xi = 0; xi < g.xseg; ++xi) {
+        const int x = x0 + xi;
+        if (ua) yz_filter(fu, L, x, y, z, Q, Yu[0]);
+        else
+#pragma unroll
+            for (int i = 0; i < 9; ++i) Yu[0][i] = make_float2(0.f, 0.f);
+        if (g.cross) {
+            if (vb) yz_filter(fv, L, x, y, z, Q, Yv[0]);
+            else
+#pragma unroll
+                for (int i = 0; i < 9; ++i) Yv[0][i] = make_float2(0.f, 0.f);
+        }
+        const float dval = d1[((int64_t)x * L + y) * L + z];
+        float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int a = 0; a < 3; ++a) {
+            const float q0 = Q[0 * 3 + a], q1 = Q[1 * 3 + a], q2 = Q[2 * 3 + a];
+#pragma unroll
+            for (int bc = 0; bc < 9; ++bc) {
+                const float2 ut = make_float2(q0 * Yu[0][bc].x + q1 * Yu[1][bc].x + q2 * Yu[2][bc].x,
+                                              q0 * Yu[0][bc].y + q1 * Yu[1][bc].y + q2 * Yu[2][bc].y);
+                float2 vt;
+                if (g.cross)
+                    vt = make_float2(q0 * Yv[0][bc].x + q1 * Yv[1][bc].x + q2 * Yv[2][bc].x,
+                                     g.vsign * (q0 * Yv[0][bc].y + q1 * Yv[1][bc].y + q2 * Yv[2][bc].y));
+                else
+                    vt = ut;
+                const float dl = dval * g.lam[a * 9 + bc];
+                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(dl * ut.x, vt.x, accr, 0, 0, 0);
+                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(dl * ut.y, vt.y, acci, 0, 0, 0);
+            }
+        }
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc64[j] += (double)accr[j] + (double)acci[j];
+#pragma unroll
+        for (int i = 0; i < 9; ++i) {
+            Yu[2][i] = Yu[1][i];
+            Yu[1][i] = Yu[0][i];
+            if (g.cross) {
+                Yv[2][i] = Yv[1][i];
+                Yv[1][i] = Yv[0][i];
+            }
+        }
+    }
+    // lane holds G[4*(lane/16) + j][lane % 16]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red[wave][(4 * (lane / 16) + j) * 16 + (lane % 16)] = acc64[j];
+    __syncthreads();
+    for (int i = threadIdx.x; i < 256; i += blockDim.x) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < GRAM_WAVES; ++w) t += red[w][i];
+        partial[(int64_t)blockIdx.x * 256 + i] = t;
+    }
+}
+
+// Lane-layout self check of the 16x16x4 FP32 MFMA: C = A B for A[m][k] = m + 16 k,
+// B[k][n] = (k + 1) * (n - 7).
+__global__ void k_mfma_selftest(float* out) {
+    const int lane = threadIdx.x;
+    const int m = lane % 16, kk = lane / 16;
+    float a = (float)(m + 16 * kk), b = (float)((kk + 1) * (lane % 16 - 7));
+    float4_t acc = {0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, acc, 0, 0, 0);
+    for (int j = 0; j < 4; ++j) out[(4 * (lane / 16) + j) * 16 + (lane % 16)] = acc[j];
+}
+
+}  // namespace
+
+extern "C" {
+
+int hermes_multipole_summand_f64(hermes_ctx* ctx, int J, const double* d1, const void* F1, const void* F2,
+                                 const double* gamma3x3_host, double* out_host) {
+    HERMES_REQUIRE(ctx && d1 && F1 && F2 && gamma3x3_host && out_host, "hermes_multipole_summand_f64: null argument");
+    HERMES_REQUIRE(J >= 2 && J <= 10, "J=%d unsupported", J);
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int L = 1 << J;
+    const int64_t N = (int64_t)L * L * L;
+    const int nb = (int)hermes_blocks(N, SUM_THREADS, 4096);
+    double* ws = (double*)hermes_workspace(ctx, WS_CONTRACT, sizeof(double) * (2 * (size_t)nb + 2 + 9));
+    HERMES_REQUIRE(ws, "%s", hermes_last_error());
+    double* dG = ws + 2 * nb + 2;
+    HERMES_CHECK_HIP(hipMemcpyAsync(dG, gamma3x3_host, sizeof(double) * 9, hipMemcpyHostToDevice, ctx->stream));
+    k_summand_f64<<<nb, SUM_THREADS, 0, ctx->stream>>>(L, d1, (const double2*)F1, (const double2*)F2, dG, ws);
+    HERMES_LAUNCH_CHECK();
+    k_sum_pairs<<<1, 64, 0, ctx->stream>>>(ws, nb, 2, ws + 2 * nb);
+    HERMES_LAUNCH_CHECK();
+    HERMES_CHECK_HIP(hipMemcpyAsync(out_host, ws + 2 * nb, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u, const void* V, int n_v,
+                              int v_conj, const double* Q3x3_host, const double* lam3_host, double* out_host) {
+    HERMES_REQUIRE(ctx && d1 && U && Q3x3_host && lam3_host && out_host, "hermes_multipole_gram_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_gram_f32: J=%d unsupported (needs L >= 8)", J);
+    HERMES_REQUIRE(n_u >= 1 && n_u <= 16, "n_u=%d outside 1..16", n_u);
+    HERMES_REQUIRE(V == nullptr || (n_v >= 1 && n_v <= 16), "n_v=%d outside 1..16", n_v);
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int L = 1 << J;
+    GramArgs g;
+    g.L = L;
+    g.S1 = n_u;
+    g.cross = V != nullptr;
+    g.S2 = g.cross ? n_v : n_u;
+    g.vsign = v_conj ? 1.0f : -1.0f;  // Re(U V) needs -V.im, Re(U conj V) needs +V.im
+    g.xseg = L < 64 ? L : 64;
+    for (int i = 0; i < 9; ++i) g.Q[i] = (float)Q3x3_host[i];
+    for (int a = 0; a < 3; ++a)
+        for (int b = 0; b < 3; ++b)
+            for (int c = 0; c < 3; ++c) g.lam[(a * 3 + b) * 3 + c] = (float)(lam3_host[a] * lam3_host[b] * lam3_host[c]);
+    const int nblocks = (L / 4) * (L / GRAM_WAVES) * (L / g.xseg);
+    double* ws = (double*)hermes_workspace(ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256));
+    HERMES_REQUIRE(ws, "%s", hermes_last_error());
+    k_gram_f32<<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
+    HERMES_LAUNCH_CHECK();
+    k_sum_pairs<<<1, 256, 0, ctx->stream>>>(ws, nblocks, 256, ws + (size_t)nblocks * 256);
+    HERMES_LAUNCH_CHECK();
+    double h[256];
+    HERMES_CHECK_HIP(hipMemcpyAsync(h, ws + (size_t)nblocks * 256, sizeof(double) * 256, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    for (int i = 0; i < g.S1; ++i)
+        for (int j = 0; j < g.S2; ++j) out_host[i * g.S2 + j] = h[i * 16 + j];
+    return 0;
+}
+
+int hermes_mfma_selftest(hermes_ctx* ctx, double* max_abs_err) {
+    HERMES_REQUIRE(ctx && max_abs_err, "null argument");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    float* d = (float*)hermes_workspace(ctx, WS_MISC, sizeof(float) * 256);
+    HERMES_REQUIRE(d, "%s", hermes_last_error());
+    k_mfma_selftest<<<1, 64, 0, ctx->stream>>>(d);
+    HERMES_LAUNCH_CHECK();
+    float h[256];
+    HERMES_CHECK_HIP(hipMemcpyAsync(h, d, sizeof(h), hipMemcpyDeviceToHost, ctx->stream));
+    HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    double err = 0.0;
+    for (int m = 0; m < 16; ++m)
+        for (int n = 0; n < 16; ++n) {
+            double ref = 0.0;
+            for (int k = 0; k < 4; ++k) ref += (double)(m + 16 * k) * (double)((k + 1) * (n - 7));
+            double e = fabs(ref - h[m * 16 + n]);
+            if (e > err) err = e;
+        }
+    *max_abs_err = err;
+    return 0;
+}
+
+}  // extern "C"

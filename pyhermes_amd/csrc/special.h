@@ -1,0 +1,250 @@
+// Device special functions and window transfer functions.
+//
+// *_f64: faithful FP64 restatements of the reference algorithms
+//   spherical_jn_numba      special_functions.py:49-128
+//   assoc_legendre_numba    special_functions.py:273-304
+//   spherical_harmonic_numba special_functions.py:307-334
+//   window transfer kernels window_functions.py:29-129, :426-438
+// *_f32: the FP32 fast path used by the batched kernels (no acos/atan2: Y_lm is
+//   evaluated from ((kx + i ky)/|k|)^m and the Legendre polynomial part, j_l from
+//   an exact revolution count and upward recurrence / power series).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "../../include/hermes_amd.h"
+
+#define HERMES_PI 3.14159265358979323846
+
+// ------------------------------------------------------------------ FP64 faithful
+__device__ inline double jl_f64(int l, double x) {
+    if (l < 0) return __builtin_nan("");
+    if (x == 0.0) return l == 0 ? 1.0 : 0.0;
+    double ax = fabs(x);
+    if (ax <= 5.0e-2) {
+        if (l == 0) {
+            double x2 = x * x;
+            return 1.0 - x2 / 6.0 + x2 * x2 / 120.0;
+        }
+        double denom = 1.0;
+        for (int n = 1; n <= l; ++n) denom *= 2.0 * n + 1.0;
+        double x2 = x * x;
+        return (pow(x, (double)l) / denom) *
+               (1.0 - x2 / (2.0 * (2.0 * l + 3.0)) + x2 * x2 / (8.0 * (2.0 * l + 3.0) * (2.0 * l + 5.0)));
+    }
+    double sx, cx;
+    sincos(x, &sx, &cx);
+    double j0 = sx / x;
+    if (l == 0) return j0;
+    double j1 = sx / (x * x) - cx / x;
+    if (l == 1) return j1;
+    if (ax < l + 1.0) {
+        int n_stop = l + 40;
+        double jp1 = 0.0, jcur = 1.0, jt = 0.0, jd0 = 0.0, jd1 = 0.0;
+        for (int n = n_stop; n > 0; --n) {
+            double jm1 = ((2.0 * n + 1.0) / x) * jcur - jp1;
+            if (n - 1 == l) jt = jm1;
+            if (n - 1 == 1) jd1 = jm1;
+            if (n - 1 == 0) jd0 = jm1;
+            jp1 = jcur;
+            jcur = jm1;
+            if (fabs(jcur) > 1.0e200) {
+                jp1 *= 1.0e-200; jcur *= 1.0e-200; jt *= 1.0e-200; jd0 *= 1.0e-200; jd1 *= 1.0e-200;
+            } else if (fabs(jcur) < 1.0e-200 && jcur != 0.0) {
+                jp1 *= 1.0e200; jcur *= 1.0e200; jt *= 1.0e200; jd0 *= 1.0e200; jd1 *= 1.0e200;
+            }
+        }
+        if (fabs(jd0) >= fabs(jd1) || jd1 == 0.0) return jt * (j0 / jd0);
+        return jt * (j1 / jd1);
+    }
+    double jm1 = j0, jcur = j1;
+    for (int ell = 1; ell < l; ++ell) {
+        double jn = ((2.0 * ell + 1.0) / x) * jcur - jm1;
+        jm1 = jcur;
+        jcur = jn;
+    }
+    return jcur;
+}
+
+__device__ inline double plm_f64(int l, int m, double x) {
+    double pmm = 1.0;
+    if (m > 0) {
+        double t = 1.0 - x * x;
+        double somx2 = sqrt(t > 0.0 ? t : 0.0);
+        double fact = 1.0;
+        for (int i = 0; i < m; ++i) {
+            pmm *= -fact * somx2;
+            fact += 2.0;
+        }
+    }
+    if (l == m) return pmm;
+    double pmmp1 = x * (2.0 * m + 1.0) * pmm;
+    if (l == m + 1) return pmmp1;
+    double a = pmm, b = pmmp1, pll = 0.0;
+    for (int ell = m + 2; ell <= l; ++ell) {
+        pll = ((2.0 * ell - 1.0) * x * b - (ell + m - 1.0) * a) / (ell - m);
+        a = b;
+        b = pll;
+    }
+    return pll;
+}
+
+__device__ inline double factorial_small_f64(int n) {
+    double out = 1.0;
+    for (int i = 2; i <= n; ++i) out *= i;
+    return out;
+}
+
+__device__ inline double2 ylm_f64(int l, int m, double ki, double kj, double kk) {
+    if (l < 0) return make_double2(__builtin_nan(""), 0.0);
+    int am = m < 0 ? -m : m;
+    if (am > l) return make_double2(0.0, 0.0);
+    double k = sqrt(ki * ki + kj * kj + kk * kk);
+    if (k == 0.0) {
+        if (l == 0 && m == 0) return make_double2(1.0 / sqrt(4.0 * HERMES_PI), 0.0);
+        return make_double2(0.0, 0.0);
+    }
+    double mu = kk / k;
+    if (mu > 1.0) mu = 1.0;
+    else if (mu < -1.0) mu = -1.0;
+    double theta = acos(mu);
+    double ph = atan2(kj, ki);
+    double x = cos(theta);
+    double P = plm_f64(l, am, x);
+    double norm = sqrt((2.0 * l + 1.0) / (4.0 * HERMES_PI) * factorial_small_f64(l - am) / factorial_small_f64(l + am));
+    double s = norm * P;
+    double re = s * cos(am * ph), im = s * sin(am * ph);
+    if (m >= 0) return make_double2(re, im);
+    double sign = (am % 2 == 1) ? -1.0 : 1.0;
+    return make_double2(sign * re, -sign * im);
+}
+
+__device__ inline double win_sphere_f64(double k, double R) {
+    double q = 2 * HERMES_PI * k * R;
+    if (q == 0.0) return 1.0;
+    double s, c;
+    sincos(q, &s, &c);
+    return 3 * (s - q * c) / (q * q * q);
+}
+
+// Isotropic transfer function of one window at |k| (cycles per grid cell).
+__device__ inline double iso_window_f64(int type, double ki, double kj, double kk, double a0, double a1) {
+    switch (type) {
+    case HERMES_WIN_SPHERE: return win_sphere_f64(sqrt(ki * ki + kj * kj + kk * kk), a0);
+    case HERMES_WIN_GAUSSIAN: {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        double q = 2 * HERMES_PI * k * a0;
+        return exp(-(q * q) / 2);
+    }
+    case HERMES_WIN_SHELL: {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        double q = 2 * HERMES_PI * k * a0;
+        if (q == 0.0) return 1.0;
+        return sin(q) / q;
+    }
+    case HERMES_WIN_THICK_SHELL: {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        double ri = a0 - 0.5 * a1;
+        if (ri < 0.0) ri = 0.0;
+        double ro = a0 + 0.5 * a1;
+        double den = ro * ro * ro - ri * ri * ri;
+        if (den == 0.0) return 1.0;
+        return (ro * ro * ro * win_sphere_f64(k, ro) - ri * ri * ri * win_sphere_f64(k, ri)) / den;
+    }
+    case HERMES_WIN_GAUSSIAN_SHELL: {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        if (k == 0) return 1.0;
+        double qs = 2 * HERMES_PI * k * a0;
+        double qm = 2 * HERMES_PI * k * a1;
+        double den = a0 * a0 + a1 * a1;
+        if (den == 0.0) return 1.0;
+        double sinc = (qs == 0.0) ? 1.0 : sin(qs) / qs;
+        return (a1 * a1 * cos(qs) + a0 * a0 * sinc) / den * exp(-(qm * qm) / 2);
+    }
+    case HERMES_WIN_INVERSE_LAPLACIAN: {
+        double k2 = ki * ki + kj * kj + kk * kk;
+        if (k2 == 0.0) return 0.0;
+        return -1.0 / (((2.0 * HERMES_PI) * (2.0 * HERMES_PI)) * k2);
+    }
+    }
+    return __builtin_nan("");
+}
+
+// Complex transfer function of a window at signed k (isotropic windows are real).
+__device__ inline double2 window_value_f64(const hermes_window_desc& d, double ki, double kj, double kk) {
+    if (d.type == HERMES_WIN_SHELL_MULTIPOLE) {
+        double k = sqrt(ki * ki + kj * kj + kk * kk);
+        double radial = jl_f64(d.l, 2.0 * HERMES_PI * k * d.a0);
+        double2 y = ylm_f64(d.l, d.m, ki, kj, kk);
+        return make_double2(radial * y.x, radial * y.y);
+    }
+    return make_double2(iso_window_f64(d.type, ki, kj, kk, d.a0, d.a1), 0.0);
+}
+
+// ------------------------------------------------------------------ FP32 fast path
+// j_l(2 pi t), FP32 fast path, from the exact revolution count t (FP64):
+// l = 0 closed form; q >= l + 1 upward recurrence from j0, j1 (stable there,
+// the same split as the reference); otherwise the power series
+// j_l(q) = q^l/(2l+1)!! sum_k (-q^2/2)^k / (k! prod_{i=1..k}(2l+2i+1)).
+__device__ inline float jl_f32(int l, double t) {
+    if (t == 0.0) return l == 0 ? 1.0f : 0.0f;
+    float q = (float)(2.0 * HERMES_PI * t);
+    if (l == 0 || q >= (float)l + 1.0f) {
+        double fr = t - floor(t);
+        float s, c;
+        sincospif((float)(2.0 * fr), &s, &c);
+        float iq = 1.0f / q;
+        float j0 = s * iq;
+        if (l == 0) return j0;
+        float j1 = (j0 - c) * iq;
+        float jm = j0, jc = j1;
+        for (int n = 1; n < l; ++n) {
+            float jn = (float)(2 * n + 1) * iq * jc - jm;
+            jm = jc;
+            jc = jn;
+        }
+        return jc;
+    }
+    float h = -0.5f * q * q;
+    float ql = 1.0f, dfact = 1.0f;
+    for (int n = 1; n <= l; ++n) {
+        ql *= q;
+        dfact *= (float)(2 * n + 1);
+    }
+    float term = 1.0f, sum = 1.0f;
+    for (int k = 1; k < 80; ++k) {
+        term *= h / (float)(k * (2 * l + 2 * k + 1));
+        sum += term;
+        if (fabsf(term) < 1e-9f * fabsf(sum)) break;
+    }
+    return ql / dfact * sum;
+}
+
+// Y_l^m(a-hat) for integer lattice vector a (as floats), m >= 0; cnorm = N_lm (-1)^m (2m-1)!!.
+__device__ inline float2 ylm_f32(int l, int m, float ax, float ay, float az, float cnorm) {
+    float r2 = ax * ax + ay * ay + az * az;
+    if (r2 == 0.0f) return make_float2((l == 0 && m == 0) ? 0.28209479177387814f : 0.0f, 0.0f);
+    float ir = rsqrtf(r2);
+    float x = az * ir;
+    float ur = ax * ir, ui = ay * ir;  // sin(theta) e^{i phi}
+    float pr = 1.0f, pi = 0.0f;
+    for (int i = 0; i < m; ++i) {
+        float t = pr * ur - pi * ui;
+        pi = pr * ui + pi * ur;
+        pr = t;
+    }
+    // polynomial part of P_l^m(x) / ((-1)^m (2m-1)!! (1-x^2)^{m/2})
+    float pmm = 1.0f, poly;
+    if (l == m) {
+        poly = pmm;
+    } else {
+        float a = pmm, b = x * (float)(2 * m + 1);
+        for (int ell = m + 2; ell <= l; ++ell) {
+            float c = ((float)(2 * ell - 1) * x * b - (float)(ell + m - 1) * a) / (float)(ell - m);
+            a = b;
+            b = c;
+        }
+        poly = b;
+    }
+    float s = cnorm * poly;
+    return make_float2(s * pr, s * pi);
+}

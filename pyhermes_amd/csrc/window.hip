@@ -1,0 +1,483 @@
+// Window stage: kernel builders, spectral multiply, fused pair reduction and the
+// fused multipole window apply of the batched SH-3PCF path.
+// P (|phi_hat|^2 samples, L+1 doubles) is a DEVICE pointer; window descriptors
+// and shell radii are HOST arrays copied to the context workspace.
+#include "../../include/hermes_amd.h"
+#include "hermes_common.h"
+#include "special.h"
+
+namespace {
+
+constexpr int RED_THREADS = 256;
+
+__device__ __forceinline__ double wave_sum(double v) {
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// ---------------------------------------------------------------- kernel builders
+// convolution.py:14-38 + :263-281: K[x,y,z] = sum over the 8 mirrors {x, L-x} x ... of
+// P[i]P[j]P[k] W(i/L, j/L, k/L), in the fold's summation order.
+template <typename T>
+__global__ void k_build_rfft(int L, const double* __restrict__ P, hermes_window_desc d, T* __restrict__ out) {
+    const int Lh = L / 2 + 1;
+    const int64_t n = (int64_t)L * L * Lh;
+    const double inv_L = 1.0 / L;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int z = (int)(i % Lh);
+        int y = (int)((i / Lh) % L);
+        int x = (int)(i / ((int64_t)Lh * L));
+        int xs[2] = {x, L - x}, ys[2] = {y, L - y}, zs[2] = {z, L - z};
+        // fold order: (0,0,0) (1,0,0) (0,1,0) (0,0,1) (1,1,0) (1,0,1) (0,1,1) (1,1,1)
+        const int ord[8][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {1, 1, 0}, {1, 0, 1}, {0, 1, 1}, {1, 1, 1}};
+        double t = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            int ii = xs[ord[a][0]], jj = ys[ord[a][1]], kk = zs[ord[a][2]];
+            double wa = 0.0;
+            wa += P[ii] * P[jj] * P[kk] * iso_window_f64(d.type, ii * inv_L, jj * inv_L, kk * inv_L, d.a0, d.a1);
+            t = (a == 0) ? wa : t + wa;
+        }
+        if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && i == 0) t = 0.0;
+        out[i] = (T)t;
+    }
+}
+
+// radial_multipole_windows.py:147-183 order: aliases (x-L | x) per axis, "0" first.
+template <typename T>
+__global__ void k_build_full(int L, const double* __restrict__ P, hermes_window_desc d, T* __restrict__ out) {
+    const int64_t n = (int64_t)L * L * L;
+    const double inv_L = 1.0 / L;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        int z = (int)(i % L);
+        int y = (int)((i / L) % L);
+        int x = (int)(i / ((int64_t)L * L));
+        double pi[2] = {P[L - x], P[x]}, pj[2] = {P[L - y], P[y]}, pk[2] = {P[L - z], P[z]};
+        double ki[2] = {(x - L) * inv_L, x * inv_L}, kj[2] = {(y - L) * inv_L, y * inv_L},
+               kk[2] = {(z - L) * inv_L, z * inv_L};
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            int u = (a >> 2) & 1, v = (a >> 1) & 1, w = a & 1;
+            double p = pi[u] * pj[v] * pk[w];
+            double2 wv = window_value_f64(d, ki[u], kj[v], kk[w]);
+            if (a == 0) {
+                re = p * wv.x;
+                im = p * wv.y;
+            } else {
+                re += p * wv.x;
+                im += p * wv.y;
+            }
+        }
+        if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && i == 0) re = im = 0.0;
+        out[2 * i] = (T)re;
+        out[2 * i + 1] = (T)im;
+    }
+}
+
+template <typename T>
+__global__ void k_spec_mul(int64_t n, T* __restrict__ spec, const T* __restrict__ ker, int kc, T scale) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        T a = spec[2 * i], b = spec[2 * i + 1];
+        if (kc) {
+            T c = ker[2 * i] * scale, e = ker[2 * i + 1] * scale;
+            spec[2 * i] = a * c - b * e;
+            spec[2 * i + 1] = a * e + b * c;
+        } else {
+            T c = ker[i] * scale;
+            spec[2 * i] = a * c;
+            spec[2 * i + 1] = b * c;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- pair reduction
+// Window value at signed lattice alias (ax, ay, az), FP32 fast path for shells and
+// shell multipoles; other window types use the FP64 formula.
+__device__ __forceinline__ float window_value_f32(const hermes_window_desc& d, int ax, int ay, int az, int L,
+                                                  float cnorm) {
+    double n2 = (double)ax * ax + (double)ay * ay + (double)az * az;
+    if (d.type == HERMES_WIN_SHELL) {
+        double t = sqrt(n2) * d.a0 / L;
+        return jl_f32(0, t);
+    }
+    if (d.type == HERMES_WIN_SHELL_MULTIPOLE) {
+        double t = sqrt(n2) * d.a0 / L;
+        float j = jl_f32(d.l, t);
+        float2 y = ylm_f32(d.l, d.m, (float)ax, (float)ay, (float)az, cnorm);
+        return j * y.x;  // m = 0 windows are real
+    }
+    double inv_L = 1.0 / L;
+    return (float)iso_window_f64(d.type, ax * inv_L, ay * inv_L, az * inv_L, d.a0, d.a1);
+}
+
+// One thread per octant representative (x, y, z) in [0, L/2]^3; its kernel value is
+// shared by the (up to) 4 modes (x|L-x, y|L-y, z) of the half spectrum.
+template <typename C, bool FP64>
+__global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __restrict__ F1,
+                                                             const C* __restrict__ F2, const double* __restrict__ P,
+                                                             const hermes_window_desc* __restrict__ descs,
+                                                             const float* __restrict__ cnorms, int nd,
+                                                             double* __restrict__ partial) {
+    extern __shared__ double wacc[];  // [waves][nd]
+    const int Lh = L / 2 + 1;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int nwaves = blockDim.x / 64;
+    for (int i = threadIdx.x; i < nwaves * nd; i += blockDim.x) wacc[i] = 0.0;
+    __syncthreads();
+    const int64_t nrep = (int64_t)Lh * Lh * Lh;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nrep; base += stride) {
+        int64_t r = base + threadIdx.x;
+        bool active = r < nrep;
+        double power = 0.0;
+        int x = 0, y = 0, z = 0;
+        if (active) {
+            z = (int)(r % Lh);
+            y = (int)((r / Lh) % Lh);
+            x = (int)(r / ((int64_t)Lh * Lh));
+            const int xm = (L - x) % L, ym = (L - y) % L;
+            const int nxm = (xm != x) ? 2 : 1, nym = (ym != y) ? 2 : 1;
+            const double wz = (z == 0 || z == L / 2) ? 1.0 : 2.0;
+            for (int a = 0; a < nxm; ++a)
+                for (int b = 0; b < nym; ++b) {
+                    int xx = a ? xm : x, yy = b ? ym : y;
+                    int64_t idx = ((int64_t)xx * L + yy) * Lh + z;
+                    C f1 = F1[idx], f2 = F2[idx];
+                    power += wz * ((double)f1.x * (double)f2.x + (double)f1.y * (double)f2.y);
+                }
+        }
+        for (int b = 0; b < nd; ++b) {
+            double kb = 0.0;
+            if (active) {
+                const hermes_window_desc d = descs[b];
+                for (int a = 0; a < 8; ++a) {
+                    int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+                    int ix = ax < 0 ? -ax : ax, iy = ay < 0 ? -ay : ay, iz = az < 0 ? -az : az;
+                    double p = P[ix] * P[iy] * P[iz];
+                    if (FP64) {
+                        double inv_L = 1.0 / L;
+                        kb += p * window_value_f64(d, ax * inv_L, ay * inv_L, az * inv_L).x;
+                    } else {
+                        kb += (double)((float)p * window_value_f32(d, ax, ay, az, L, cnorms[b]));
+                    }
+                }
+                if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && x == 0 && y == 0 && z == 0) kb = 0.0;
+            }
+            double v = wave_sum(power * kb);
+            if (lane == 0) wacc[wave * nd + b] += v;
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nd; b += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < nwaves; ++w) s += wacc[w * nd + b];
+        partial[(int64_t)blockIdx.x * nd + b] = s;
+    }
+}
+
+__global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, int nd, double* __restrict__ out) {
+    // one thread per output; fixed order -> deterministic
+    for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nd; b += gridDim.x * blockDim.x) {
+        double s = 0.0;
+        for (int i = 0; i < nblocks; ++i) s += partial[(int64_t)i * nd + b];
+        out[b] = s;
+    }
+}
+
+// ---------------------------------------------------------------- multipole apply (FP32)
+constexpr int MAX_SHELLS = 64;
+constexpr int SHELL_CHUNK = 16;
+
+struct ApplyArgs {
+    int L, l, m, n_shell;
+    float cnorm, scale;
+    float sx, sy, sz;  // sign factors for x-, y- (1) and z-mirrors: (-1)^m, (-1)^(l+m)
+    double radii_over_L[MAX_SHELLS];
+};
+
+__device__ __forceinline__ float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
+__device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
+
+__global__ __launch_bounds__(256) void k_multipole_apply(const float2* __restrict__ H, const float* __restrict__ Pf,
+                                                         ApplyArgs A, float2* __restrict__ out) {
+    const int L = A.L, Lh = L / 2 + 1;
+    const int64_t nrep = (int64_t)Lh * Lh * Lh;
+    const int64_t N = (int64_t)L * L * L;
+    for (int64_t r = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; r < nrep; r += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(r % Lh), y = (int)((r / Lh) % Lh), x = (int)(r / ((int64_t)Lh * Lh));
+        const int xm = (L - x) % L, ym = (L - y) % L, zm = (L - z) % L;
+        // spectrum values at the 8 mirror modes from 4 half-spectrum reads
+        const float2 h00 = H[((int64_t)x * L + y) * Lh + z], h10 = H[((int64_t)xm * L + y) * Lh + z];
+        const float2 h01 = H[((int64_t)x * L + ym) * Lh + z], h11 = H[((int64_t)xm * L + ym) * Lh + z];
+        // alias geometry
+        float pw[8];
+        float2 yv[8];
+        double nrm[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+            pw[a] = Pf[ax < 0 ? -ax : ax] * Pf[ay < 0 ? -ay : ay] * Pf[az < 0 ? -az : az];
+            nrm[a] = sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
+            float2 yy = ylm_f32(A.l, A.m < 0 ? -A.m : A.m, (float)ax, (float)ay, (float)az, A.cnorm);
+            if (A.m < 0) {  // Y_{l,-m} = (-1)^m conj(Y_{l,m})
+                float sg = ((-A.m) & 1) ? -1.0f : 1.0f;
+                yy = make_float2(sg * yy.x, -sg * yy.y);
+            }
+            yv[a] = yy;
+        }
+        for (int s0 = 0; s0 < A.n_shell; s0 += SHELL_CHUNK) {
+            float2 K[SHELL_CHUNK];
+#pragma unroll
+            for (int s = 0; s < SHELL_CHUNK; ++s) K[s] = make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                if (pw[a] == 0.0f) continue;
+#pragma unroll
+                for (int s = 0; s < SHELL_CHUNK; ++s) {
+                    if (s0 + s < A.n_shell) {
+                        float j = jl_f32(A.l, nrm[a] * A.radii_over_L[s0 + s]) * pw[a];
+                        K[s].x += j * yv[a].x;
+                        K[s].y += j * yv[a].y;
+                    }
+                }
+            }
+#pragma unroll
+            for (int s = 0; s < SHELL_CHUNK; ++s) {
+                if (s0 + s >= A.n_shell) break;
+                float2* o = out + (int64_t)(s0 + s) * N;
+                const float2 k = make_float2(K[s].x * A.scale, K[s].y * A.scale);
+                const float2 kc = conjf2(k);
+                // x-mirror: (-1)^m conj K; y-mirror: conj K; z-mirror: (-1)^(l+m) K
+                o[((int64_t)x * L + y) * L + z] = cmulf(h00, k);
+                if (xm != x) o[((int64_t)xm * L + y) * L + z] = cmulf(h10, make_float2(A.sx * kc.x, A.sx * kc.y));
+                if (ym != y) o[((int64_t)x * L + ym) * L + z] = cmulf(h01, kc);
+                if (xm != x && ym != y) o[((int64_t)xm * L + ym) * L + z] = cmulf(h11, make_float2(A.sx * k.x, A.sx * k.y));
+                if (zm != z) {
+                    const float szf = A.sz;
+                    o[((int64_t)x * L + y) * L + zm] = cmulf(conjf2(h11), make_float2(szf * k.x, szf * k.y));
+                    if (xm != x)
+                        o[((int64_t)xm * L + y) * L + zm] =
+                            cmulf(conjf2(h01), make_float2(szf * A.sx * kc.x, szf * A.sx * kc.y));
+                    if (ym != y) o[((int64_t)x * L + ym) * L + zm] = cmulf(conjf2(h10), make_float2(szf * kc.x, szf * kc.y));
+                    if (xm != x && ym != y)
+                        o[((int64_t)xm * L + ym) * L + zm] =
+                            cmulf(conjf2(h00), make_float2(szf * A.sx * k.x, szf * A.sx * k.y));
+                }
+            }
+        }
+    }
+}
+
+__global__ void k_p_to_float(const double* __restrict__ P, int n, float* __restrict__ Pf) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) Pf[i] = (float)P[i];
+}
+
+// ---------------------------------------------------------------- FP64 sums
+template <typename T>
+__global__ __launch_bounds__(RED_THREADS) void k_reduce_sum(const T* __restrict__ x, int64_t n, int ncomp,
+                                                            double* __restrict__ partial) {
+    __shared__ double sh[2][RED_THREADS / 64];
+    double s0 = 0.0, s1 = 0.0;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        if (ncomp == 1) {
+            s0 += (double)x[i];
+        } else {
+            s0 += (double)x[2 * i];
+            s1 += (double)x[2 * i + 1];
+        }
+    }
+    s0 = wave_sum(s0);
+    s1 = wave_sum(s1);
+    if (threadIdx.x % 64 == 0) {
+        sh[0][threadIdx.x / 64] = s0;
+        sh[1][threadIdx.x / 64] = s1;
+    }
+    __syncthreads();
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int w = 0; w < RED_THREADS / 64; ++w) t += sh[threadIdx.x][w];
+        partial[2 * blockIdx.x + threadIdx.x] = t;
+    }
+}
+
+__global__ void k_reduce_final(const double* __restrict__ partial, int nb, double* __restrict__ out) {
+    if (threadIdx.x < 2) {
+        double t = 0.0;
+        for (int i = 0; i < nb; ++i) t += partial[2 * i + threadIdx.x];
+        out[threadIdx.x] = t;
+    }
+}
+
+int check_desc(const hermes_window_desc& d) {
+    if (d.type < 0 || d.type > HERMES_WIN_SHELL_MULTIPOLE) {
+        hermes_set_error("unknown window type %d", d.type);
+        return 2;
+    }
+    if (d.type == HERMES_WIN_SHELL_MULTIPOLE && (d.l < 0 || d.l > 40 || d.m > d.l || d.m < -d.l)) {
+        hermes_set_error("invalid multipole (l=%d, m=%d)", d.l, d.m);
+        return 2;
+    }
+    return 0;
+}
+
+double factorial_d(int n) {
+    double o = 1.0;
+    for (int i = 2; i <= n; ++i) o *= i;
+    return o;
+}
+
+}  // namespace
+
+// N_lm (-1)^m (2m-1)!! for the FP32 spherical harmonics (m >= 0).
+double hermes_ylm_cnorm(int l, int m) {
+    double n = sqrt((2.0 * l + 1.0) / (4.0 * HERMES_PI) * factorial_d(l - m) / factorial_d(l + m));
+    double df = 1.0;
+    for (int i = 1; i <= m; ++i) df *= (2.0 * i - 1.0);
+    return ((m & 1) ? -1.0 : 1.0) * df * n;
+}
+
+extern "C" {
+
+int hermes_window_build_rfft(hermes_ctx* ctx, int J, const double* P, const hermes_window_desc* desc, int dbl,
+                             void* out) {
+    HERMES_REQUIRE(ctx && desc && P && out, "hermes_window_build_rfft: null argument");
+    HERMES_REQUIRE(J >= 1 && J <= 10, "J=%d unsupported", J);
+    HERMES_REQUIRE(desc->type != HERMES_WIN_SHELL_MULTIPOLE, "multipole windows need the full complex layout");
+    if (int rc = check_desc(*desc)) return rc;
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    int L = 1 << J;
+    int64_t n = (int64_t)L * L * (L / 2 + 1);
+    if (dbl) k_build_rfft<double><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *desc, (double*)out);
+    else k_build_rfft<float><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *desc, (float*)out);
+    HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+int hermes_window_build_full(hermes_ctx* ctx, int J, const double* P, const hermes_window_desc* desc, int dbl,
+                             void* out) {
+    HERMES_REQUIRE(ctx && desc && P && out, "hermes_window_build_full: null argument");
+    HERMES_REQUIRE(J >= 1 && J <= 10, "J=%d unsupported", J);
+    if (int rc = check_desc(*desc)) return rc;
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    int L = 1 << J;
+    int64_t n = (int64_t)L * L * L;
+    if (dbl) k_build_full<double><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *desc, (double*)out);
+    else k_build_full<float><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *desc, (float*)out);
+    HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+int hermes_spectrum_mul(hermes_ctx* ctx, int J, int half, int dbl, void* spec, const void* kernel, int kernel_complex,
+                        double scale) {
+    HERMES_REQUIRE(ctx && spec && kernel, "hermes_spectrum_mul: null argument");
+    HERMES_REQUIRE(J >= 1 && J <= 10, "J=%d unsupported", J);
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    int L = 1 << J;
+    int64_t n = (int64_t)L * L * (half ? (L / 2 + 1) : L);
+    if (dbl)
+        k_spec_mul<double><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(n, (double*)spec, (const double*)kernel,
+                                                                                 kernel_complex, scale);
+    else
+        k_spec_mul<float><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(n, (float*)spec, (const float*)kernel,
+                                                                                kernel_complex, (float)scale);
+    HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
+                       const hermes_window_desc* descs, int n_desc, double* out_host) {
+    HERMES_REQUIRE(ctx && F1 && F2 && P && descs && out_host, "hermes_pair_reduce: null argument");
+    HERMES_REQUIRE(J >= 1 && J <= 10, "J=%d unsupported", J);
+    HERMES_REQUIRE(n_desc >= 1 && n_desc <= 4096, "hermes_pair_reduce: n_desc=%d out of range", n_desc);
+    for (int b = 0; b < n_desc; ++b) {
+        if (int rc = check_desc(descs[b])) return rc;
+        HERMES_REQUIRE(descs[b].type != HERMES_WIN_SHELL_MULTIPOLE || descs[b].m == 0,
+                       "pair reduction supports only m = 0 multipole windows");
+    }
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int L = 1 << J;
+    const int Lh = L / 2 + 1;
+    const int64_t nrep = (int64_t)Lh * Lh * Lh;
+    const int nblocks = (int)hermes_blocks(nrep, RED_THREADS, 2048);
+    size_t need = sizeof(hermes_window_desc) * n_desc + sizeof(float) * n_desc + 64 +
+                  sizeof(double) * ((size_t)nblocks * n_desc + n_desc);
+    char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);
+    HERMES_REQUIRE(ws, "%s", hermes_last_error());
+    hermes_window_desc* d_desc = (hermes_window_desc*)ws;
+    float* d_cn = (float*)(ws + sizeof(hermes_window_desc) * n_desc);
+    double* partial = (double*)(ws + ((sizeof(hermes_window_desc) * n_desc + sizeof(float) * n_desc + 63) / 64) * 64);
+    double* d_out = partial + (size_t)nblocks * n_desc;
+    std::vector<float> cn(n_desc);
+    for (int b = 0; b < n_desc; ++b)
+        cn[b] = descs[b].type == HERMES_WIN_SHELL_MULTIPOLE ? (float)hermes_ylm_cnorm(descs[b].l, 0) : 0.0f;
+    HERMES_CHECK_HIP(hipMemcpyAsync(d_desc, descs, sizeof(hermes_window_desc) * n_desc, hipMemcpyHostToDevice, ctx->stream));
+    HERMES_CHECK_HIP(hipMemcpyAsync(d_cn, cn.data(), sizeof(float) * n_desc, hipMemcpyHostToDevice, ctx->stream));
+    size_t shmem = sizeof(double) * (RED_THREADS / 64) * n_desc;
+    HERMES_REQUIRE(shmem <= 64 * 1024, "too many windows for one fused pass");
+    if (dbl)
+        k_pair_reduce<double2, true><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
+            L, (const double2*)F1, (const double2*)F2, P, d_desc, d_cn, n_desc, partial);
+    else
+        k_pair_reduce<float2, false><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
+            L, (const float2*)F1, (const float2*)F2, P, d_desc, d_cn, n_desc, partial);
+    HERMES_LAUNCH_CHECK();
+    k_sum_partials<<<hermes_blocks(n_desc, 64), 64, 0, ctx->stream>>>(partial, nblocks, n_desc, d_out);
+    HERMES_LAUNCH_CHECK();
+    HERMES_CHECK_HIP(hipMemcpyAsync(out_host, d_out, sizeof(double) * n_desc, hipMemcpyDeviceToHost, ctx->stream));
+    HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return 0;
+}
+
+int hermes_multipole_apply_f32(hermes_ctx* ctx, int J, const void* H, const double* P, const double* radii,
+                               int n_shell, int l, int m, double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_apply_f32: null argument");
+    HERMES_REQUIRE(J >= 2 && J <= 10, "J=%d unsupported", J);
+    HERMES_REQUIRE(n_shell >= 1 && n_shell <= MAX_SHELLS, "n_shell=%d outside 1..%d", n_shell, MAX_SHELLS);
+    HERMES_REQUIRE(l >= 0 && l <= 40 && m >= -l && m <= l, "invalid multipole (l=%d, m=%d)", l, m);
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int L = 1 << J;
+    float* Pf = (float*)hermes_workspace(ctx, WS_SPEC, sizeof(float) * (L + 1));
+    HERMES_REQUIRE(Pf, "%s", hermes_last_error());
+    k_p_to_float<<<hermes_blocks(L + 1, 256), 256, 0, ctx->stream>>>(P, L + 1, Pf);
+    HERMES_LAUNCH_CHECK();
+    ApplyArgs A;
+    A.L = L;
+    A.l = l;
+    A.m = m;
+    A.n_shell = n_shell;
+    int am = m < 0 ? -m : m;
+    A.cnorm = (float)hermes_ylm_cnorm(l, am);
+    A.scale = (float)scale;
+    A.sx = (am & 1) ? -1.0f : 1.0f;
+    A.sy = 1.0f;
+    A.sz = ((l + am) & 1) ? -1.0f : 1.0f;
+    for (int s = 0; s < MAX_SHELLS; ++s) A.radii_over_L[s] = s < n_shell ? radii[s] / L : 0.0;
+    const int64_t nrep = (int64_t)(L / 2 + 1) * (L / 2 + 1) * (L / 2 + 1);
+    k_multipole_apply<<<hermes_blocks(nrep, 256, 1 << 20), 256, 0, ctx->stream>>>((const float2*)H, Pf, A,
+                                                                                   (float2*)out);
+    HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+int hermes_reduce_sum(hermes_ctx* ctx, const void* x, int64_t n, int ncomp, int dbl, double* out_host) {
+    HERMES_REQUIRE(ctx && x && out_host, "hermes_reduce_sum: null argument");
+    HERMES_REQUIRE(ncomp == 1 || ncomp == 2, "ncomp must be 1 or 2");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int nb = (int)hermes_blocks(n, RED_THREADS, 1024);
+    double* ws = (double*)hermes_workspace(ctx, WS_MISC, sizeof(double) * (2 * nb + 2));
+    HERMES_REQUIRE(ws, "%s", hermes_last_error());
+    if (dbl) k_reduce_sum<double><<<nb, RED_THREADS, 0, ctx->stream>>>((const double*)x, n, ncomp, ws);
+    else k_reduce_sum<float><<<nb, RED_THREADS, 0, ctx->stream>>>((const float*)x, n, ncomp, ws);
+    HERMES_LAUNCH_CHECK();
+    k_reduce_final<<<1, 64, 0, ctx->stream>>>(ws, nb, ws + 2 * nb);
+    HERMES_LAUNCH_CHECK();
+    double h[2];
+    HERMES_CHECK_HIP(hipMemcpyAsync(h, ws + 2 * nb, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
+    HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    out_host[0] = h[0];
+    if (ncomp == 2) out_host[1] = h[1];
+    return 0;
+}
+
+}  // extern "C"

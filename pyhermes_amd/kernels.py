@@ -4,11 +4,19 @@ All tensors are torch tensors on the HIP device; outputs are allocated here
 (caller-owned afterwards) unless passed in.  No CPU fallback exists.
 """
 
+import ctypes
+
 import numpy as np
 import torch
 
 from . import _lib
 from ._lib import ptr
+
+_DP = ctypes.POINTER(ctypes.c_double)
+
+
+def ctypes_byref(obj):
+    return ctypes.byref(obj)
 
 
 def _dev():
@@ -96,3 +104,109 @@ def fftn_c2c(x, inverse=False, out=None):
         out = torch.empty_like(x)
     _lib.call("hermes_fft_c2c", _lib.context(), _J_of(L), batch, dbl, 1 if inverse else 0, ptr(x), ptr(out))
     return out
+
+
+# ---------------------------------------------------------------- windows
+WIN_CODES = {"sphere": 0, "gaussian": 1, "shell": 2, "thick_shell": 3, "gaussian_shell": 4,
+             "inverse_laplacian": 5, "shell_multipole": 6}
+
+
+def window_desc(wtype, a0=0.0, a1=0.0, l=0, m=0):
+    """A hermes_window_desc; lengths in grid units."""
+    return _lib.WindowDesc(WIN_CODES[wtype], int(l), int(m), 0, float(a0), float(a1))
+
+
+def _np_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def window_build_rfft(J, P, desc, dbl=True):
+    """(L, L, L//2+1) real kernel of one isotropic window (P: device tensor of L+1 doubles)."""
+    L = 1 << J
+    out = torch.empty((L, L, L // 2 + 1), dtype=torch.float64 if dbl else torch.float32, device=P.device)
+    _lib.call("hermes_window_build_rfft", _lib.context(), int(J), ptr(P), ctypes_byref(desc), 1 if dbl else 0,
+              ptr(out))
+    return out
+
+
+def window_build_full(J, P, desc, dbl=True):
+    """(L, L, L) complex kernel of one window."""
+    L = 1 << J
+    out = torch.empty((L, L, L), dtype=torch.complex128 if dbl else torch.complex64, device=P.device)
+    _lib.call("hermes_window_build_full", _lib.context(), int(J), ptr(P), ctypes_byref(desc), 1 if dbl else 0,
+              ptr(out))
+    return out
+
+
+def spectrum_mul(spec, kernel, scale=1.0):
+    """spec *= kernel * scale in place (half spectrum with a real kernel, or full with a complex one)."""
+    L = spec.shape[0]
+    half = 1 if spec.shape[-1] != L else 0
+    dbl = 1 if spec.dtype == torch.complex128 else 0
+    kc = 1 if kernel.is_complex() else 0
+    _lib.call("hermes_spectrum_mul", _lib.context(), _J_of(L), half, dbl, ptr(spec), ptr(kernel), kc, float(scale))
+    return spec
+
+
+def pair_reduce(F1, F2, P, descs):
+    """sum_k K_b(k) Re(F1 conj F2) over the full grid, for every descriptor (host float64 array)."""
+    L = F1.shape[0]
+    n = len(descs)
+    arr = (_lib.WindowDesc * n)(*descs)
+    out = np.zeros(n, dtype=np.float64)
+    dbl = 1 if F1.dtype == torch.complex128 else 0
+    _lib.call("hermes_pair_reduce", _lib.context(), _J_of(L), dbl, ptr(F1), ptr(F2), ptr(P), arr, n,
+              out.ctypes.data_as(_DP))
+    return out
+
+
+def multipole_apply_f32(H, P, radii_grid, l, m, scale, out=None):
+    """(S, L, L, L) complex64 spectra H_full * K_{R_s,l,m} * scale from the half spectrum H (complex64)."""
+    L = H.shape[0]
+    r = _np_f64(radii_grid)
+    S = r.size
+    if out is None:
+        out = torch.empty((S, L, L, L), dtype=torch.complex64, device=H.device)
+    _lib.call("hermes_multipole_apply_f32", _lib.context(), _J_of(L), ptr(H), ptr(P), r.ctypes.data_as(_DP), S,
+              int(l), int(m), float(scale), ptr(out))
+    return out
+
+
+def reduce_sum(x):
+    """Deterministic FP64 sum of a real or complex tensor."""
+    out = np.zeros(2, dtype=np.float64)
+    comp = 2 if x.is_complex() else 1
+    dbl = 1 if x.dtype in (torch.float64, torch.complex128) else 0
+    n = x.numel()
+    _lib.call("hermes_reduce_sum", _lib.context(), ptr(x.contiguous()), n, comp, dbl, out.ctypes.data_as(_DP))
+    return complex(out[0], out[1]) if comp == 2 else float(out[0])
+
+
+# ---------------------------------------------------------------- contraction
+def multipole_summand_f64(d1, F1, F2, gamma):
+    """Raw complex sum of the plain FP64 triple-product contraction (no 4 pi / N)."""
+    L = d1.shape[0]
+    g = _np_f64(gamma)
+    out = np.zeros(2, dtype=np.float64)
+    _lib.call("hermes_multipole_summand_f64", _lib.context(), _J_of(L), ptr(d1), ptr(F1), ptr(F2),
+              g.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
+    return complex(out[0], out[1])
+
+
+def multipole_gram_f32(d1, U, Q, lam, V=None, v_conj=False):
+    """(n_u, n_v) FP64 Gram sums; see hermes_multipole_gram_f32."""
+    L = d1.shape[0]
+    n_u = U.shape[0]
+    n_v = V.shape[0] if V is not None else n_u
+    q = _np_f64(Q)
+    lm = _np_f64(lam)
+    out = np.zeros((n_u, n_v), dtype=np.float64)
+    _lib.call("hermes_multipole_gram_f32", _lib.context(), _J_of(L), ptr(d1), ptr(U), n_u, ptr(V), n_v,
+              1 if v_conj else 0, q.ctypes.data_as(_DP), lm.ctypes.data_as(_DP), out.ctypes.data_as(_DP))
+    return out
+
+
+def mfma_selftest():
+    out = np.zeros(1, dtype=np.float64)
+    _lib.call("hermes_mfma_selftest", _lib.context(), out.ctypes.data_as(_DP))
+    return float(out[0])

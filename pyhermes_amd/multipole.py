@@ -1,0 +1,125 @@
+"""Device orchestration of the spherical-harmonic 3PCF multipoles.
+
+Two paths compute ``ddd_l`` for shell pairs (r12, r13):
+
+* ``ddd_multipole_plain_f64`` -- a per-set restatement of calc_DDD_multipole
+  (corr3pcf_multipoles.py:460-579) on the GPU in FP64: for every (l, m >= 0) two
+  window builds, two forward and two inverse complex FFTs and the FP64
+  contraction, exactly as the reference sequences them.
+* ``ddd_multipole_batched_f32`` -- all shell pairs at once: one real FFT per
+  field, one fused multipole window launch per (l, m) writing every shell's
+  spectrum, one batched inverse FFT, and one MFMA Gram contraction (FP32
+  products, FP64 accumulation).  When fields 2 and 3 and their shell lists are
+  identical, F_{l,-m} = (-1)^(l+m) conj(F_{l,m}) removes the second set of FFTs.
+
+Both return ddd_l following combine_multipole_m_terms (corr3pcf_multipoles.py:173-180):
+  ddd_l = (-1)^l Re[v_0 + sum_{m>=1} (-1)^m (v_m + conj v_m)],
+  v_m = 4 pi / N * sum_x d1 sum_{l1,l2} G3 F_{r12,l,m}[x-l1] F_{r13,l,-m}[x-l2].
+"""
+
+import numpy as np
+import torch
+
+from . import kernels
+from .utils.wavelet_grid import cal_gamma, fourier_power_spectrum
+
+
+class MultipoleGeometry:
+    """Host tables shared by both paths: |phi_hat|^2 (device), Gamma and its eigen-decomposition."""
+
+    def __init__(self, phi, L, phi_resolution=1024, device=None):
+        self.L = int(L)
+        self.J = int(L).bit_length() - 1
+        self.phi = np.asarray(phi, dtype=np.float64)
+        self.P_host = fourier_power_spectrum(self.phi, 0, 1, self.L, phi_resolution)
+        self.P = kernels.as_device(self.P_host, torch.float64)
+        self.gamma = cal_gamma(self.phi, self.phi.size // phi_resolution, phi_resolution)
+        lam, Q = np.linalg.eigh(self.gamma)
+        self.lam = lam
+        self.Q = Q  # Q[tap, eigenvector]
+
+
+def combine_m_terms(v_re_by_m, l):
+    """(-1)^l [Re v_0 + 2 sum_{m>=1} (-1)^m Re v_m] (only real parts enter)."""
+    out = np.array(v_re_by_m[0], dtype=np.float64, copy=True)
+    for m in range(1, l + 1):
+        out = out + 2.0 * ((-1) ** m) * np.asarray(v_re_by_m[m])
+    return ((-1) ** l) * out
+
+
+# ---------------------------------------------------------------- plain FP64
+def _multipole_field_f64(geo, delta_c, R_grid, l, m):
+    """field @ radial_multipole window (complex_full_fft), FP64: fftn -> x K -> ifftn."""
+    K = kernels.window_build_full(geo.J, geo.P, kernels.window_desc("shell_multipole", R_grid, l=l, m=m), dbl=True)
+    spec = kernels.fftn_c2c(delta_c)
+    kernels.spectrum_mul(spec, K, 1.0 / geo.L ** 3)
+    del K
+    kernels.fftn_c2c(spec, inverse=True, out=spec)
+    return spec
+
+
+def ddd_multipole_plain_f64(geo, d1, d2, d3, R12_grid, R13_grid, l_min, l_max, return_m_values=False):
+    """One (r12, r13) set, every (l, m) recomputed from scratch as the reference does.  d*: (L,L,L) float64."""
+    N = geo.L ** 3
+    d2c = torch.complex(d2, torch.zeros_like(d2))
+    d3c = d2c if d3 is d2 else torch.complex(d3, torch.zeros_like(d3))
+    out = []
+    mvals = {}
+    for l in range(l_min, l_max + 1):
+        vm = np.empty(l + 1, dtype=np.complex128)
+        for m in range(l + 1):
+            F1 = _multipole_field_f64(geo, d2c, R12_grid, l, m)
+            F2 = _multipole_field_f64(geo, d3c, R13_grid, l, -m)
+            raw = kernels.multipole_summand_f64(d1, F1, F2, geo.gamma)
+            vm[m] = 4.0 * np.pi * raw / N
+            del F1, F2
+        mvals[l] = vm
+        out.append(combine_m_terms([v.real for v in vm], l))
+    out = np.asarray(out, dtype=np.float64)
+    return (out, mvals) if return_m_values else out
+
+
+# ---------------------------------------------------------------- batched FP32
+def _shell_blocks(n, size=16):
+    return [(i, min(n, i + size)) for i in range(0, n, size)]
+
+
+def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min, l_max, timings=None):
+    """All (r12, r13) shell pairs: returns (n12, n13, n_l) float64.  d*: (L,L,L) float32 device tensors."""
+    L, J = geo.L, geo.J
+    N = L ** 3
+    r12 = np.asarray(radii12_grid, dtype=np.float64)
+    r13 = np.asarray(radii13_grid, dtype=np.float64)
+    auto = (d3 is d2) and r12.shape == r13.shape and np.array_equal(r12, r13)
+    H2 = kernels.rfftn(d2)
+    H3 = H2 if d3 is d2 else kernels.rfftn(d3)
+    n_l = l_max - l_min + 1
+    result = np.zeros((r12.size, r13.size, n_l), dtype=np.float64)
+    blocks12 = _shell_blocks(r12.size)
+    blocks13 = _shell_blocks(r13.size)
+    U = torch.empty((r12.size, L, L, L), dtype=torch.complex64, device=d1.device)
+    V = None if auto else torch.empty((r13.size, L, L, L), dtype=torch.complex64, device=d1.device)
+    for li, l in enumerate(range(l_min, l_max + 1)):
+        v_re = []
+        for m in range(l + 1):
+            kernels.multipole_apply_f32(H2, geo.P, r12, l, m, 1.0 / N, out=U)
+            kernels.fftn_c2c(U, inverse=True, out=U)
+            G = np.zeros((r12.size, r13.size))
+            if auto:
+                sign = (-1.0) ** (l + m)
+                for a0, a1 in blocks12:
+                    for b0, b1 in blocks13:
+                        if b0 == a0:
+                            g = kernels.multipole_gram_f32(d1, U[a0:a1], geo.Q, geo.lam)
+                        else:
+                            g = kernels.multipole_gram_f32(d1, U[a0:a1], geo.Q, geo.lam, V=U[b0:b1], v_conj=True)
+                        G[a0:a1, b0:b1] = sign * g
+            else:
+                kernels.multipole_apply_f32(H3, geo.P, r13, l, -m, 1.0 / N, out=V)
+                kernels.fftn_c2c(V, inverse=True, out=V)
+                for a0, a1 in blocks12:
+                    for b0, b1 in blocks13:
+                        G[a0:a1, b0:b1] = kernels.multipole_gram_f32(d1, U[a0:a1], geo.Q, geo.lam, V=V[b0:b1])
+            v_re.append(4.0 * np.pi * G / N)
+        result[:, :, li] = combine_m_terms(v_re, l)
+    return result

@@ -1,0 +1,103 @@
+"""GPU parity: window kernels, fused pair reduction and fused multipole apply vs the oracle."""
+
+import numpy as np
+import pytest
+
+from oracle import scaling, windows as ow
+
+pytestmark = pytest.mark.gpu
+
+ISO = [("sphere", 2.5, 0.0), ("gaussian", 1.5, 0.0), ("shell", 3.0, 0.0), ("thick_shell", 3.0, 1.5),
+       ("gaussian_shell", 3.0, 1.0), ("inverse_laplacian", 0.0, 0.0)]
+
+
+def _P(phi, L):
+    import torch
+    from pyhermes_amd import kernels
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    return P, kernels.as_device(P, torch.float64)
+
+
+@pytest.mark.parametrize("wt,a0,a1", ISO)
+def test_build_rfft_matches_octant_oracle(phi, wt, a0, a1):
+    from pyhermes_amd import kernels
+    J = 4
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    got = kernels.window_build_rfft(J, Pd, kernels.window_desc(wt, a0, a1), dbl=True).cpu().numpy()
+    ref = ow.octant_rfft_kernel(L, P, wt, a0, a1)
+    assert np.abs(got - ref).max() <= 1e-13 * np.abs(ref).max()
+    got32 = kernels.window_build_rfft(J, Pd, kernels.window_desc(wt, a0, a1), dbl=False).cpu().numpy()
+    assert np.abs(got32 - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("l,m", [(0, 0), (1, 1), (2, -1), (3, 2), (5, 0), (10, 7)])
+def test_build_full_multipole_matches_oracle(phi, l, m):
+    from pyhermes_amd import kernels
+    J = 4
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    R = 3.7
+    got = kernels.window_build_full(J, Pd, kernels.window_desc("shell_multipole", R, l=l, m=m)).cpu().numpy()
+    ref = ow.radial_multipole_window(L, P, R, l, m)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dbl", [True, False])
+def test_pair_reduce_matches_convolution_oracle(phi, dbl):
+    import torch
+    from pyhermes_amd import kernels
+    J = 5
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    rng = np.random.default_rng(3)
+    f1 = rng.normal(size=(L, L, L))
+    f2 = f1 + 0.3 * rng.normal(size=(L, L, L))
+    td = torch.float64 if dbl else torch.float32
+    F1 = kernels.rfftn(kernels.as_device(f1, td))
+    F2 = kernels.rfftn(kernels.as_device(f2, td))
+    descs = [kernels.window_desc("shell", R) for R in (1.5, 3.0, 5.25, 9.0)]
+    descs += [kernels.window_desc("sphere", 2.0), kernels.window_desc("gaussian_shell", 3.0, 1.0)]
+    descs += [kernels.window_desc("shell_multipole", R, l=l) for R in (2.0, 6.5) for l in (0, 2, 4)]
+    got = kernels.pair_reduce(F1, F2, Pd, descs) / L ** 6
+    tol = 1e-11 if dbl else 2e-5
+    for d, g in zip(descs, got):
+        if d.type == 6:
+            K = ow.radial_multipole_window(L, P, d.a0, d.l, 0)
+            conv = np.real(ow.convolve_complex(f1, K))
+            ref = float(np.sum(conv * f2) / L ** 3)
+        else:
+            wt = [k for k, v in kernels.WIN_CODES.items() if v == d.type][0]
+            ref = ow.pair_product(f1, f2, ow.octant_rfft_kernel(L, P, wt, d.a0, d.a1))
+        scale = np.sqrt(np.mean(f1 ** 2) * np.mean(f2 ** 2))
+        assert abs(g - ref) <= tol * scale, (d.type, d.a0, d.l, g, ref)
+
+
+@pytest.mark.parametrize("l,m", [(0, 0), (2, 1), (3, -2), (6, 6)])
+def test_multipole_apply_matches_oracle(phi, l, m):
+    import torch
+    from pyhermes_amd import kernels
+    J = 4
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(L, L, L))
+    H = kernels.rfftn(kernels.as_device(d, torch.float32))
+    radii = [1.25, 2.5, 4.0, 6.75, 7.9]
+    out = kernels.multipole_apply_f32(H, Pd, radii, l, m, 1.0 / L ** 3).cpu().numpy()
+    Fd = np.fft.fftn(d)
+    for s, R in enumerate(radii):
+        ref = Fd * ow.radial_multipole_window(L, P, R, l, m) / L ** 3
+        err = np.abs(out[s] - ref).max() / np.abs(ref).max()
+        assert err < 2e-5, (s, R, err)
+
+
+def test_reduce_sum(phi):
+    import torch
+    from pyhermes_amd import kernels
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=(17, 33, 65))
+    assert kernels.reduce_sum(kernels.as_device(x, torch.float64)) == pytest.approx(x.sum(), rel=1e-13, abs=1e-12)
+    c = x + 1j * rng.normal(size=x.shape)
+    s = kernels.reduce_sum(kernels.as_device(c, torch.complex64))
+    assert abs(s - c.sum()) < 1e-4 * np.abs(c).sum()
