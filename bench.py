@@ -1,0 +1,206 @@
+"""Headline benchmark: spherical-harmonic 3PCF multipole sets per second at J=9, l <= 10.
+
+Workload (synthetic): a 2,000,000-particle seeded lognormal mock (Gaussian field
+with P(k) ~ k^-1.5 on a 256^3 grid, sigma_LN = 1), box 1000 Mpc/h, projected at
+J=9 (512^3), delta = eps - rho; 15 thin radial shells R = linspace(10, 120, 15)
+Mpc/h for both edges; multipoles l = 0..10 (66 m-terms per set).
+
+A "set" is one row of delta_ddd_l (l = 0..10) for one (r12, r13) pair; the 15 shells
+give 120 unique (r12 <= r13) sets (225 ordered).  One step = one pass of the
+FP32 batched path over all pairs (one r2c FFT, 66 fused window launches, 66
+batched inverse FFTs, 66 MFMA Gram contractions).  value = unique sets / s.
+
+Also measured in the same run: the plain FP64 GPU path on one set (the
+per-set restatement of calc_DDD_multipole), the FP32-vs-FP64 relative L2 error
+on that set, per-stage HIP-event timings, the roofline of the dominant kernel,
+and the CPU oracle on a bounded sample (see cpu_baseline.sample).
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+MFMA_F32_PEAK_TFLOPS = 157.3
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_baseline(J_sample, lm_terms, shells_grid_at_sample, box, seed):
+    """Time the CPU oracle (C/OpenMP restatement + scipy.fft) on a bounded sample.
+
+    Sample: the given (l, m) terms of one set at J_sample (mock projected at that J).
+    Returns seconds per m-term at J_sample and the thread count used.
+    """
+    import numpy as np
+    from scipy.fft import fftn, ifftn
+
+    from oracle import multipole as om, native, projection as oproj, scaling
+    from oracle.windows import radial_multipole_window
+    from pyhermes_amd.mocks import lognormal_mock
+
+    threads = native.num_threads()
+    L = 1 << J_sample
+    phi = scaling.sample_scaling_function()
+    pos, _ = lognormal_mock(300000, 64, box, seed)
+    w = np.full(pos.shape[0], 1.0 / pos.shape[0])
+    eps, _, _ = oproj.project_c(pos, w, phi, J_sample, box)
+    delta = eps - 1.0 / L ** 3
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    gamma = scaling.cal_gamma(phi, 3, 1024)
+    R1, R2 = shells_grid_at_sample
+    t0 = time.perf_counter()
+    for (l, m) in lm_terms:
+        K1 = radial_multipole_window(L, P, R1, l, m)
+        F1 = ifftn(fftn(delta, workers=threads) * K1, workers=threads)
+        K2 = radial_multipole_window(L, P, R2, l, -m)
+        F2 = ifftn(fftn(delta, workers=threads) * K2, workers=threads)
+        om.summand_c(delta, F1, F2, gamma)
+    dt = time.perf_counter() - t0
+    return dt / len(lm_terms), threads
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=3)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--J", type=int, default=9)
+    ap.add_argument("--lmax", type=int, default=10)
+    ap.add_argument("--nshell", type=int, default=15)
+    ap.add_argument("--particles", type=int, default=2_000_000)
+    ap.add_argument("--skip-fp64", action="store_true")
+    ap.add_argument("--skip-cpu", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1", "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from pyhermes_amd import kernels, multipole
+    from pyhermes_amd.mocks import lognormal_mock
+    from pyhermes_amd.utils.wavelet_grid import sample_scaling_function
+
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    torch.cuda.set_device(0)
+    box = 1000.0
+    J = args.J
+    L = 1 << J
+    N = L ** 3
+    radii_phys = np.linspace(10.0, 120.0, args.nshell)
+    radii = radii_phys * L / box
+    lmax = args.lmax
+    n_mterms = (lmax + 1) * (lmax + 2) // 2
+    n_unique = args.nshell * (args.nshell + 1) // 2
+    n_ordered = args.nshell ** 2
+
+    log(f"mock: {args.particles} particles, ng=256, box={box}")
+    pos, _ = lognormal_mock(args.particles, 256, box, 3000)
+    w = np.full(pos.shape[0], 1.0 / pos.shape[0])
+    phi = sample_scaling_function()
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    pos_d = kernels.as_device(pos, torch.float64)
+    w_d = kernels.as_device(w, torch.float64)
+    kernels.project(pos_d, w_d, J, box, phi, want32=True, want64=True)  # warm
+    torch.cuda.synchronize()
+    ev0.record()
+    proj = kernels.project(pos_d, w_d, J, box, phi, want32=True, want64=False)
+    ev1.record()
+    torch.cuda.synchronize()
+    proj_ms = ev0.elapsed_time(ev1)
+    proj64 = kernels.project(pos_d, w_d, J, box, phi, want32=False, want64=True)
+    rho = 1.0 / N
+    d64 = proj64["eps64"] - rho
+    d32 = proj["eps32"] - np.float32(rho)
+    del proj, proj64
+    log(f"projection J={J}: {proj_ms:.3f} ms (eps32 output)")
+
+    geo = multipole.MultipoleGeometry(phi, L)
+    log(f"FP32 batched: warmup {args.warmup}, steps {args.steps}")
+    for _ in range(args.warmup):
+        res32 = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, lmax)
+    torch.cuda.synchronize()
+    timer = multipole.StageTimer()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        res32 = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, lmax, timer=timer)
+    torch.cuda.synchronize()
+    t_fp32 = (time.perf_counter() - t0) / args.steps
+    stages = {k: round(v[0] / args.steps, 3) for k, v in timer.summary().items()}
+    gram_calls = timer.summary()["gram"][1]
+    gram_ms = timer.summary()["gram"][0] / gram_calls
+    log(f"FP32 step {t_fp32 * 1e3:.1f} ms; stages (ms/step) {stages}")
+
+    out = {}
+    i1, i2 = 4, 9
+    if not args.skip_fp64:
+        log("plain FP64 GPU path: one set")
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        res64 = multipole.ddd_multipole_plain_f64(geo, d64, d64, d64, radii[i1], radii[i2], 0, lmax,
+                                                  progress=lambda l: log(f"  FP64 l={l} done"))
+        torch.cuda.synchronize()
+        t_fp64 = time.perf_counter() - t0
+        rel = float(np.linalg.norm(res32[i1, i2] - res64) / np.linalg.norm(res64))
+        out["fp64_plain"] = {"sets_per_s": 1.0 / t_fp64, "s_per_set": t_fp64, "set": [i1, i2]}
+        out["fp32_vs_fp64_rel_l2"] = rel
+        out["speedup_vs_fp64_plain"] = (n_unique / t_fp32) * t_fp64
+        log(f"FP64 set {t_fp64:.2f} s; rel L2 err FP32 vs FP64 = {rel:.3e}")
+
+    cpu = None
+    if not args.skip_cpu:
+        Js = J - 1
+        terms = [(0, 0), (1, 0), (1, 1)]
+        scale_R = (1 << Js) / box
+        log(f"CPU oracle sample: {terms} of one set at J={Js}")
+        per_term, threads = cpu_baseline(Js, terms, (radii_phys[i1] * scale_R, radii_phys[i2] * scale_R), box, 3000)
+        vol = 8.0 ** (J - Js)
+        s_per_set = per_term * vol * n_mterms
+        cpu = {"value": 1.0 / s_per_set, "unit": "sets/s", "cores": threads, "kind": "port",
+               "sample": f"C/OpenMP oracle + scipy.fft, (l,m) in {terms} of one set at J={Js} "
+                         f"({per_term:.2f} s per m-term), scaled x{vol:.0f} to J={J} and x{n_mterms} m-terms"}
+        log(f"CPU: {per_term:.2f} s per m-term at J={Js} -> {1.0 / s_per_set:.3e} sets/s at J={J}")
+
+    # roofline of the dominant kernel (MFMA Gram contraction): useful FP32 MFMA flops per launch
+    S = args.nshell
+    gram_flops = 2.0 * S * S * 54 * N
+    achieved = gram_flops / (gram_ms * 1e-3) / 1e12
+    line = {
+        "metric": "J=9 3PCF(l<=10) sets/s, FP32+batched vs FP64 GPU at <=1e-4 rel err; %HBM peak",
+        "value": n_unique / t_fp32,
+        "unit": "sets/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": t_fp32 * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 (fp64 accumulation)",
+        "data": "synthetic lognormal mock (seeded)",
+        "config": {"workload": f"sh3pcf_multipoles_J{J}_lmax{lmax}_{S}shells", "particles": args.particles,
+                   "J": J, "l_max": lmax, "n_shell": S, "unique_sets_per_step": n_unique,
+                   "ordered_sets_per_step": n_ordered, "m_terms_per_set": n_mterms},
+        "roofline": {"bound": "mfma", "kernel": "k_gram_f32", "achieved": achieved,
+                     "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MFMA_F32_PEAK_TFLOPS,
+                     "traffic": None, "avg_launch_ms": gram_ms},
+        "cpu_baseline": cpu,
+        "stages_ms_per_step": stages,
+        "projection_ms": proj_ms,
+        "projection_hbm_frac": ((args.particles * 32 + N * 4) / (proj_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
+    }
+    line.update(out)
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

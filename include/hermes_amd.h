@@ -52,6 +52,13 @@ int hermes_project(hermes_ctx *ctx, const double *pos, const double *w, int64_t 
                    double *eps64, int32_t *cell_out, int32_t *frac_out, int32_t *cell_counts);
 
 
+/* ---- 3-D FFTs (rocFFT), replacing scipy.fft rfftn / irfftn / fftn / ifftn in
+ * convolution.py:284-295.  Unnormalised; batch transforms are contiguous
+ * (batch, L, L, L).  c2r clobbers its input.  in == out selects in-place (c2c only). */
+int hermes_fft_r2c(hermes_ctx *ctx, int J, int batch, int dbl, const void *in_real, void *out_half);
+int hermes_fft_c2r(hermes_ctx *ctx, int J, int batch, int dbl, void *in_half, void *out_real);
+int hermes_fft_c2c(hermes_ctx *ctx, int J, int batch, int dbl, int inverse, const void *in, void *out);
+
 /* ---- window descriptors (window_functions.py:29-129, :426-438;
  *      radial_multipole_windows.py:128-144 thin-shell profile).
  * Length arguments a0, a1 are in grid units (len_args * L / box_size). */

@@ -106,22 +106,30 @@ struct GramArgs {
 
 constexpr int GRAM_WAVES = 4;
 
-// yz-filtered values Y[b][c] (complex) of field f at plane x for cell (y, z).
-__device__ __forceinline__ void yz_filter(const float2* __restrict__ f, int L, int x, int y, int z, const float* Q,
-                                          float2 Y[9]) {
+typedef float float4_t __attribute__((ext_vector_type(4)));
+
+// Raw values of field f at plane x around cell (y, z): raw[ly * 3 + lz] = f[x, y-ly, z-lz].
+__device__ __forceinline__ void load_plane(const float2* __restrict__ f, int L, int x, int y, int z, float2 raw[9]) {
     const int m = L - 1;
-    float2 Z[3][3];  // Z[ly][c]
 #pragma unroll
     for (int ly = 0; ly < 3; ++ly) {
-        const int yy = (y - ly) & m;
-        const float2* row = f + ((int64_t)x * L + yy) * L;
-        const float2 u0 = row[z & m], u1 = row[(z - 1) & m], u2 = row[(z - 2) & m];
+        const float2* row = f + ((int64_t)x * L + ((y - ly) & m)) * L;
+#pragma unroll
+        for (int lz = 0; lz < 3; ++lz) raw[ly * 3 + lz] = row[(z - lz) & m];
+    }
+}
+
+// yz-filter: Y[b * 3 + c] = sum_{ly, lz} Q[ly][b] Q[lz][c] raw[ly * 3 + lz].
+__device__ __forceinline__ void filter_plane(const float2 raw[9], const float* Q, float2 Y[9]) {
+    float2 Z[3][3];  // Z[ly][c]
+#pragma unroll
+    for (int ly = 0; ly < 3; ++ly)
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const float q0 = Q[0 * 3 + c], q1 = Q[1 * 3 + c], q2 = Q[2 * 3 + c];
+            const float2 u0 = raw[ly * 3 + 0], u1 = raw[ly * 3 + 1], u2 = raw[ly * 3 + 2];
             Z[ly][c] = make_float2(q0 * u0.x + q1 * u1.x + q2 * u2.x, q0 * u0.y + q1 * u1.y + q2 * u2.y);
         }
-    }
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
         const float q0 = Q[0 * 3 + b], q1 = Q[1 * 3 + b], q2 = Q[2 * 3 + b];
@@ -132,8 +140,33 @@ __device__ __forceinline__ void yz_filter(const float2* __restrict__ f, int L, i
     }
 }
 
-typedef float float4_t __attribute__((ext_vector_type(4)));
+// One x plane: x-filter of the three yz-filtered planes and 54 MFMA updates.
+template <bool CROSS>
+__device__ __forceinline__ void gram_plane(const float2 Yc[9], const float2 Y1[9], const float2 Y2[9],
+                                           const float2 Vc[9], const float2 V1[9], const float2 V2[9],
+                                           const float* Q, const float* lam, float dval, float vsign,
+                                           float4_t& accr, float4_t& acci) {
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float q0 = Q[0 * 3 + a], q1 = Q[1 * 3 + a], q2 = Q[2 * 3 + a];
+#pragma unroll
+        for (int bc = 0; bc < 9; ++bc) {
+            const float2 ut = make_float2(q0 * Yc[bc].x + q1 * Y1[bc].x + q2 * Y2[bc].x,
+                                          q0 * Yc[bc].y + q1 * Y1[bc].y + q2 * Y2[bc].y);
+            float2 vt;
+            if (CROSS)
+                vt = make_float2(q0 * Vc[bc].x + q1 * V1[bc].x + q2 * V2[bc].x,
+                                 vsign * (q0 * Vc[bc].y + q1 * V1[bc].y + q2 * V2[bc].y));
+            else
+                vt = ut;
+            const float dl = dval * lam[a * 9 + bc];
+            accr = __builtin_amdgcn_mfma_f32_16x16x4f32(dl * ut.x, vt.x, accr, 0, 0, 0);
+            acci = __builtin_amdgcn_mfma_f32_16x16x4f32(dl * ut.y, vt.y, acci, 0, 0, 0);
+        }
+    }
+}
 
+template <bool CROSS>
 __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __restrict__ d1,
                                                               const float2* __restrict__ U,
                                                               const float2* __restrict__ V, GramArgs g,
@@ -148,76 +181,57 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
     const int xs = blockIdx.x / (nzg * nyg);
     const int y = yg * GRAM_WAVES + wave;
     const int z = zg * 4 + k;
-    const int x0 = xs * g.xseg;
+    const int x0 = xs * g.xseg, xend = x0 + g.xseg;
     const int64_t N = (int64_t)L * L * L;
     const bool ua = s < g.S1;
     const bool vb = s < g.S2;
     const float2* fu = U + (int64_t)(ua ? s : 0) * N;
-    const float2* fv = (g.cross ? V : U) + (int64_t)(vb ? s : 0) * N;
-    float Q[9];
+    const float2* fv = (CROSS ? V : U) + (int64_t)(vb ? s : 0) * N;
+    float Q[9], lam[27];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Q[i] = g.Q[i];
+#pragma unroll
+    for (int i = 0; i < 27; ++i) lam[i] = g.lam[i];
+    const float2 zero = make_float2(0.f, 0.f);
 
-    float2 Yu[3][9], Yv[3][9];  // sliding window over x: [0] = x, [1] = x-1, [2] = x-2
+    float2 rawU[9], rawV[9];
+    float2 P0[9], P1[9], P2[9], W0[9], W1[9], W2[9];
+    // warm-up: P1 = Y(x0-2), P2 = Y(x0-1)
 #pragma unroll
-    for (int w = 1; w <= 2; ++w) {
-        const int xx = (x0 - w) & (L - 1);
-        if (ua) yz_filter(fu, L, xx, y, z, Q, Yu[w]);
-        else
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Yu[w][i] = make_float2(0.f, 0.f);
-        if (g.cross) {
-            if (vb) yz_filter(fv, L, xx, y, z, Q, Yv[w]);
-            else
-#pragma unroll
-                for (int i = 0; i < 9; ++i) Yv[w][i] = make_float2(0.f, 0.f);
-        }
-    }
+    for (int i = 0; i < 9; ++i) { rawU[i] = zero; rawV[i] = zero; }
+    if (ua) load_plane(fu, L, (x0 - 2) & (L - 1), y, z, rawU);
+    if (CROSS && vb) load_plane(fv, L, (x0 - 2) & (L - 1), y, z, rawV);
+    filter_plane(rawU, Q, P1);
+    if (CROSS) filter_plane(rawV, Q, W1);
+    if (ua) load_plane(fu, L, (x0 - 1) & (L - 1), y, z, rawU);
+    if (CROSS && vb) load_plane(fv, L, (x0 - 1) & (L - 1), y, z, rawV);
+    filter_plane(rawU, Q, P2);
+    if (CROSS) filter_plane(rawV, Q, W2);
+    if (ua) load_plane(fu, L, x0, y, z, rawU);
+    if (CROSS && vb) load_plane(fv, L, x0, y, z, rawV);
+
     double acc64[4] = {0.0, 0.0, 0.0, 0.0};
-    for (int xi = 0; xi < g.xseg; ++xi) {
-        const int x = x0 + xi;
-        if (ua) yz_filter(fu, L, x, y, z, Q, Yu[0]);
-        else
-#pragma unroll
-            for (int i = 0; i < 9; ++i) Yu[0][i] = make_float2(0.f, 0.f);
-        if (g.cross) {
-            if (vb) yz_filter(fv, L, x, y, z, Q, Yv[0]);
-            else
-#pragma unroll
-                for (int i = 0; i < 9; ++i) Yv[0][i] = make_float2(0.f, 0.f);
-        }
-        const float dval = d1[((int64_t)x * L + y) * L + z];
-        float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-        for (int a = 0; a < 3; ++a) {
-            const float q0 = Q[0 * 3 + a], q1 = Q[1 * 3 + a], q2 = Q[2 * 3 + a];
-#pragma unroll
-            for (int bc = 0; bc < 9; ++bc) {
-                const float2 ut = make_float2(q0 * Yu[0][bc].x + q1 * Yu[1][bc].x + q2 * Yu[2][bc].x,
-                                              q0 * Yu[0][bc].y + q1 * Yu[1][bc].y + q2 * Yu[2][bc].y);
-                float2 vt;
-                if (g.cross)
-                    vt = make_float2(q0 * Yv[0][bc].x + q1 * Yv[1][bc].x + q2 * Yv[2][bc].x,
-                                     g.vsign * (q0 * Yv[0][bc].y + q1 * Yv[1][bc].y + q2 * Yv[2][bc].y));
-                else
-                    vt = ut;
-                const float dl = dval * g.lam[a * 9 + bc];
-                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(dl * ut.x, vt.x, accr, 0, 0, 0);
-                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(dl * ut.y, vt.y, acci, 0, 0, 0);
-            }
-        }
-#pragma unroll
-        for (int j = 0; j < 4; ++j) acc64[j] += (double)accr[j] + (double)acci[j];
-#pragma unroll
-        for (int i = 0; i < 9; ++i) {
-            Yu[2][i] = Yu[1][i];
-            Yu[1][i] = Yu[0][i];
-            if (g.cross) {
-                Yv[2][i] = Yv[1][i];
-                Yv[1][i] = Yv[0][i];
-            }
-        }
+    // each step: filter the prefetched plane x, prefetch plane x+1, then the MFMA work of plane x
+#define GRAM_STEP(CUR, PREV1, PREV2, WCUR, WPREV1, WPREV2)                                           \
+    if (x < xend) {                                                                                  \
+        filter_plane(rawU, Q, CUR);                                                                  \
+        if (CROSS) filter_plane(rawV, Q, WCUR);                                                      \
+        const float dval = d1[((int64_t)x * L + y) * L + z];                                         \
+        if (x + 1 < xend) {                                                                          \
+            if (ua) load_plane(fu, L, x + 1, y, z, rawU);                                            \
+            if (CROSS && vb) load_plane(fv, L, x + 1, y, z, rawV);                                   \
+        }                                                                                            \
+        float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};                           \
+        gram_plane<CROSS>(CUR, PREV1, PREV2, WCUR, WPREV1, WPREV2, Q, lam, dval, g.vsign, accr, acci); \
+        _Pragma("unroll") for (int j = 0; j < 4; ++j) acc64[j] += (double)accr[j] + (double)acci[j];  \
+        ++x;                                                                                         \
     }
+    for (int x = x0; x < xend;) {
+        GRAM_STEP(P0, P2, P1, W0, W2, W1)
+        GRAM_STEP(P1, P0, P2, W1, W0, W2)
+        GRAM_STEP(P2, P1, P0, W2, W1, W0)
+    }
+#undef GRAM_STEP
     // lane holds G[4*(lane/16) + j][lane % 16]
 #pragma unroll
     for (int j = 0; j < 4; ++j) red[wave][(4 * (lane / 16) + j) * 16 + (lane % 16)] = acc64[j];
@@ -288,7 +302,10 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
     const int nblocks = (L / 4) * (L / GRAM_WAVES) * (L / g.xseg);
     double* ws = (double*)hermes_workspace(ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
-    k_gram_f32<<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
+    if (g.cross)
+        k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
+    else
+        k_gram_f32<false><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     HERMES_LAUNCH_CHECK();
     k_sum_pairs<<<1, 256, 0, ctx->stream>>>(ws, nblocks, 256, ws + (size_t)nblocks * 256);
     HERMES_LAUNCH_CHECK();

@@ -21,8 +21,8 @@ struct Plan {
     void* work = nullptr;
 };
 
-// key: (L, kind, double, batch, inplace)
-using PlanKey = std::tuple<int, int, int, int, int>;
+// key: (L, kind, double, batch, inplace, interleaved batch)
+using PlanKey = std::tuple<int, int, int, int, int, int>;
 }  // namespace
 
 struct hermes_fft_cache {
@@ -41,10 +41,12 @@ void hermes_fft_cache_destroy(hermes_fft_cache* c) {
 
 namespace {
 // kind: 0 = r2c forward, 1 = c2r inverse, 2 = c2c forward, 3 = c2c inverse
-int get_plan(hermes_ctx* ctx, int L, int kind, int dbl, int batch, int inplace, Plan** out) {
+// ibatch != 0: complex transforms whose batch index is the fastest axis
+// (element stride = batch, batch distance = 1), i.e. data laid out (L, L, L, batch).
+int get_plan(hermes_ctx* ctx, int L, int kind, int dbl, int batch, int inplace, int ibatch, Plan** out) {
     std::call_once(g_rocfft_once, []() { rocfft_setup(); });
     if (!ctx->fft) ctx->fft = new hermes_fft_cache();
-    PlanKey key{L, kind, dbl, batch, inplace};
+    PlanKey key{L, kind, dbl, batch, inplace, ibatch};
     auto it = ctx->fft->plans.find(key);
     if (it != ctx->fft->plans.end()) {
         *out = &it->second;
@@ -58,7 +60,16 @@ int get_plan(hermes_ctx* ctx, int L, int kind, int dbl, int batch, int inplace, 
                                            : rocfft_transform_type_complex_inverse;
     rocfft_precision prec = dbl ? rocfft_precision_double : rocfft_precision_single;
     rocfft_result_placement pl = inplace ? rocfft_placement_inplace : rocfft_placement_notinplace;
-    rocfft_status s = rocfft_plan_create(&p.plan, pl, tt, prec, 3, lengths, (size_t)batch, nullptr);
+    rocfft_plan_description desc = nullptr;
+    if (ibatch) {
+        rocfft_plan_description_create(&desc);
+        size_t strides[3] = {(size_t)batch, (size_t)batch * L, (size_t)batch * L * L};
+        rocfft_plan_description_set_data_layout(desc, rocfft_array_type_complex_interleaved,
+                                                rocfft_array_type_complex_interleaved, nullptr, nullptr, 3, strides, 1,
+                                                3, strides, 1);
+    }
+    rocfft_status s = rocfft_plan_create(&p.plan, pl, tt, prec, 3, lengths, (size_t)batch, desc);
+    if (desc) rocfft_plan_description_destroy(desc);
     if (s != rocfft_status_success) {
         hermes_set_error("rocfft_plan_create failed (status %d) L=%d kind=%d dbl=%d batch=%d", (int)s, L, kind,
                          dbl, batch);
@@ -78,7 +89,7 @@ int get_plan(hermes_ctx* ctx, int L, int kind, int dbl, int batch, int inplace, 
     return 0;
 }
 
-int run(hermes_ctx* ctx, int J, int kind, int dbl, int batch, const void* in, void* out) {
+int run(hermes_ctx* ctx, int J, int kind, int dbl, int batch, const void* in, void* out, int ibatch = 0) {
     HERMES_REQUIRE(ctx, "null context");
     HERMES_REQUIRE(J >= 1 && J <= 10, "FFT: J=%d unsupported", J);
     HERMES_REQUIRE(batch >= 1, "FFT: batch must be >= 1");
@@ -86,7 +97,7 @@ int run(hermes_ctx* ctx, int J, int kind, int dbl, int batch, const void* in, vo
     int inplace = (in == out) ? 1 : 0;
     HERMES_REQUIRE(!(inplace && (kind == 0 || kind == 1)), "real<->complex FFTs must be out of place");
     Plan* p = nullptr;
-    int rc = get_plan(ctx, 1 << J, kind, dbl, batch, inplace, &p);
+    int rc = get_plan(ctx, 1 << J, kind, dbl, batch, inplace, ibatch, &p);
     if (rc) return rc;
     rocfft_execution_info_set_stream(p->info, ctx->stream);
     void* ib[1] = {const_cast<void*>(in)};

@@ -58,7 +58,8 @@ def _multipole_field_f64(geo, delta_c, R_grid, l, m):
     return spec
 
 
-def ddd_multipole_plain_f64(geo, d1, d2, d3, R12_grid, R13_grid, l_min, l_max, return_m_values=False):
+def ddd_multipole_plain_f64(geo, d1, d2, d3, R12_grid, R13_grid, l_min, l_max, return_m_values=False,
+                            progress=None):
     """One (r12, r13) set, every (l, m) recomputed from scratch as the reference does.  d*: (L,L,L) float64."""
     N = geo.L ** 3
     d2c = torch.complex(d2, torch.zeros_like(d2))
@@ -75,24 +76,59 @@ def ddd_multipole_plain_f64(geo, d1, d2, d3, R12_grid, R13_grid, l_min, l_max, r
             del F1, F2
         mvals[l] = vm
         out.append(combine_m_terms([v.real for v in vm], l))
+        if progress is not None:
+            progress(l)
     out = np.asarray(out, dtype=np.float64)
     return (out, mvals) if return_m_values else out
 
 
 # ---------------------------------------------------------------- batched FP32
+class StageTimer:
+    """HIP-event timing of pipeline stages on torch's current stream (the stream the library uses)."""
+
+    def __init__(self):
+        self.events = {}
+
+    def start(self, name):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e0.record()
+        return (name, e0)
+
+    def stop(self, token):
+        name, e0 = token
+        e1 = torch.cuda.Event(enable_timing=True)
+        e1.record()
+        self.events.setdefault(name, []).append((e0, e1))
+
+    def summary(self):
+        """{stage: (total_ms, count)} -- call after synchronising."""
+        return {k: (sum(a.elapsed_time(b) for a, b in v), len(v)) for k, v in self.events.items()}
+
+
+class _NoTimer:
+    def start(self, name):
+        return None
+
+    def stop(self, token):
+        pass
+
+
 def _shell_blocks(n, size=16):
     return [(i, min(n, i + size)) for i in range(0, n, size)]
 
 
-def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min, l_max, timings=None):
+def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min, l_max, timer=None):
     """All (r12, r13) shell pairs: returns (n12, n13, n_l) float64.  d*: (L,L,L) float32 device tensors."""
     L, J = geo.L, geo.J
+    tm = timer if timer is not None else _NoTimer()
     N = L ** 3
     r12 = np.asarray(radii12_grid, dtype=np.float64)
     r13 = np.asarray(radii13_grid, dtype=np.float64)
     auto = (d3 is d2) and r12.shape == r13.shape and np.array_equal(r12, r13)
+    t = tm.start("fft_r2c")
     H2 = kernels.rfftn(d2)
     H3 = H2 if d3 is d2 else kernels.rfftn(d3)
+    tm.stop(t)
     n_l = l_max - l_min + 1
     result = np.zeros((r12.size, r13.size, n_l), dtype=np.float64)
     blocks12 = _shell_blocks(r12.size)
@@ -102,9 +138,14 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     for li, l in enumerate(range(l_min, l_max + 1)):
         v_re = []
         for m in range(l + 1):
+            t = tm.start("window_apply")
             kernels.multipole_apply_f32(H2, geo.P, r12, l, m, 1.0 / N, out=U)
+            tm.stop(t)
+            t = tm.start("fft_c2c_inv")
             kernels.fftn_c2c(U, inverse=True, out=U)
+            tm.stop(t)
             G = np.zeros((r12.size, r13.size))
+            t = tm.start("gram")
             if auto:
                 sign = (-1.0) ** (l + m)
                 for a0, a1 in blocks12:
@@ -115,11 +156,18 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
                             g = kernels.multipole_gram_f32(d1, U[a0:a1], geo.Q, geo.lam, V=U[b0:b1], v_conj=True)
                         G[a0:a1, b0:b1] = sign * g
             else:
+                tm.stop(t)
+                t = tm.start("window_apply")
                 kernels.multipole_apply_f32(H3, geo.P, r13, l, -m, 1.0 / N, out=V)
+                tm.stop(t)
+                t = tm.start("fft_c2c_inv")
                 kernels.fftn_c2c(V, inverse=True, out=V)
+                tm.stop(t)
+                t = tm.start("gram")
                 for a0, a1 in blocks12:
                     for b0, b1 in blocks13:
                         G[a0:a1, b0:b1] = kernels.multipole_gram_f32(d1, U[a0:a1], geo.Q, geo.lam, V=V[b0:b1])
+            tm.stop(t)
             v_re.append(4.0 * np.pi * G / N)
         result[:, :, li] = combine_m_terms(v_re, l)
     return result
