@@ -146,6 +146,69 @@ int hermes_multipole_gram_f32(hermes_ctx *ctx, int J, const float *d1, const voi
  * max_abs_err = 0 on a conforming device.  (No counterpart in the reference.) */
 int hermes_mfma_selftest(hermes_ctx *ctx, double *max_abs_err);
 
+/* ---- Monte-Carlo 3PCF ----
+ * Interpolation n(x) = sum_27 grid[(c - s) & (L-1)] phi phi phi at scaled
+ * positions spos (n, 3) + (dx, dy, dz), FP64.  Replaces
+ * interpolate_grid_at_pos_numba, wavelet_grid.py:143-186. */
+int hermes_interpolate(hermes_ctx *ctx, int J, const double *grid, const double *spos, int64_t n, const double *phi,
+                       int phi_res, double dx, double dy, double dz, double *out);
+
+/* Box-random-centre triplet estimator for one (R1, R2, mu) (grid units):
+ * sum_rot sum_i n1[i] n2(c_i + o2) n3(c_i + o3) / (n_rot n), with the per-rotation
+ * uniforms (a, b, c) given on the host (n_rot x 3).  Replaces
+ * estimate_triplet_product_box_random_centers, corr3pcf_kernels.py:100-140. */
+int hermes_mc_triplet_box(hermes_ctx *ctx, int J, double R1, double R2, double mu, const double *spos,
+                          const double *n1, int64_t n, int n_rot, const double *abc_host, const double *g2,
+                          const double *g3, const double *phi, int phi_res, double *out_host);
+
+/* Importance proposal tables for hermes_mc_binned (device arrays): cells are
+ * grouped into nbucket density buckets; bucket k is drawn with CDF bucket_cdf,
+ * a cell uniformly from bucket_cells[bucket_off[k] .. bucket_off[k+1]), a point
+ * uniformly in the cell, and the (r1, r2, mu) bin from bin_cdf[k]; the density
+ * relative to the uniform one is bucket_density[k] * bin_pdf[k][bin].  With
+ * probability alpha a sample is drawn from the uniform proposal instead. */
+typedef struct {
+    const uint8_t *cell_bucket;     /* (L^3) bucket of each cell */
+    const int32_t *bucket_cells;    /* (L^3) cells ordered by bucket */
+    const int64_t *bucket_off;      /* (nbucket + 1) */
+    const float *bucket_cdf;        /* (nbucket) */
+    const double *bucket_density;   /* (nbucket) V B P(k) / n_k */
+    const float *bin_cdf;           /* (nbucket, B) */
+    const float *bin_pdf;           /* (nbucket, B), sums to 1 over bins */
+    int32_t nbucket;
+    int32_t pad;
+    double alpha;
+} hermes_mc_proposal;
+
+/* Binned Monte-Carlo triangle sampler (no reference counterpart: the reference
+ * estimates one fixed triangle per call).  Triangles (r1, r2, mu) in
+ * nr1 x nr2 x nmu uniform bins over [rmin, rmax) (grid units) and [-1, 1), centres
+ * in the box, orientations uniform; f = n1 n2 n3 from grids g1, g2, g3.  prop == NULL
+ * draws everything uniformly; otherwise the importance mixture above with weights
+ * w = p / q.  Outputs per bin: sum w f, sum (w f)^2 and counts (deterministic).
+ * wf_bound must bound |w f| for every sample.  record (n_record x 6 floats:
+ * r1, r2, mu, g1[cell], f, bin) is optional. */
+int hermes_mc_binned(hermes_ctx *ctx, int J, const double *g1, const double *g2, const double *g3, const double *phi,
+                     int phi_res, int nr1, int nr2, int nmu, double rmin1, double rmax1, double rmin2, double rmax2,
+                     int64_t n_samples, uint64_t seed, double wf_bound, const hermes_mc_proposal *prop, float *record,
+                     int64_t n_record, double *sum_wf_host, double *sum_wf2_host, int64_t *count_host);
+
+/* ---- learned importance proposal: 3-layer MLP (4 -> 64 -> 64 -> 1, SiLU),
+ * bf16 weights for the forward pass, FP32 accumulation and FP32 master weights.
+ * No counterpart in the reference.  Parameter layout (FP32, hermes_mlp_nparam()):
+ * W1 (64x4) | b1 (64) | W2 (64x64) | b2 (64) | w3 (64) | b3 (1). */
+int hermes_mlp_nparam(void);
+int hermes_mlp_to_bf16(hermes_ctx *ctx, const float *w, void *wb);
+/* out[i] = s(x_i), x (n, 4) float. */
+int hermes_mlp_forward(hermes_ctx *ctx, const void *wb, const float *x, int n, float *out);
+/* Mean gradient of l = y exp(-s) + s over the batch (dl/ds clipped to [-clip, 1]);
+ * grad_out has nparam + 1 floats, the last one the mean loss.  Deterministic. */
+int hermes_mlp_grad(hermes_ctx *ctx, const void *wb, const float *x, const float *y, int n, float clip,
+                    float *grad_out);
+/* One Adam step on the FP32 weights w with moments m, v; refreshes the bf16 copy wb. */
+int hermes_mlp_adam(hermes_ctx *ctx, float *w, float *m, float *v, const float *grad, void *wb, float lr,
+                    float beta1, float beta2, float eps, int step);
+
 #ifdef __cplusplus
 }
 #endif

@@ -53,7 +53,28 @@ _SIGNATURES.update({
     "hermes_multipole_gram_f32": [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, _dpp, _dpp,
                                   _dpp],
     "hermes_mfma_selftest": [c_void_p, _dpp],
+    "hermes_interpolate": [c_void_p, c_int, c_void_p, c_void_p, c_i64, c_void_p, c_int, c_double, c_double, c_double,
+                           c_void_p],
+    "hermes_mc_triplet_box": [c_void_p, c_int, c_double, c_double, c_double, c_void_p, c_void_p, c_i64, c_int, _dpp,
+                              c_void_p, c_void_p, c_void_p, c_int, _dpp],
+    "hermes_mc_binned": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double,
+                         c_double, c_double, c_double, c_i64, ctypes.c_uint64, c_double, c_void_p, c_void_p, c_i64,
+                         _dpp, _dpp, ctypes.POINTER(ctypes.c_int64)],
+    "hermes_mlp_nparam": [],
+    "hermes_mlp_to_bf16": [c_void_p, c_void_p, c_void_p],
+    "hermes_mlp_forward": [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
+    "hermes_mlp_grad": [c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_float, c_void_p],
+    "hermes_mlp_adam": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
+                        ctypes.c_float, ctypes.c_float, c_int],
 })
+
+
+class MCProposal(ctypes.Structure):
+    """hermes_mc_proposal (include/hermes_amd.h)."""
+    _fields_ = [("cell_bucket", c_void_p), ("bucket_cells", c_void_p), ("bucket_off", c_void_p),
+                ("bucket_cdf", c_void_p), ("bucket_density", c_void_p), ("bin_cdf", c_void_p),
+                ("bin_pdf", c_void_p), ("nbucket", ctypes.c_int32), ("pad", ctypes.c_int32),
+                ("alpha", c_double)]
 _RESTYPES = {"hermes_last_error": ctypes.c_char_p}
 
 

@@ -50,6 +50,8 @@ __device__ __forceinline__ bool cell_frac(const double* pos, int64_t p, double s
         }
         c[d] = (int)floor(s);
         f[d] = (int)((s - (double)c[d]) * (double)res);
+        // s in (-2^-54, 0) rounds s - floor(s) to 1.0; keep the phi index inside the table
+        if (f[d] > res - 1) f[d] = res - 1;
     }
     return ok;
 }
@@ -160,7 +162,8 @@ __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
     double bound = (double)(end - beg) * __longlong_as_double((long long)tile_maxw[t]) * phimax3;
     int e = 0;
     frexp(bound, &e);
-    const int S = 61 - e;
+    int S = 61 - e;
+    if (S > 1000) S = 1000;  // keep 2^S finite for vanishing weights
     const double mult = ldexp(1.0, S), inv = ldexp(1.0, -S);
 
     for (int i = tid; i < ncell; i += blockDim.x) {

@@ -210,3 +210,75 @@ def mfma_selftest():
     out = np.zeros(1, dtype=np.float64)
     _lib.call("hermes_mfma_selftest", _lib.context(), out.ctypes.data_as(_DP))
     return float(out[0])
+
+
+# ---------------------------------------------------------------- Monte Carlo
+def interpolate(grid, spos, phi, offset=(0.0, 0.0, 0.0), out=None):
+    """n(x) at scaled positions (+offset) from an FP64 coefficient grid (device tensors)."""
+    L = grid.shape[0]
+    phi_t = phi if isinstance(phi, torch.Tensor) else as_device(np.asarray(phi, dtype=np.float64), torch.float64)
+    n = int(spos.shape[0])
+    if out is None:
+        out = torch.empty(n, dtype=torch.float64, device=grid.device)
+    _lib.call("hermes_interpolate", _lib.context(), _J_of(L), ptr(grid), ptr(spos), n, ptr(phi_t), 1024,
+              float(offset[0]), float(offset[1]), float(offset[2]), ptr(out))
+    return out
+
+
+def mc_triplet_box(R1, R2, mu, spos, n1, abc, grid2, grid3, phi):
+    """Box-random-centre triplet estimate for one (R1, R2, mu) (grid units)."""
+    L = grid2.shape[0]
+    phi_t = phi if isinstance(phi, torch.Tensor) else as_device(np.asarray(phi, dtype=np.float64), torch.float64)
+    abc = _np_f64(abc)
+    out = np.zeros(1, dtype=np.float64)
+    _lib.call("hermes_mc_triplet_box", _lib.context(), _J_of(L), float(R1), float(R2), float(mu), ptr(spos), ptr(n1),
+              int(spos.shape[0]), int(abc.shape[0]), abc.ctypes.data_as(_DP), ptr(grid2), ptr(grid3), ptr(phi_t), 1024,
+              out.ctypes.data_as(_DP))
+    return float(out[0])
+
+
+def mc_binned(g1, g2, g3, phi, nbins, rrange1, rrange2, n_samples, seed, wf_bound, proposal=None, record=None):
+    """Binned triangle sampler; returns (sum_wf, sum_wf2, counts) as (nr1, nr2, nmu) host arrays."""
+    L = g1.shape[0]
+    phi_t = phi if isinstance(phi, torch.Tensor) else as_device(np.asarray(phi, dtype=np.float64), torch.float64)
+    nr1, nr2, nmu = (int(v) for v in nbins)
+    B = nr1 * nr2 * nmu
+    swf = np.zeros(B)
+    swf2 = np.zeros(B)
+    cnt = np.zeros(B, dtype=np.int64)
+    prop = ctypes.byref(proposal) if proposal is not None else None
+    rec_ptr = ptr(record) if record is not None else None
+    n_rec = int(record.shape[0]) if record is not None else 0
+    _lib.call("hermes_mc_binned", _lib.context(), _J_of(L), ptr(g1), ptr(g2), ptr(g3), ptr(phi_t), 1024, nr1, nr2,
+              nmu, float(rrange1[0]), float(rrange1[1]), float(rrange2[0]), float(rrange2[1]), int(n_samples),
+              int(seed) & 0xFFFFFFFFFFFFFFFF, float(wf_bound), prop, rec_ptr, n_rec, swf.ctypes.data_as(_DP),
+              swf2.ctypes.data_as(_DP), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    shape = (nr1, nr2, nmu)
+    return swf.reshape(shape), swf2.reshape(shape), cnt.reshape(shape)
+
+
+# ---------------------------------------------------------------- proposal MLP
+def mlp_nparam():
+    return int(_lib.load().hermes_mlp_nparam())
+
+
+def mlp_to_bf16(w, wb):
+    _lib.call("hermes_mlp_to_bf16", _lib.context(), ptr(w), ptr(wb))
+
+
+def mlp_forward(wb, x, out=None):
+    n = int(x.shape[0])
+    if out is None:
+        out = torch.empty(n, dtype=torch.float32, device=x.device)
+    _lib.call("hermes_mlp_forward", _lib.context(), ptr(wb), ptr(x), n, ptr(out))
+    return out
+
+
+def mlp_grad(wb, x, y, clip, grad_out):
+    _lib.call("hermes_mlp_grad", _lib.context(), ptr(wb), ptr(x), ptr(y), int(x.shape[0]), float(clip), ptr(grad_out))
+    return grad_out
+
+
+def mlp_adam(w, m, v, grad, wb, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
+    _lib.call("hermes_mlp_adam", _lib.context(), ptr(w), ptr(m), ptr(v), ptr(grad), ptr(wb), float(lr), float(beta1),
+              float(beta2), float(eps), int(step))

@@ -1,0 +1,158 @@
+"""Parameter files and defaults (mirrors pyhermes.param.parambase).
+
+``read_param(path)`` reads YAML or JSON (with // and /* */ comments and trailing
+commas, the JSON5 subset the PyHermes configs use); ``ParamBase.recursive_update``
+merges user parameters over the per-package ``default_params.json`` with the
+same replacement keys and type checks as parambase.py:16-23, :116-205.
+"""
+
+import argparse
+import copy
+import importlib
+import inspect
+import json
+import os
+import re
+
+import yaml
+
+from ..utils import func_util
+from ..utils.mpi_util import MPI
+from .logbase import setup_logger
+
+REPLACE_KEYS = {
+    "Corr_2PCF.binning_window",
+    "Corr_2PCF.sampling",
+    "Corr_3PCF_Multipole.binning_window12",
+    "Corr_3PCF_Multipole.binning_window13",
+    "Corr_3PCF_Multipole.sampling",
+    "Corr_3PCF_Multipole.sample_mpi",
+}
+
+_COMMENT_RE = re.compile(r'"(?:\\.|[^"\\])*"|//[^\n]*|/\*.*?\*/', re.S)
+_TRAILING_COMMA_RE = re.compile(r",(\s*[}\]])")
+
+
+def loads_json_relaxed(text):
+    """json.loads after removing comments (outside strings) and trailing commas."""
+    stripped = _COMMENT_RE.sub(lambda m: m.group(0) if m.group(0).startswith('"') else "", text)
+    return json.loads(_TRAILING_COMMA_RE.sub(r"\1", stripped))
+
+
+def read_param(config_path=None):
+    if not config_path:
+        config_path = ParamBase.get_parser().parse_args().config
+    return ParamBase(config_file_path=config_path).read_config()
+
+
+class ParamBase(object):
+    def __init__(self, config_file_path=None):
+        self.default_params = {}
+        self.logger = setup_logger(__name__, self.__class__.__name__)
+        self.config_file_path = config_file_path
+        self.comm = MPI.COMM_WORLD
+        self.rank = self.comm.Get_rank()
+
+    @classmethod
+    def get_parser(cls):
+        parser = argparse.ArgumentParser(description="pyhermes_amd task runner")
+        parser.add_argument("-c", "--config", type=str, default="", help="path for the parameter file", metavar="")
+        return parser
+
+    def read_paramfile(self, file_path):
+        try:
+            with open(file_path) as f:
+                content = f.read()
+        except FileNotFoundError:
+            self.logger.error(f"Parameter file not found: '{file_path}'.")
+            func_util.safe_exit(1)
+        try:
+            return loads_json_relaxed(content)
+        except ValueError:
+            pass
+        try:
+            return yaml.safe_load(content)
+        except yaml.YAMLError as exc:
+            self.logger.error(f"Cannot determine file type: {exc}")
+            self.logger.error("Support parameter file formats: <JSON> and <YAML>")
+            func_util.safe_exit(1)
+
+    def _is_type_compatible(self, full_key, default_value, new_value):
+        if isinstance(new_value, type(default_value)):
+            return True
+        if isinstance(default_value, (int, float)) and isinstance(new_value, (int, float)):
+            return True
+        seq = (str, list, tuple)
+        if full_key.endswith(".products"):
+            return isinstance(default_value, seq) and isinstance(new_value, seq)
+        if full_key.endswith(".random"):
+            return default_value is None and isinstance(new_value, str)
+        if full_key.endswith((".theta", ".mu", ".s")):
+            return isinstance(default_value, (dict, list, tuple)) and isinstance(new_value, (dict, list, tuple))
+        if full_key.endswith(".binning_window"):
+            return isinstance(default_value, (dict, str)) and isinstance(new_value, (dict, str))
+        if full_key.endswith(".len_args"):
+            return isinstance(default_value, (dict, list, tuple, str)) and isinstance(new_value, (dict, list, tuple, str))
+        if full_key.endswith(".los_args"):
+            return isinstance(default_value, (dict, list, tuple)) and isinstance(new_value, (dict, list, tuple))
+        return False
+
+    def recursive_update(self, default_dict, new_dict, parent_key="", section=None):
+        if section:
+            in_def, in_new = section in default_dict, section in new_dict
+            if in_def and in_new:
+                return self.recursive_update(default_dict[section], new_dict[section], parent_key=section)
+            if in_def:
+                self.logger.error(f"Section <{section}> found only in default parameters, but not user-input.")
+                return None
+            if in_new:
+                default_dict[section] = new_dict[section]
+                self.logger.error(f"Section <{section}> found only in user-input parameters, but not default.")
+                return None
+            self.logger.error(f"Section <{section}> found in neither dictionary.")
+            func_util.safe_exit(1)
+        for key, value in new_dict.items():
+            full_key = f"{parent_key}.{key}" if parent_key else key
+            if key not in default_dict:
+                if isinstance(value, dict):
+                    self.logger.info(f"Adding non-default level: '{full_key}'")
+                    default_dict[key] = {}
+                    self.recursive_update(default_dict[key], value, full_key)
+                else:
+                    self.logger.info(f"Adding non-default key: '{full_key}'")
+                    default_dict[key] = value
+            elif full_key in REPLACE_KEYS:
+                default_dict[key] = copy.deepcopy(value)
+            elif isinstance(value, dict) and isinstance(default_dict[key], dict):
+                self.recursive_update(default_dict[key], value, full_key)
+            else:
+                if not self._is_type_compatible(full_key, default_dict[key], value):
+                    self.logger.warning(f"Type mismatch for key: '{full_key}' !!!")
+                default_dict[key] = value
+        return None
+
+    def _read_default(self, default_fname):
+        try:
+            with open(default_fname) as f:
+                params = loads_json_relaxed(f.read())
+        except FileNotFoundError:
+            self.logger.error(f"Default parameter file not found: '{default_fname}'.")
+            func_util.safe_exit(1)
+        if params.get("enable_default_param") is not True:
+            return
+        for key in params:
+            if key != "enable_default_param" and key in self.default_params:
+                self.logger.error(f"Key '{key}' already exists in default parameters.")
+                func_util.safe_exit(1)
+        self.default_params.update(params)
+
+    def read_default(self, class_task):
+        module = importlib.import_module(class_task.__module__)
+        module_dir = os.path.dirname(inspect.getfile(module))
+        self._read_default(os.path.join(module_dir, "default_params.json"))
+
+    def read_config(self):
+        if not self.config_file_path:
+            self.logger.error("No configure file specified in pipeline")
+            func_util.safe_exit(1)
+        return self.read_paramfile(self.config_file_path)

@@ -1,0 +1,192 @@
+"""Host-side API behaviour that needs no GPU: package surface, single-process MPI,
+parameter handling, validation errors and result I/O (tests/test_no_mpi.py of the
+reference is the model for the first group)."""
+
+import numpy as np
+import pytest
+
+
+def test_package_version_and_imports():
+    import pyhermes_amd
+    import pyhermes_amd.base  # noqa: F401
+    import pyhermes_amd.theory  # noqa: F401
+    assert pyhermes_amd.__version__
+
+
+def test_package_data_present():
+    from pathlib import Path
+
+    import pyhermes_amd.base
+    import pyhermes_amd.theory
+    assert (Path(pyhermes_amd.base.__file__).parent / "default_params.json").is_file()
+    assert (Path(pyhermes_amd.theory.__file__).parent / "default_params.json").is_file()
+
+
+def test_single_process_collectives():
+    from pyhermes_amd.utils.mpi_util import MPI
+    comm = MPI.COMM_WORLD
+    assert comm.Get_rank() == 0 and comm.Get_size() == 1
+    assert comm.bcast({"value": 1}, root=0) == {"value": 1}
+    assert comm.reduce(3, op=MPI.SUM, root=0) == 3
+    assert comm.allreduce(3, op=MPI.MAX) == 3
+    assert comm.allgather(3) == [3]
+    assert comm.Split_type(MPI.COMM_TYPE_SHARED) is comm
+    assert comm.Split(0) is comm
+    assert comm.Split(MPI.UNDEFINED) is MPI.COMM_NULL
+    send = np.arange(4, dtype=np.float64)
+    red, sc = np.empty_like(send), np.empty_like(send)
+    comm.Allreduce(send, red, op=MPI.SUM)
+    comm.Scatterv([send, [4], [0], MPI.DOUBLE], sc, root=0)
+    np.testing.assert_array_equal(red, send)
+    np.testing.assert_array_equal(sc, send)
+
+
+def test_safe_exit_raises_system_exit():
+    from pyhermes_amd.utils.func_util import safe_exit
+    with pytest.raises(SystemExit) as exc:
+        safe_exit(7)
+    assert exc.value.code == 7
+
+
+def test_thick_shell_validation_and_custom_override():
+    from pyhermes_amd.io import WindowFunc
+    from pyhermes_amd.utils.window_params import normalize_binning_window_template
+    sfc = {"J": 3, "box_size": 64.0, "phi_resolution": 1024, "wavelet_mode": "db2", "wavelet_level": 10}
+
+    def custom(ki, kj, kk, R_in, R_out):
+        return np.exp(-(ki ** 2 + kj ** 2 + kk ** 2) * R_in * R_out)
+
+    params = {"type": "thick_shell", "func": custom, "len_args": {"R_in": 8.0, "R_out": 12.0}}
+    norm = normalize_binning_window_template(params)
+    assert norm["len_args"] == {"R_in": 8.0, "R_out": 12.0}
+    assert norm["los_args"] == {}
+    assert norm["kernel_mode"] == "full_rfft"
+    params["kernel_mode"] = "octant"
+    w = WindowFunc(params, sfc)
+    assert w.type == "thick_shell" and w.has_custom_func and w.func is custom
+    assert np.all(np.isfinite(w._host_custom_kernel()))
+    with pytest.raises(ValueError, match="requires finite len_args"):
+        WindowFunc({"type": "thick_shell", "len_args": {"R_in": 8.0, "R_out": 12.0}}, sfc)
+    with pytest.raises(NotImplementedError):
+        WindowFunc({"type": "cylinder", "len_args": {"R": 1.0, "H": 2.0}}, sfc)
+
+
+def test_custom_octant_kernel_equals_alias_sum(phi):
+    """A custom transfer function evaluated on the host follows the 8-alias octant fold."""
+    from oracle import scaling
+    from oracle import windows as ow
+    from pyhermes_amd.io import WindowFunc
+    sfc = {"J": 3, "box_size": 64.0, "phi_resolution": 1024, "wavelet_mode": "db2", "wavelet_level": 10}
+
+    def shell(ki, kj, kk, R):
+        q = 2 * np.pi * np.sqrt(ki * ki + kj * kj + kk * kk) * R
+        return 1.0 if q == 0 else np.sin(q) / q
+
+    w = WindowFunc({"type": "custom_shell", "func": shell, "len_args": {"R": 16.0}, "kernel_mode": "octant"}, sfc)
+    got = w._host_custom_kernel()
+    P = scaling.fourier_power_spectrum(phi, 0, 1, 8, 1024)
+    ref = ow.octant_rfft_kernel(8, P, "shell", 16.0 * 8 / 64.0)
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+
+
+def _mp_params(**over):
+    base = {"sfc_field": "unused", "random": "uniform", "l_min": 0, "l_max": 3,
+            "sampling": {"mode": "grid", "r12": {"values": [10.0, 20.0]}, "r13": {"min": 10.0, "max": 30.0, "n": 3}},
+            "binning_window12": {"type": "shell", "mapping": {"R": "r12"}},
+            "binning_window13": {"type": "shell", "mapping": {"R": "r13"}}}
+    base.update(over)
+    return {"Corr_3PCF_Multipole": base}
+
+
+def test_multipole_sampling_grid_and_paired():
+    from pyhermes_amd.theory import Corr_3PCF_Multipole
+    t = Corr_3PCF_Multipole(_mp_params())
+    assert len(t.samples) == 6
+    np.testing.assert_array_equal(t.r12, [10, 10, 10, 20, 20, 20])
+    np.testing.assert_array_equal(t.r13, [10, 20, 30, 10, 20, 30])
+    assert [w["len_args"]["R"] for w in t.binning_window13] == [10, 20, 30, 10, 20, 30]
+    assert t.summation_backend == "gpu"
+    paired = Corr_3PCF_Multipole(_mp_params(sampling={"mode": "paired", "r12": [5.0, 6.0, 7.0], "r13": 9.0}))
+    np.testing.assert_array_equal(paired.r12, [5, 6, 7])
+    np.testing.assert_array_equal(paired.r13, [9, 9, 9])
+    with pytest.raises(ValueError, match="paired"):
+        Corr_3PCF_Multipole(_mp_params(sampling={"mode": "paired", "r12": [5.0, 6.0], "r13": [1.0, 2.0, 3.0]}))
+
+
+def test_multipole_parameter_errors():
+    from pyhermes_amd.theory import Corr_3PCF_Multipole
+    with pytest.raises(ValueError, match="top-level radius"):
+        Corr_3PCF_Multipole(_mp_params(r12=10.0))
+    with pytest.raises(ValueError, match="Unsupported product"):
+        Corr_3PCF_Multipole(_mp_params(products=["nope"]))
+    with pytest.raises(ValueError, match="l_min"):
+        Corr_3PCF_Multipole(_mp_params(l_min=3, l_max=1))
+    with pytest.raises(ValueError, match="summation_backend"):
+        Corr_3PCF_Multipole(_mp_params(summation_backend="tpu"))
+    t = Corr_3PCF_Multipole(_mp_params(summation_backend="cpu"))
+    assert t.summation_backend == "gpu_fp64"
+
+
+def _3pcf_params(**over):
+    base = {"sfc_field": "unused", "random": "uniform", "r12": 10.0, "r13": 20.0, "angle_param": "theta",
+            "theta": {"theta_min": 0.0, "theta_max": 3.0, "n_theta": 4}, "n_rot": 2, "center": "box_random",
+            "n_box_centers": 10, "base_seed": 3}
+    base.update(over)
+    return {"Corr_3PCF": base}
+
+
+def test_corr3pcf_products_and_angles():
+    from pyhermes_amd.theory import Corr_3PCF
+    t = Corr_3PCF(_3pcf_params(products=["q"]))
+    assert t.products == ["Q"]
+    assert t._expanded_products() == ["Q", "zeta", "zeta_H", "delta_ddd", "rrr", "xi12", "xi13", "xi23"]
+    t._resolve_angle_sampling()
+    np.testing.assert_allclose(t.mu_arr, np.cos(np.linspace(0, 3.0, 4)))
+    with pytest.raises(NotImplementedError):
+        Corr_3PCF(_3pcf_params(center="particle"))
+    with pytest.raises(ValueError, match="monotonic"):
+        t2 = Corr_3PCF(_3pcf_params(angle_param="mu", mu=[0.1, 0.5, 0.2]))
+        t2._resolve_angle_sampling()
+
+
+def test_rotation_draws_follow_seed_rule():
+    from oracle import mc as omc
+    from pyhermes_amd.theory.corr3pcf import rotation_draws
+    for it in (0, 1, 5):
+        np.testing.assert_array_equal(rotation_draws(44, 6, it), omc.rotation_uniforms(44, 6, it))
+
+
+def test_corr2pcf_products_and_sampling():
+    from pyhermes_amd.theory import Corr_2PCF
+    from pyhermes_amd.theory.corr2pcf import expand_products, normalize_products
+    assert expand_products(normalize_products("xi")) == ["xi", "delta_dd", "rr"]
+    with pytest.raises(ValueError):
+        normalize_products(["xx"])
+    t = Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform",
+                                 "sampling": {"s": {"min": 5.0, "max": 25.0, "n": 5}}}})
+    t._resolve_sampling()
+    np.testing.assert_allclose(t.sampling_arrays["s"], [5, 10, 15, 20, 25])
+
+
+def test_result_save_load_roundtrip(tmp_path):
+    from pyhermes_amd.io import Corr2PCFData, Corr3PCFData
+    d = Corr2PCFData()
+    d.set_sampling(["s"], {"s": np.array([1.0, 2.0])})
+    d.xi = np.array([0.5, 0.25])
+    d.corr2pcf_info = {"products": ["xi"]}
+    p = str(tmp_path / "xi.pkl")
+    d.save_corr2pcf(p, overwrite=True)
+    e = Corr2PCFData()
+    e.load_corr2pcf(p)
+    np.testing.assert_array_equal(e.xi, d.xi)
+    c = Corr3PCFData()
+    c.zeta = np.arange(3.0)
+    c.mu = np.linspace(-1, 1, 3)
+    c.theta = np.arccos(c.mu)
+    c.corr3pcf_info = {"center": "box_random"}
+    q = str(tmp_path / "z.pkl")
+    c.save_corr3pcf(q, overwrite=True)
+    f = Corr3PCFData()
+    f.load_corr3pcf(q)
+    np.testing.assert_array_equal(f.zeta, c.zeta)
+    np.testing.assert_array_equal(f.mu, c.mu)

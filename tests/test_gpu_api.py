@@ -1,0 +1,217 @@
+"""GPU parity of the drop-in task API against the oracle.
+
+Each test drives the reference-shaped entry point (SFCProjection(params).run(),
+field @ WindowFunc, Corr_2PCF / Corr_3PCF / Corr_3PCF_Multipole(params).run())
+and checks the result with the oracle's restatement on the same inputs.
+"""
+
+import numpy as np
+import pytest
+
+from oracle import mc as omc
+from oracle import multipole as omul
+from oracle import projection as oproj
+from oracle import scaling
+from oracle import windows as ow
+
+pytestmark = pytest.mark.gpu
+
+
+def _catalogue(n, box, seed):
+    from pyhermes_amd.mocks import lognormal_mock
+    pos, _ = lognormal_mock(n, 16, box, seed)
+    w = np.random.default_rng(seed + 7).lognormal(0.0, 0.3, pos.shape[0])
+    return pos, w
+
+
+def _project(pos, w, J, box, norm="catalog"):
+    from pyhermes_amd.base import SFCProjection
+    task = SFCProjection({"SFCProjection": {"J": J, "box_size": box, "particle_pos": pos, "catalog_weight": w,
+                                            "weight_normalization": norm, "fout_path": ""}})
+    return task.run(save_result=False)
+
+
+def _oracle_eps(pos, w, J, box, phi, norm="catalog"):
+    pw, sums = oproj.projection_weights(w, np.ones_like(w), norm)
+    return oproj.project_c(pos, pw, phi, J, box)[0], sums
+
+
+@pytest.mark.parametrize("norm", ["catalog", "raw", "field"])
+def test_sfc_projection_task(phi, norm):
+    print(f"SFCProjection J=5 norm={norm}", flush=True)
+    box, J = 400.0, 5
+    pos, w = _catalogue(20000, box, 11)
+    field = _project(pos, w, J, box, norm)
+    ref, sums = _oracle_eps(pos, w, J, box, phi, norm)
+    got = field.epsilon
+    assert got.shape == (32, 32, 32) and got.dtype == np.float64
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert field.field_integral == pytest.approx(sums["field_integral"], rel=1e-14)
+    assert field.catalog_weight_sum == pytest.approx(sums["catalog_weight_sum"], rel=1e-14)
+    assert got.sum() == pytest.approx(sums["field_integral"], rel=1e-10)  # partition of unity
+
+
+@pytest.mark.parametrize("wt,args", [("shell", {"R": 40.0}), ("gaussian", {"R": 25.0}),
+                                     ("thick_shell", {"R": 50.0, "delta_R": 20.0})])
+def test_field_matmul_window(phi, wt, args):
+    from pyhermes_amd.io import WindowFunc
+    print(f"field @ WindowFunc {wt}", flush=True)
+    box, J = 400.0, 5
+    L = 1 << J
+    pos, w = _catalogue(20000, box, 12)
+    field = _project(pos, w, J, box)
+    out = field @ WindowFunc({"type": wt, "len_args": dict(args)}, field.sfc_info)
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    resc = {k: v * L / box for k, v in args.items()}
+    K = ow.octant_rfft_kernel(L, P, wt, *ow.window_args(wt, resc))
+    ref = ow.convolve_rfft(field.epsilon, K)
+    assert np.abs(out.epsilon - ref).max() <= 1e-12 * np.abs(ref).max()
+    assert out.field_kind == "derived_field"
+
+
+def test_corr2pcf_task(phi):
+    from pyhermes_amd.theory import Corr_2PCF
+    print("Corr_2PCF J=5 uniform random, multipoles 0,2", flush=True)
+    box, J = 400.0, 5
+    L = 1 << J
+    pos, w = _catalogue(30000, box, 13)
+    field = _project(pos, w, J, box)
+    s = np.linspace(30.0, 110.0, 5)
+    task = Corr_2PCF({"Corr_2PCF": {"sfc_field": field, "random": "uniform",
+                                    "sampling": {"s": {"min": 30.0, "max": 110.0, "n": 5}},
+                                    "products": ["xi", "dd"], "multipoles": [0, 2], "fout_path": ""}})
+    res = task.run(save_result=False)
+    eps = field.epsilon
+    rho = field.field_integral / L ** 3
+    delta = eps - rho
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    scale = (L / box) ** 6
+    dd_ref, ddd_ref = [], []
+    for sv in s:
+        K = ow.octant_rfft_kernel(L, P, "shell", sv * L / box)
+        dd_ref.append(ow.pair_product(eps, eps, K) * scale)
+        ddd_ref.append(ow.pair_product(delta, delta, K) * scale)
+    dd_ref, ddd_ref = np.array(dd_ref), np.array(ddd_ref)
+    rr_ref = rho * rho * scale
+    np.testing.assert_allclose(res.dd, dd_ref, rtol=1e-10)
+    np.testing.assert_allclose(res.delta_dd, ddd_ref, rtol=1e-9, atol=1e-12 * np.abs(dd_ref).max())
+    np.testing.assert_allclose(res.rr, rr_ref, rtol=1e-14)
+    np.testing.assert_allclose(res.xi, ddd_ref / rr_ref, rtol=1e-9, atol=1e-12)
+    # xi_0 from the j_0 Y_00 window equals the isotropic shell xi
+    np.testing.assert_allclose(res.xi_l[0], res.xi, rtol=1e-9, atol=1e-12)
+    # xi_2 against the full complex multipole window on the CPU
+    for j, sv in enumerate(s):
+        K2 = ow.radial_multipole_window(L, P, sv * L / box, 2, 0)
+        pp = np.real(np.sum(ow.convolve_complex(delta, K2) * delta)) / L ** 3 * scale
+        ref2 = -np.sqrt(4 * np.pi * 5) * pp / rr_ref
+        assert res.xi_l[1][j] == pytest.approx(ref2, rel=1e-8, abs=1e-12)
+
+
+def _multipole_case(phi, backend, products):
+    from pyhermes_amd.theory import Corr_3PCF_Multipole
+    box, J = 160.0, 4
+    pos, w = _catalogue(8000, box, 14)
+    field = _project(pos, w, J, box)
+    params = {"Corr_3PCF_Multipole": {
+        "sfc_field": field, "random": "uniform", "l_min": 0, "l_max": 2, "products": products,
+        "summation_backend": backend, "fout_path": "",
+        "sampling": {"mode": "grid", "r12": {"values": [30.0, 50.0]}, "r13": {"values": [30.0, 50.0]}},
+        "binning_window12": {"type": "shell", "mapping": {"R": "r12"}},
+        "binning_window13": {"type": "shell", "mapping": {"R": "r13"}}}}
+    return field, Corr_3PCF_Multipole(params).run(save_result=False)
+
+
+def _multipole_oracle(phi, field, samples, fieldname):
+    L = field.L
+    eps = field.epsilon
+    rho = field.field_integral / L ** 3
+    d = eps if fieldname == "ddd" else eps - rho
+    scale = (L / field.box_size) ** 9
+    rows = []
+    for r12, r13 in samples:
+        rows.append(omul.calc_ddd_multipole(d, d, d, phi, r12 * L / field.box_size, r13 * L / field.box_size, 0, 2)
+                    * scale)
+    return np.array(rows), rho ** 3 * scale
+
+
+@pytest.mark.parametrize("backend,rtol", [("gpu_fp64", 1e-9), ("gpu", 1e-4)])
+def test_corr3pcf_multipole_task(phi, backend, rtol):
+    print(f"Corr_3PCF_Multipole L=16 lmax=2 backend={backend}", flush=True)
+    field, res = _multipole_case(phi, backend, ["zeta_l", "ddd_l"])
+    samples = list(zip(res.sample_params["r12"], res.sample_params["r13"]))
+    assert samples == [(30.0, 30.0), (30.0, 50.0), (50.0, 30.0), (50.0, 50.0)]
+    ref_ddd, _ = _multipole_oracle(phi, field, samples, "ddd")
+    ref_delta, rrr0 = _multipole_oracle(phi, field, samples, "delta")
+    np.testing.assert_array_equal(res.l, [0, 1, 2])
+    for got, ref in ((res.ddd_l, ref_ddd), (res.delta_ddd_l, ref_delta)):
+        err = np.abs(got - ref).max(axis=1) / np.abs(ref).max(axis=1)
+        assert err.max() <= rtol, err
+    np.testing.assert_allclose(res.rrr_l[:, 0], rrr0, rtol=1e-13)
+    assert np.all(res.rrr_l[:, 1:] == 0.0)
+    err = np.abs(res.zeta_l - ref_delta / rrr0).max() / np.abs(ref_delta / rrr0).max()
+    assert err <= rtol
+
+
+def test_corr3pcf_box_random_task(phi):
+    from pyhermes_amd.theory import Corr_3PCF
+    print("Corr_3PCF box_random J=5", flush=True)
+    box, J = 320.0, 5
+    L = 1 << J
+    pos, w = _catalogue(30000, box, 15)
+    field = _project(pos, w, J, box)
+    seed, n_rot, n_c = 42, 5, 3000
+    task = Corr_3PCF({"Corr_3PCF": {"sfc_field": field, "random": "uniform", "r12": 60.0, "r13": 80.0,
+                                    "angle_param": "mu", "mu": {"mu_min": -0.9, "mu_max": 0.9, "n_mu": 3},
+                                    "n_rot": n_rot, "center": "box_random", "n_box_centers": n_c,
+                                    "base_seed": seed, "products": ["Q", "ddd"], "fout_path": ""}})
+    res = task.run(save_result=False)
+    eps = field.epsilon
+    rho = field.field_integral / L ** 3
+    delta = eps - rho
+    sf = L / box
+    centres = omc.box_centers(n_c, L, seed + 1000003)
+    mu = np.linspace(-0.9, 0.9, 3)
+    ddd, dddd = [], []
+    for it, m in enumerate(mu):
+        abc = omc.rotation_uniforms(seed + 1, n_rot, it)
+        ddd.append(omc.triplet_box_random(60 * sf, 80 * sf, m, centres, abc, eps, eps, eps, phi))
+        dddd.append(omc.triplet_box_random(60 * sf, 80 * sf, m, centres, abc, delta, delta, delta, phi))
+    s9 = sf ** 9
+    np.testing.assert_allclose(res.ddd, np.array(ddd) * s9, rtol=1e-10)
+    np.testing.assert_allclose(res.delta_ddd, np.array(dddd) * s9, rtol=1e-9, atol=1e-12 * np.abs(ddd).max() * s9)
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+
+    def xi(r):
+        K = ow.octant_rfft_kernel(L, P, "shell", r * sf)
+        return ow.pair_product(delta, delta, K) / rho ** 2
+
+    r23 = np.sqrt(60.0 ** 2 + 80.0 ** 2 - 2 * 60.0 * 80.0 * mu)
+    x12, x13 = xi(60.0), xi(80.0)
+    x23 = np.array([xi(r) for r in r23])
+    np.testing.assert_allclose(res.xi12, x12, rtol=1e-9)
+    np.testing.assert_allclose(res.xi13, x13, rtol=1e-9)
+    np.testing.assert_allclose(res.xi23, x23, rtol=1e-9)
+    np.testing.assert_allclose(res.rrr, rho ** 3 * s9 * np.ones(3), rtol=1e-12)
+    zeta = np.array(dddd) * s9 / (rho ** 3 * s9)
+    zh = x12 * x13 + x12 * x23 + x13 * x23
+    np.testing.assert_allclose(res.zeta, zeta, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(res.Q, zeta / zh, rtol=1e-8, atol=1e-10)
+
+
+def test_sfc_field_and_result_roundtrip(phi, tmp_path):
+    from pyhermes_amd.io import Corr3PCFMultipoleData, SFCField
+    box, J = 400.0, 4
+    pos, w = _catalogue(5000, box, 16)
+    field = _project(pos, w, J, box)
+    p = str(tmp_path / "field.pkl")
+    field.save_sfc_field(p, overwrite=True)
+    back = SFCField(data_path=p)
+    np.testing.assert_array_equal(back.epsilon, field.epsilon)
+    assert back.J == J and back.box_size == box and back.field_kind == "catalog_field"
+    _, res = _multipole_case(phi, "gpu", ["ddd_l"])
+    q = str(tmp_path / "mp.pkl")
+    res.save_corr3pcf_multipole(q, overwrite=True)
+    again = Corr3PCFMultipoleData()
+    again.load_corr3pcf_multipole(q)
+    np.testing.assert_array_equal(again.ddd_l, res.ddd_l)
+    np.testing.assert_array_equal(again.l, res.l)
