@@ -78,6 +78,8 @@ def main():
     ap.add_argument("--particles", type=int, default=2_000_000)
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--copy-probe-mib", type=int, default=0,
+                    help="after the run, copy a buffer of this many MiB once (HBM counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1", "n_gpus": args.gpus}))
@@ -174,6 +176,23 @@ def main():
     S = args.nshell
     gram_flops = 2.0 * S * S * 54 * N
     achieved = gram_flops / (gram_ms * 1e-3) / 1e12
+    summ = timer.summary()
+    n_modes = L * L * (L // 2 + 1)
+    # window apply: read the half spectrum (complex64) once, write S full complex64 shell spectra
+    win_ms = summ["window_apply"][0] / summ["window_apply"][1]
+    win_bytes = n_modes * 8 + S * N * 8
+    # inverse c2c FFT of S shells: read + write S complex64 grids (one pass is the algorithmic minimum)
+    fft_ms = summ["fft_c2c_inv"][0] / summ["fft_c2c_inv"][1]
+    fft_bytes = 2 * S * N * 8
+    if args.copy_probe_mib > 0:
+        n_el = args.copy_probe_mib * (1 << 20) // 4
+        src = torch.ones(n_el, dtype=torch.float32, device="cuda")
+        dst = torch.empty_like(src)
+        torch.cuda.synchronize()
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        log(f"copy probe: {args.copy_probe_mib} MiB read + {args.copy_probe_mib} MiB written")
+        del src, dst
     line = {
         "metric": "J=9 3PCF(l<=10) sets/s, FP32+batched vs FP64 GPU at <=1e-4 rel err; %HBM peak",
         "value": n_unique / t_fp32,
@@ -197,6 +216,10 @@ def main():
         "stages_ms_per_step": stages,
         "projection_ms": proj_ms,
         "projection_hbm_frac": ((args.particles * 32 + N * 4) / (proj_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
+        "window_apply": {"avg_launch_ms": win_ms, "bytes": win_bytes,
+                         "hbm_frac": win_bytes / (win_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+        "fft_c2c_inv": {"avg_call_ms": fft_ms, "bytes": fft_bytes,
+                        "hbm_frac": fft_bytes / (fft_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     line.update(out)
     print(json.dumps(line))

@@ -88,12 +88,39 @@ __global__ __launch_bounds__(SUM_THREADS) void k_summand_f64(int L, const double
     }
 }
 
-__global__ void k_sum_pairs(const double* __restrict__ partial, int nb, int width, double* __restrict__ out) {
-    for (int j = threadIdx.x; j < width; j += blockDim.x) {
-        double t = 0.0;
-        for (int i = 0; i < nb; ++i) t += partial[(int64_t)i * width + j];
-        out[j] = t;
+// Column sums of a row-major (nb x width) FP64 array, width a divisor of RED_THREADS.  Block b
+// sums rows [b * rows_per_block, (b + 1) * rows_per_block) in a fixed order (thread phase p takes
+// rows p, p + phases, ...; phases are then added in index order), so results are run-to-run identical.
+constexpr int RED_THREADS = 256;
+
+__global__ __launch_bounds__(RED_THREADS) void k_sum_rows(const double* __restrict__ in, int64_t nb, int width,
+                                                          int64_t rows_per_block, double* __restrict__ out) {
+    __shared__ double sh[RED_THREADS];
+    const int phases = RED_THREADS / width;
+    const int j = threadIdx.x % width, ph = threadIdx.x / width;
+    const int64_t r0 = (int64_t)blockIdx.x * rows_per_block;
+    const int64_t r1 = r0 + rows_per_block < nb ? r0 + rows_per_block : nb;
+    double t = 0.0;
+    if (ph < phases)
+        for (int64_t r = r0 + ph; r < r1; r += phases) t += in[r * width + j];
+    sh[threadIdx.x] = t;
+    __syncthreads();
+    if (threadIdx.x < width) {
+        double s = 0.0;
+        for (int p = 0; p < phases; ++p) s += sh[p * width + threadIdx.x];
+        out[(int64_t)blockIdx.x * width + threadIdx.x] = s;
     }
+}
+
+constexpr int RED_MAX_BLOCKS = 1024;
+
+// Two launches: <= RED_MAX_BLOCKS partial rows, then one row.  tmp holds RED_MAX_BLOCKS * width doubles.
+static void sum_rows(hipStream_t st, const double* in, int64_t nb, int width, double* tmp, double* out) {
+    int64_t rpb = (nb + RED_MAX_BLOCKS - 1) / RED_MAX_BLOCKS;
+    if (rpb < 16) rpb = 16;
+    const int64_t nblk = (nb + rpb - 1) / rpb;
+    k_sum_rows<<<(unsigned)nblk, RED_THREADS, 0, st>>>(in, nb, width, rpb, tmp);
+    k_sum_rows<<<1, RED_THREADS, 0, st>>>(tmp, nblk, width, nblk, out);
 }
 
 // ---------------------------------------------------------------- batched FP32 MFMA Gram
@@ -267,13 +294,14 @@ int hermes_multipole_summand_f64(hermes_ctx* ctx, int J, const double* d1, const
     const int L = 1 << J;
     const int64_t N = (int64_t)L * L * L;
     const int nb = (int)hermes_blocks(N, SUM_THREADS, 4096);
-    double* ws = (double*)hermes_workspace(ctx, WS_CONTRACT, sizeof(double) * (2 * (size_t)nb + 2 + 9));
+    double* ws = (double*)hermes_workspace(ctx, WS_CONTRACT,
+                                           sizeof(double) * (2 * (size_t)nb + 2 + 9 + 2 * RED_MAX_BLOCKS));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
     double* dG = ws + 2 * nb + 2;
     HERMES_CHECK_HIP(hipMemcpyAsync(dG, gamma3x3_host, sizeof(double) * 9, hipMemcpyHostToDevice, ctx->stream));
     k_summand_f64<<<nb, SUM_THREADS, 0, ctx->stream>>>(L, d1, (const double2*)F1, (const double2*)F2, dG, ws);
     HERMES_LAUNCH_CHECK();
-    k_sum_pairs<<<1, 64, 0, ctx->stream>>>(ws, nb, 2, ws + 2 * nb);
+    sum_rows(ctx->stream, ws, nb, 2, dG + 9, ws + 2 * nb);
     HERMES_LAUNCH_CHECK();
     HERMES_CHECK_HIP(hipMemcpyAsync(out_host, ws + 2 * nb, sizeof(double) * 2, hipMemcpyDeviceToHost, ctx->stream));
     HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
@@ -300,14 +328,15 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
         for (int b = 0; b < 3; ++b)
             for (int c = 0; c < 3; ++c) g.lam[(a * 3 + b) * 3 + c] = (float)(lam3_host[a] * lam3_host[b] * lam3_host[c]);
     const int nblocks = (L / 4) * (L / GRAM_WAVES) * (L / g.xseg);
-    double* ws = (double*)hermes_workspace(ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256));
+    double* ws = (double*)hermes_workspace(
+        ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256 + (size_t)RED_MAX_BLOCKS * 256));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
     if (g.cross)
         k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     else
         k_gram_f32<false><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     HERMES_LAUNCH_CHECK();
-    k_sum_pairs<<<1, 256, 0, ctx->stream>>>(ws, nblocks, 256, ws + (size_t)nblocks * 256);
+    sum_rows(ctx->stream, ws, nblocks, 256, ws + (size_t)nblocks * 256 + 256, ws + (size_t)nblocks * 256);
     HERMES_LAUNCH_CHECK();
     double h[256];
     HERMES_CHECK_HIP(hipMemcpyAsync(h, ws + (size_t)nblocks * 256, sizeof(double) * 256, hipMemcpyDeviceToHost,
