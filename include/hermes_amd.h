@@ -59,6 +59,19 @@ int hermes_fft_r2c(hermes_ctx *ctx, int J, int batch, int dbl, const void *in_re
 int hermes_fft_c2r(hermes_ctx *ctx, int J, int batch, int dbl, void *in_half, void *out_real);
 int hermes_fft_c2c(hermes_ctx *ctx, int J, int batch, int dbl, int inverse, const void *in, void *out);
 
+/* In-place 3-D complex64 FFT of a contiguous batch of (L, L, L) grids, L = 2^J, 3 <= J <= 10,
+ * by three LDS-tiled passes (z lines, then y, then x); unnormalised like hermes_fft_c2c.
+ * The FP32 counterpart of scipy.fft.fftn / ifftn in convolution.py:291-295. */
+int hermes_fft3_c2c_f32(hermes_ctx *ctx, int J, int batch, int inverse, void *data);
+
+/* Tile shape the autotuner chose for the strided passes of size 2^J and direction (tile width in
+ * z lines, threads per workgroup); both 0 while that size has not been transformed yet.
+ * The first transform of each size times every candidate tile once on a scratch grid.
+ * No reference counterpart (tuning of this build's own kernels). */
+int hermes_fft3_tile_choice(hermes_ctx *ctx, int J, int inverse, int *tw_out, int *threads_out);
+/* Pins the strided-pass tile (must be one of the candidates) instead of autotuning. */
+int hermes_fft3_set_tile(hermes_ctx *ctx, int J, int inverse, int tw, int threads);
+
 /* ---- window descriptors (window_functions.py:29-129, :426-438;
  *      radial_multipole_windows.py:128-144 thin-shell profile).
  * Length arguments a0, a1 are in grid units (len_args * L / box_size). */
@@ -116,6 +129,16 @@ int hermes_pair_reduce(hermes_ctx *ctx, int J, int dbl, const void *F1, const vo
  * corr3pcf_multipoles.py:267-321 for all shells at once. */
 int hermes_multipole_apply_f32(hermes_ctx *ctx, int J, const void *H, const double *P, const double *radii,
                                int n_shell, int l, int m, double scale, void *out);
+
+/* hermes_multipole_apply_f32 followed by the inverse 3-D FFT of every shell, fused:
+ *   out[s] = IFFT3(H_full * K_{R_s,l,m} * scale)   (unnormalised inverse), out (n_shell, L, L, L) complex64.
+ * One workgroup per octant representative (x, y) evaluates K once for the four mirror
+ * lines (x, y), (L-x, y), (x, L-y), (L-x, L-y), forms them from four half-spectrum reads,
+ * transforms them along z in LDS and writes each line once; the y and x passes follow.
+ * This is field @ radial_multipole window (corr3pcf_multipoles.py:267-321 with
+ * convolution.py:291-295) for all shells at once. */
+int hermes_multipole_window_ifft_f32(hermes_ctx *ctx, int J, const void *H, const double *P, const double *radii,
+                                     int n_shell, int l, int m, double scale, void *out);
 
 /* Deterministic FP64 sum of n real (ncomp = 1) or complex (ncomp = 2) values,
  * stored as float (dbl = 0) or double (dbl = 1); result to host.  Replaces the

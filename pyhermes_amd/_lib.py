@@ -30,6 +30,9 @@ _SIGNATURES = {
     "hermes_fft_r2c": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p],
     "hermes_fft_c2r": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p],
     "hermes_fft_c2c": [c_void_p, c_int, c_int, c_int, c_int, c_void_p, c_void_p],
+    "hermes_fft3_c2c_f32": [c_void_p, c_int, c_int, c_int, c_void_p],
+    "hermes_fft3_tile_choice": [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
+    "hermes_fft3_set_tile": [c_void_p, c_int, c_int, c_int, c_int],
 }
 
 
@@ -47,6 +50,8 @@ _SIGNATURES.update({
     "hermes_pair_reduce": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(WindowDesc), c_int,
                            _dpp],
     "hermes_multipole_apply_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_int, c_double,
+                                   c_void_p],
+    "hermes_multipole_window_ifft_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_int, c_double,
                                    c_void_p],
     "hermes_reduce_sum": [c_void_p, c_void_p, c_i64, c_int, c_int, _dpp],
     "hermes_multipole_summand_f64": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, _dpp, _dpp],

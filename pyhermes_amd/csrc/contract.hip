@@ -129,6 +129,8 @@ struct GramArgs {
     float vsign;      // sign applied to V's imaginary part in the cross case (-1: Re(U V))
     float Q[9];       // Q[tap * 3 + col]
     float lam[27];    // lam_a lam_b lam_c, index (a * 3 + b) * 3 + c
+    float lamyz[9];   // lam_b lam_c, index b * 3 + c (auto kernel)
+    float gx[9];      // Gamma[l1 * 3 + l2] = sum_a Q[l1][a] lam_a Q[l2][a] (auto kernel)
 };
 
 constexpr int GRAM_WAVES = 4;
@@ -236,6 +238,7 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
     if (CROSS) filter_plane(rawV, Q, W2);
     if (ua) load_plane(fu, L, x0, y, z, rawU);
     if (CROSS && vb) load_plane(fv, L, x0, y, z, rawV);
+    float dnext = d1[((int64_t)x0 * L + y) * L + z];  // d at plane x, loaded one step ahead
 
     double acc64[4] = {0.0, 0.0, 0.0, 0.0};
     // each step: filter the prefetched plane x, prefetch plane x+1, then the MFMA work of plane x
@@ -243,10 +246,11 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
     if (x < xend) {                                                                                  \
         filter_plane(rawU, Q, CUR);                                                                  \
         if (CROSS) filter_plane(rawV, Q, WCUR);                                                      \
-        const float dval = d1[((int64_t)x * L + y) * L + z];                                         \
+        const float dval = dnext;                                                                    \
         if (x + 1 < xend) {                                                                          \
             if (ua) load_plane(fu, L, x + 1, y, z, rawU);                                            \
             if (CROSS && vb) load_plane(fv, L, x + 1, y, z, rawV);                                   \
+            dnext = d1[((int64_t)(x + 1) * L + y) * L + z];                                          \
         }                                                                                            \
         float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};                           \
         gram_plane<CROSS>(CUR, PREV1, PREV2, WCUR, WPREV1, WPREV2, Q, lam, dval, g.vsign, accr, acci); \
@@ -267,6 +271,168 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
         double t = 0.0;
 #pragma unroll
         for (int w = 0; w < GRAM_WAVES; ++w) t += red[w][i];
+        partial[(int64_t)blockIdx.x * 256 + i] = t;
+    }
+}
+
+// Auto Gram (V = U): the x-direction Gamma stencil moves from the fields onto d1.  For a cell
+// column (y, z) and yz-filtered values Y_p[bc] of plane p,
+//   sum_x d(x) sum_{l1,l2} Gamma[l1][l2] Re(Y_{x-l1}(a) conj Y_{x-l2}(b))
+//     = A[a][b] + A[b][a],  A = sum_p [c0/2 Re(Y_p Y_p*) + c1 Re(Y_p Y_{p+1}*) + c2 Re(Y_p Y_{p+2}*)],
+// with c0 = sum_l Gamma[l][l] d(p+l), c1 = Gamma[1][0] d(p+1) + Gamma[2][1] d(p+2), c2 = Gamma[2][0] d(p+2)
+// (periodic sums over all planes; the transpose is added by the host after the reduction).  The
+// three terms share the left operand Y_p(a), so per bc the right operand is the combination
+// lam_bc (c0/2 Y_p + c1 Y_{p+1} + c2 Y_{p+2})(b) and one MFMA pair does the update: 18 MFMAs per
+// plane (the eigen form needs 54) and no x filter.
+//
+// Memory: the MFMA lane layout puts 16 shells (arrays N apart) in lanes 0..15, so direct loads touch
+// 16 cache lines per instruction.  Instead a workgroup of 16 waves owns a 16 (z) x 4 (y) cell tile and
+// stages each plane's halo'd tile (6 rows x 18 z x 16 shells) into LDS with loads that run along z,
+// double-buffered one plane ahead; lanes read their 3 x 3 (y, z) neighbourhoods from LDS.
+constexpr int AUTO_WAVES = 16;
+constexpr int AUTO_TY = 4, AUTO_TZ = 16;                       // cells per workgroup tile
+constexpr int AUTO_RY = AUTO_TY + 2, AUTO_RZ = AUTO_TZ + 2;   // halo'd tile
+constexpr int AUTO_PITCH = 17;                                // 16 shells + 1 pad per tile position
+constexpr int AUTO_TILE = AUTO_RY * AUTO_RZ * AUTO_PITCH;     // float2 per buffer
+constexpr int AUTO_STAGE = AUTO_RY * AUTO_RZ * 16;            // elements staged per plane
+constexpr int AUTO_SPT = (AUTO_STAGE + 64 * AUTO_WAVES - 1) / (64 * AUTO_WAVES);
+
+__global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __restrict__ d1,
+                                                               const float2* __restrict__ U, GramArgs g,
+                                                               double* __restrict__ partial) {
+    __shared__ float2 tile[2][AUTO_TILE];
+    __shared__ double red[AUTO_WAVES][256];
+    const int L = g.L, m = L - 1;
+    const int tid = threadIdx.x;
+    const int lane = tid % 64, wave = tid / 64;
+    const int s = lane % 16, k = lane / 16;
+    const int zw = wave % 4, yw = wave / 4;
+    // Workgroups are dealt to the 8 XCDs round-robin; the remap gives each XCD a contiguous range of
+    // logical blocks (x segment slowest, then columns 4 tiles wide walked along y), so co-resident
+    // workgroups share halo rows in the XCD's L2.
+    const int nzb = L / AUTO_TZ, nyb = L / AUTO_TY;
+    const int nb = nzb * nyb * (L / g.xseg);
+    const int q = (nb % 8 == 0) ? (int)(blockIdx.x % 8) * (nb / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
+    const int xs = q / (nzb * nyb);
+    const int r = q % (nzb * nyb);
+    const int PZ = nzb < 4 ? nzb : 4;
+    const int zb = (r / (PZ * nyb)) * PZ + (r % (PZ * nyb)) % PZ;
+    const int yb = (r % (PZ * nyb)) / PZ;
+    const int z0 = zb * AUTO_TZ, y0 = yb * AUTO_TY;
+    const int y = y0 + yw, z = z0 + 4 * zw + k;
+    const int x0 = xs * g.xseg, xend = x0 + g.xseg;
+    const int64_t N = (int64_t)L * L * L;
+    const int64_t LL = (int64_t)L * L;
+
+    float Q[9], lamyz[9];
+#pragma unroll
+    for (int i = 0; i < 9; ++i) {
+        Q[i] = g.Q[i];
+        lamyz[i] = g.lamyz[i];
+    }
+    const float g00 = 0.5f * g.gx[0], g11 = 0.5f * g.gx[4], g22 = 0.5f * g.gx[8];
+    const float g10 = g.gx[3], g21 = g.gx[7], g20 = g.gx[6];
+    const int64_t col = (int64_t)y * L + z;
+
+    // staging: element e = (sh * AUTO_RY + ry) * AUTO_RZ + rz (z fastest: loads run along z);
+    // padding shells (sh >= S1) read shell 0 (their result rows are never read)
+    const float2* src[AUTO_SPT];
+    int dst[AUTO_SPT];
+    bool act[AUTO_SPT];
+#pragma unroll
+    for (int i = 0; i < AUTO_SPT; ++i) {
+        const int e = tid + i * 64 * AUTO_WAVES;
+        act[i] = e < AUTO_STAGE;
+        const int ee = act[i] ? e : 0;
+        const int sh = ee / (AUTO_RY * AUTO_RZ), ry = (ee / AUTO_RZ) % AUTO_RY, rz = ee % AUTO_RZ;
+        src[i] = U + (int64_t)(sh < g.S1 ? sh : 0) * N + (int64_t)((y0 - 2 + ry) & m) * L + ((z0 - 2 + rz) & m);
+        dst[i] = (ry * AUTO_RZ + rz) * AUTO_PITCH + sh;
+    }
+    float2 stg[AUTO_SPT];
+    auto stage_load = [&](int plane) {
+#pragma unroll
+        for (int i = 0; i < AUTO_SPT; ++i)
+            if (act[i]) stg[i] = src[i][(int64_t)(plane & m) * LL];
+    };
+    auto stage_store = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < AUTO_SPT; ++i)
+            if (act[i]) tile[buf][dst[i]] = stg[i];
+    };
+    // this lane's neighbourhood: raw[ly * 3 + lz] = F(y - ly, z - lz) at tile row yw+2-ly, column 4zw+k+2-lz
+    const int me = ((yw + 2) * AUTO_RZ + (4 * zw + k + 2)) * AUTO_PITCH + s;
+    auto tile_filter = [&](int buf, float2 Y[9]) {
+        float2 raw[9];
+#pragma unroll
+        for (int ly = 0; ly < 3; ++ly)
+#pragma unroll
+            for (int lz = 0; lz < 3; ++lz) raw[ly * 3 + lz] = tile[buf][me - (ly * AUTO_RZ + lz) * AUTO_PITCH];
+        filter_plane(raw, Q, Y);
+    };
+
+    double* slot = &red[wave][(4 * (lane / 16)) * 16 + (lane % 16)];
+#pragma unroll
+    for (int j = 0; j < 4; ++j) slot[j * 16] = 0.0;
+
+    float2 Y0[9], Y1[9], Y2[9];
+    // warm-up: Y0 = plane x0, Y1 = plane x0+1, tile[(x0+2)&1] = plane x0+2, stg = plane x0+3
+    stage_load(x0);
+    stage_store(x0 & 1);
+    stage_load(x0 + 1);
+    __syncthreads();
+    tile_filter(x0 & 1, Y0);
+    stage_store((x0 + 1) & 1);
+    stage_load(x0 + 2);
+    __syncthreads();
+    tile_filter((x0 + 1) & 1, Y1);
+    __syncthreads();
+    stage_store(x0 & 1);
+    stage_load(x0 + 3);
+    float dA = d1[(int64_t)(x0 & m) * LL + col];
+    float dB = d1[(int64_t)((x0 + 1) & m) * LL + col];
+    float dC = 0.0f;
+    float dnext = d1[(int64_t)((x0 + 2) & m) * LL + col];
+
+    float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
+    // step p: (barrier) filter plane p+2 from LDS, store staged plane p+3, load plane p+4, MFMAs of plane p
+#define AUTO_STEP(YP, YP1, YN, DP, DP1, DN)                                                      \
+    if (p < xend) {                                                                            \
+        __syncthreads();                                                                       \
+        tile_filter((p + 2) & 1, YN);                                                          \
+        stage_store((p + 3) & 1);                                                              \
+        stage_load(p + 4);                                                                     \
+        DN = dnext;                                                                            \
+        dnext = d1[(int64_t)((p + 3) & m) * LL + col];                                         \
+        const float c0 = g00 * DP + g11 * DP1 + g22 * DN;                                      \
+        const float c1 = g10 * DP1 + g21 * DN;                                                 \
+        const float c2 = g20 * DN;                                                             \
+        _Pragma("unroll") for (int bc = 0; bc < 9; ++bc) {                                     \
+            const float t0 = c0 * lamyz[bc], t1 = c1 * lamyz[bc], t2 = c2 * lamyz[bc];         \
+            const float2 yp = YP[bc];                                                          \
+            const float bx = t0 * yp.x + t1 * YP1[bc].x + t2 * YN[bc].x;                       \
+            const float by = t0 * yp.y + t1 * YP1[bc].y + t2 * YN[bc].y;                       \
+            accr = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.x, bx, accr, 0, 0, 0);              \
+            acci = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.y, by, acci, 0, 0, 0);              \
+        }                                                                                      \
+        ++p;                                                                                   \
+    }
+    for (int p = x0; p < xend;) {
+        AUTO_STEP(Y0, Y1, Y2, dA, dB, dC)
+        AUTO_STEP(Y1, Y2, Y0, dB, dC, dA)
+        AUTO_STEP(Y2, Y0, Y1, dC, dA, dB)
+#pragma unroll
+        for (int j = 0; j < 4; ++j) {
+            slot[j * 16] += (double)accr[j] + (double)acci[j];
+            accr[j] = 0.f;
+            acci[j] = 0.f;
+        }
+    }
+#undef AUTO_STEP
+    __syncthreads();
+    for (int i = tid; i < 256; i += 64 * AUTO_WAVES) {
+        double t = 0.0;
+#pragma unroll
+        for (int w = 0; w < AUTO_WAVES; ++w) t += red[w][i];
         partial[(int64_t)blockIdx.x * 256 + i] = t;
     }
 }
@@ -327,14 +493,25 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)
             for (int c = 0; c < 3; ++c) g.lam[(a * 3 + b) * 3 + c] = (float)(lam3_host[a] * lam3_host[b] * lam3_host[c]);
-    const int nblocks = (L / 4) * (L / GRAM_WAVES) * (L / g.xseg);
+    for (int b = 0; b < 3; ++b)
+        for (int c = 0; c < 3; ++c) g.lamyz[b * 3 + c] = (float)(lam3_host[b] * lam3_host[c]);
+    for (int l1 = 0; l1 < 3; ++l1)
+        for (int l2 = 0; l2 < 3; ++l2) {
+            double t = 0.0;
+            for (int a = 0; a < 3; ++a) t += Q3x3_host[l1 * 3 + a] * lam3_host[a] * Q3x3_host[l2 * 3 + a];
+            g.gx[l1 * 3 + l2] = (float)t;
+        }
+    // auto kernel: 16 z x 4 y cells per workgroup (needs L >= 16); cross kernel: 4 z x 4 y
+    HERMES_REQUIRE(g.cross || J >= 4, "hermes_multipole_gram_f32: J=%d, the auto Gram needs J >= 4", J);
+    const int nblocks = g.cross ? (L / 4) * (L / GRAM_WAVES) * (L / g.xseg)
+                                : (L / AUTO_TZ) * (L / AUTO_TY) * (L / g.xseg);
     double* ws = (double*)hermes_workspace(
         ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256 + (size_t)RED_MAX_BLOCKS * 256));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
     if (g.cross)
         k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     else
-        k_gram_f32<false><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
+        k_gram_auto<<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws);
     HERMES_LAUNCH_CHECK();
     sum_rows(ctx->stream, ws, nblocks, 256, ws + (size_t)nblocks * 256 + 256, ws + (size_t)nblocks * 256);
     HERMES_LAUNCH_CHECK();
@@ -343,7 +520,8 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
                                     ctx->stream));
     HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     for (int i = 0; i < g.S1; ++i)
-        for (int j = 0; j < g.S2; ++j) out_host[i * g.S2 + j] = h[i * 16 + j];
+        for (int j = 0; j < g.S2; ++j)
+            out_host[i * g.S2 + j] = g.cross ? h[i * 16 + j] : h[i * 16 + j] + h[j * 16 + i];
     return 0;
 }
 

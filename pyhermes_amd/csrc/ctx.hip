@@ -63,6 +63,7 @@ int hermes_ctx_destroy(hermes_ctx* ctx) {
     for (int i = 0; i < 8; ++i)
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     hermes_fft_cache_destroy(ctx->fft);
+    if (ctx->twiddles) (void)hipFree(ctx->twiddles);
     delete ctx;
     return 0;
 }

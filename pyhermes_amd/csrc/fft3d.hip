@@ -1,0 +1,296 @@
+// In-place 3-D complex FFT (FP32) of a batch of (L, L, L) grids in three LDS-tiled passes.
+//
+// Pass 1 transforms contiguous z lines (rows of L complex), passes 2 and 3 the
+// strided y and x lines; a workgroup of the strided passes stages a tile of TW
+// adjacent z columns (TW * 8 contiguous bytes per line element) by L elements,
+// so every global access is a run of TW complex values and every pass reads and
+// writes each grid value once.  Used for the batched FP32 multipole fields in
+// place of the rocFFT 3-D plan (which splits the strided axes into transposes).
+#include <cmath>
+#include <vector>
+
+#include "../../include/hermes_amd.h"
+#include "fft_lds.h"
+#include "hermes_common.h"
+
+using namespace hermes_fft;
+
+// e^{dir 2 pi i k / 2^j} for j = 0..10, k < 2^j, computed in FP64 and rounded once.
+const float2* hermes_twiddles(hermes_ctx* ctx, int log2L, int dir) {
+    if (!ctx->twiddles) {
+        std::vector<float2> h((size_t)11 * 2 * 1024);
+        for (int j = 0; j <= 10; ++j)
+            for (int d = 0; d < 2; ++d) {
+                const int L = 1 << j;
+                const double sgn = d ? 1.0 : -1.0;
+                for (int k = 0; k < L; ++k) {
+                    const double a = 2.0 * M_PI * (double)k / (double)L;
+                    h[(size_t)(j * 2 + d) * 1024 + k] = make_float2((float)cos(a), (float)(sgn * sin(a)));
+                }
+            }
+        float2* p = nullptr;
+        if (hipMalloc(&p, h.size() * sizeof(float2)) != hipSuccess) {
+            hermes_set_error("hipMalloc of the twiddle table failed");
+            return nullptr;
+        }
+        if (hipMemcpy(p, h.data(), h.size() * sizeof(float2), hipMemcpyHostToDevice) != hipSuccess) {
+            (void)hipFree(p);
+            hermes_set_error("upload of the twiddle table failed");
+            return nullptr;
+        }
+        ctx->twiddles = p;
+    }
+    return ctx->twiddles + (size_t)(log2L * 2 + (dir > 0 ? 1 : 0)) * 1024;
+}
+
+namespace {
+
+constexpr int FFT_THREADS = 256;
+
+template <int LOG2L>
+constexpr int rows_per_block() {
+    return (2048 >> LOG2L) > 0 ? (2048 >> LOG2L) : 1;
+}
+
+// Pass over contiguous lines: rows_per_block() rows of L per workgroup.
+template <int LOG2L, int DIR>
+__global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(float2* __restrict__ data, int64_t nrows,
+                                                          const float2* __restrict__ twg) {
+    constexpr int L = 1 << LOG2L;
+    constexpr int NL = rows_per_block<LOG2L>();
+    using F = LdsFft<LOG2L, NL, FFT_THREADS, DIR, false>;
+    __shared__ float2 lds[F::ELEMS];
+    __shared__ float2 tw[L];
+    const int tid = threadIdx.x;
+    const int64_t r0 = (int64_t)blockIdx.x * NL;
+    load_twiddles<LOG2L>(tw, twg, tid, FFT_THREADS);
+    float2* base = data + r0 * L;
+    for (int i = tid; i < NL * L; i += FFT_THREADS)
+        if (r0 + i / L < nrows) lds[F::addr(i / L, i % L)] = base[i];
+    __syncthreads();
+    F::run(lds, tw, tid);
+    for (int i = tid; i < NL * L; i += FFT_THREADS)
+        if (r0 + i / L < nrows) base[i] = lds[F::addr(i / L, i % L)];
+}
+
+// Pass over strided lines.  Tile t = (outer, mid, zt): lines z = zt*TW .. zt*TW+TW-1, element e at
+// base + e * axis_stride (e * axis_stride < 2^31 for L <= 1024).  LDS (dynamic) holds the L twiddles
+// and TW * L complex values, element-major.
+template <int LOG2L, int TW, int NT, int DIR>
+__global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, const float2* __restrict__ twg,
+                                                 int axis_stride, int64_t mid_stride, int n_mid,
+                                                 int64_t outer_stride, int ntile_z) {
+    constexpr int L = 1 << LOG2L;
+    using F = LdsFft<LOG2L, TW, NT, DIR, true>;
+    extern __shared__ float2 smem[];
+    float2* tw = smem;
+    float2* lds = smem + L;
+    const int tid = threadIdx.x;
+    int64_t b = blockIdx.x;
+    const int zt = (int)(b % ntile_z);
+    b /= ntile_z;
+    const int mid = (int)(b % n_mid);
+    const int64_t outer = b / n_mid;
+    float2* base = data + outer * outer_stride + (int64_t)mid * mid_stride + (int64_t)zt * TW;
+    load_twiddles<LOG2L>(tw, twg, tid, NT);
+#pragma unroll 4
+    for (int i = tid; i < TW * L; i += NT) lds[i] = base[(i / TW) * axis_stride + (i % TW)];
+    __syncthreads();
+    F::run(lds, tw, tid);
+#pragma unroll 4
+    for (int i = tid; i < TW * L; i += NT) base[(i / TW) * axis_stride + (i % TW)] = lds[i];
+}
+
+template <int LOG2L>
+constexpr int col_tile() {
+    // TW * L = 8192 complex (64 KB) for L >= 128, all z lines for smaller grids
+    return (8192 >> LOG2L) < (1 << LOG2L) ? (8192 >> LOG2L) : (1 << LOG2L);
+}
+
+// Column-pass tile shapes tried by the autotuner: (TW, threads).  Grids below 128^3 use variant 0 only.
+constexpr int N_COL_VARIANTS = 4;
+
+template <int LOG2L, int V>
+struct ColVariant {
+    static constexpr int TW = (V & 2) ? col_tile<LOG2L>() / 2 : col_tile<LOG2L>();
+    static constexpr int NT = (V & 1) ? 512 : 256;
+};
+
+template <int LOG2L, int V, int DIR>
+int launch_yx_variant(hermes_ctx* ctx, int batch, float2* data, const float2* twg) {
+    constexpr int L = 1 << LOG2L;
+    constexpr int TW = ColVariant<LOG2L, V>::TW, NT = ColVariant<LOG2L, V>::NT;
+    const size_t shmem = sizeof(float2) * (size_t)(L + TW * L);
+    auto kc = k_fft_cols<LOG2L, TW, NT, DIR>;
+    HERMES_CHECK_HIP(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
+    const int ntz = L / TW;
+    const unsigned nblk = (unsigned)((int64_t)batch * L * ntz);
+    // y lines: tiles (b * L + x, zt)
+    kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L, 0, 1, (int64_t)L * L, ntz);
+    HERMES_LAUNCH_CHECK();
+    // x lines: tiles (b, y, zt)
+    kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L * L, L, L, (int64_t)L * L * L, ntz);
+    HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+template <int LOG2L, int DIR>
+int launch_yx(hermes_ctx* ctx, int variant, int batch, float2* data, const float2* twg) {
+    if (LOG2L < 7) return launch_yx_variant<LOG2L, 0, DIR>(ctx, batch, data, twg);
+    switch (variant) {
+        case 1: return launch_yx_variant<LOG2L, 1, DIR>(ctx, batch, data, twg);
+        case 2: return launch_yx_variant<LOG2L, 2, DIR>(ctx, batch, data, twg);
+        case 3: return launch_yx_variant<LOG2L, 3, DIR>(ctx, batch, data, twg);
+        default: return launch_yx_variant<LOG2L, 0, DIR>(ctx, batch, data, twg);
+    }
+}
+
+// Autotuner: on the first strided pass of a grid size and direction, time every variant on a
+// scratch grid (median of three y+x passes, HIP events on the context stream) and keep the
+// fastest in the context.  The scratch grid is freed afterwards.  Runs eagerly, so the first
+// call of a size must not be inside stream capture.
+template <int LOG2L, int DIR>
+int tuned_variant(hermes_ctx* ctx, const float2* twg, int* out) {
+    const int d = DIR > 0 ? 1 : 0;
+    if (ctx->fft_cols_variant[LOG2L][d] > 0) {
+        *out = ctx->fft_cols_variant[LOG2L][d] - 1;
+        return 0;
+    }
+    if (LOG2L < 7) {
+        ctx->fft_cols_variant[LOG2L][d] = 1;
+        *out = 0;
+        return 0;
+    }
+    constexpr int L = 1 << LOG2L;
+    const size_t bytes = sizeof(float2) * (size_t)L * L * L;
+    float2* scratch = nullptr;
+    HERMES_CHECK_HIP(hipMalloc(&scratch, bytes));
+    HERMES_CHECK_HIP(hipMemsetAsync(scratch, 0, bytes, ctx->stream));
+    hipEvent_t e0, e1;
+    HERMES_CHECK_HIP(hipEventCreate(&e0));
+    HERMES_CHECK_HIP(hipEventCreate(&e1));
+    int best = 0;
+    float best_ms = 1e30f;
+    int rc = 0;
+    for (int v = 0; v < N_COL_VARIANTS && rc == 0; ++v) {
+        float t[3];
+        rc = launch_yx<LOG2L, DIR>(ctx, v, 1, scratch, twg);  // warm-up
+        for (int r = 0; r < 3 && rc == 0; ++r) {
+            if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = 1;
+            if (rc == 0) rc = launch_yx<LOG2L, DIR>(ctx, v, 1, scratch, twg);
+            if (rc == 0 && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = 1;
+            if (rc == 0 && hipEventSynchronize(e1) != hipSuccess) rc = 1;
+            if (rc == 0 && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = 1;
+        }
+        if (rc) break;
+        const float med = t[0] < t[1] ? (t[1] < t[2] ? t[1] : (t[0] < t[2] ? t[2] : t[0]))
+                                      : (t[0] < t[2] ? t[0] : (t[1] < t[2] ? t[2] : t[1]));
+        if (med < best_ms) {
+            best_ms = med;
+            best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(scratch);
+    if (rc) {
+        if (rc == 1) hermes_set_error("FFT autotuning failed for L=%d", L);
+        return rc;
+    }
+    ctx->fft_cols_variant[LOG2L][d] = best + 1;
+    *out = best;
+    return 0;
+}
+
+// Strided passes (y lines, then x lines) of a batch of grids with the tuned tile shape.
+template <int LOG2L, int DIR>
+int run_yx(hermes_ctx* ctx, int batch, float2* data, const float2* twg) {
+    int v = 0;
+    int rc = tuned_variant<LOG2L, DIR>(ctx, twg, &v);
+    if (rc) return rc;
+    return launch_yx<LOG2L, DIR>(ctx, v, batch, data, twg);
+}
+
+template <int LOG2L, int DIR>
+int run_fft3(hermes_ctx* ctx, int batch, float2* data, bool z_pass) {
+    constexpr int L = 1 << LOG2L;
+    const float2* twg = hermes_twiddles(ctx, LOG2L, DIR);
+    HERMES_REQUIRE(twg, "%s", hermes_last_error());
+    if (z_pass) {
+        const int64_t nrows = (int64_t)batch * L * L;
+        constexpr int NL = rows_per_block<LOG2L>();
+        k_fft_rows<LOG2L, DIR><<<(unsigned)((nrows + NL - 1) / NL), FFT_THREADS, 0, ctx->stream>>>(data, nrows, twg);
+        HERMES_LAUNCH_CHECK();
+    }
+    return run_yx<LOG2L, DIR>(ctx, batch, data, twg);
+}
+
+template <int DIR>
+int dispatch_fft3(hermes_ctx* ctx, int J, int batch, float2* data, bool z_pass) {
+    switch (J) {
+        case 3: return run_fft3<3, DIR>(ctx, batch, data, z_pass);
+        case 4: return run_fft3<4, DIR>(ctx, batch, data, z_pass);
+        case 5: return run_fft3<5, DIR>(ctx, batch, data, z_pass);
+        case 6: return run_fft3<6, DIR>(ctx, batch, data, z_pass);
+        case 7: return run_fft3<7, DIR>(ctx, batch, data, z_pass);
+        case 8: return run_fft3<8, DIR>(ctx, batch, data, z_pass);
+        case 9: return run_fft3<9, DIR>(ctx, batch, data, z_pass);
+        case 10: return run_fft3<10, DIR>(ctx, batch, data, z_pass);
+        default: break;
+    }
+    hermes_set_error("3-D FFT: J=%d outside 3..10", J);
+    return 2;
+}
+
+}  // namespace
+
+// Inverse strided passes only: completes a batch whose z lines were already transformed
+// (the fused multipole window kernel does the z pass itself).
+int hermes_fft3_inverse_yx(hermes_ctx* ctx, int J, int batch, float2* data) {
+    return dispatch_fft3<1>(ctx, J, batch, data, false);
+}
+
+extern "C" {
+
+int hermes_fft3_tile_choice(hermes_ctx* ctx, int J, int inverse, int* tw_out, int* threads_out) {
+    HERMES_REQUIRE(ctx && tw_out && threads_out, "hermes_fft3_tile_choice: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_fft3_tile_choice: J=%d outside 3..10", J);
+    const int v = ctx->fft_cols_variant[J][inverse ? 1 : 0] - 1;
+    if (v < 0) {
+        *tw_out = 0;
+        *threads_out = 0;
+        return 0;
+    }
+    const int L = 1 << J;
+    const int base = (8192 >> J) < L ? (8192 >> J) : L;
+    *tw_out = (v & 2) ? base / 2 : base;
+    *threads_out = (v & 1) ? 512 : 256;
+    return 0;
+}
+
+int hermes_fft3_set_tile(hermes_ctx* ctx, int J, int inverse, int tw, int threads) {
+    HERMES_REQUIRE(ctx, "hermes_fft3_set_tile: null context");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_fft3_set_tile: J=%d outside 3..10", J);
+    const int L = 1 << J;
+    const int base = (8192 >> J) < L ? (8192 >> J) : L;
+    int v = -1;
+    for (int c = 0; c < (J < 7 ? 1 : N_COL_VARIANTS); ++c) {
+        const int ctw = (c & 2) ? base / 2 : base, cnt = (c & 1) ? 512 : 256;
+        if (ctw == tw && cnt == threads) v = c;
+    }
+    HERMES_REQUIRE(v >= 0, "hermes_fft3_set_tile: (tw=%d, threads=%d) is not a candidate tile for J=%d", tw, threads,
+                   J);
+    ctx->fft_cols_variant[J][inverse ? 1 : 0] = v + 1;
+    return 0;
+}
+
+int hermes_fft3_c2c_f32(hermes_ctx* ctx, int J, int batch, int inverse, void* data) {
+    HERMES_REQUIRE(ctx && data, "hermes_fft3_c2c_f32: null argument");
+    HERMES_REQUIRE(batch >= 1, "hermes_fft3_c2c_f32: batch=%d", batch);
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    return inverse ? dispatch_fft3<1>(ctx, J, batch, (float2*)data, true)
+                   : dispatch_fft3<-1>(ctx, J, batch, (float2*)data, true);
+}
+
+}  // extern "C"

@@ -19,6 +19,8 @@ struct hermes_ctx {
     void* ws[8] = {nullptr};
     size_t ws_size[8] = {0};
     hermes_fft_cache* fft = nullptr;
+    float2* twiddles = nullptr;  // e^{+-2 pi i k / 2^j} tables for the LDS FFTs (first use)
+    int fft_cols_variant[11][2] = {};  // autotuned strided-pass tile per (log2 L, inverse); 0 = not tuned
 };
 
 // Workspace slots (each slot is an independent grow-only buffer).
@@ -35,6 +37,10 @@ enum hermes_ws_slot {
 
 void hermes_set_error(const char* fmt, ...);
 void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes);
+// Twiddle table e^{dir 2 pi i k / 2^log2L}, k < 2^log2L (device memory owned by ctx).
+const float2* hermes_twiddles(hermes_ctx* ctx, int log2L, int dir);
+// Inverse FFT along y and x of a batch of (L, L, L) complex64 grids already transformed along z.
+int hermes_fft3_inverse_yx(hermes_ctx* ctx, int J, int batch, float2* data);
 
 #define HERMES_CHECK_HIP(expr)                                                            \
     do {                                                                                   \

@@ -3,6 +3,7 @@
 // P (|phi_hat|^2 samples, L+1 doubles) is a DEVICE pointer; window descriptors
 // and shell radii are HOST arrays copied to the context workspace.
 #include "../../include/hermes_amd.h"
+#include "fft_lds.h"
 #include "hermes_common.h"
 #include "special.h"
 
@@ -269,6 +270,96 @@ __global__ __launch_bounds__(256) void k_multipole_apply(const float2* __restric
     }
 }
 
+// Fused multipole window and inverse z-FFT (FP32).  Workgroup (x, y), 0 <= x, y <= L/2, evaluates
+// K_{R_s,l,m} at its octant representatives (x, y, z <= L/2) exactly as k_multipole_apply does, forms
+// the four mirror lines (x, y), (xm, y), (x, ym), (xm, ym) of H_full * K * scale in LDS with the same
+// mirror rules, transforms them along z and writes each distinct line once.  Loops over shells.
+template <int LOG2L>
+__global__ __launch_bounds__(256) void k_mpwin_zfft(const float2* __restrict__ H, const float* __restrict__ Pf,
+                                                    ApplyArgs A, const float2* __restrict__ twg,
+                                                    float2* __restrict__ out) {
+    constexpr int L = 1 << LOG2L, Lh = L / 2 + 1;
+    constexpr int NT = 256;
+    constexpr int ZPT = (Lh + NT - 1) / NT;
+    using F = hermes_fft::LdsFft<LOG2L, 4, NT, 1, false>;
+    __shared__ float2 lds[F::ELEMS];
+    __shared__ float2 tw[L];
+    const int tid = threadIdx.x;
+    const int x = blockIdx.x / Lh, y = blockIdx.x % Lh;
+    const int xm = (L - x) & (L - 1), ym = (L - y) & (L - 1);
+    const int64_t N = (int64_t)L * L * L;
+    hermes_fft::load_twiddles<LOG2L>(tw, twg, tid, NT);
+    // shell-independent data of this thread's representatives z = tid + i * NT
+    float pw[ZPT][8];
+    float2 yv[ZPT][8];
+    double nrm[ZPT][8];
+    float2 h[ZPT][4];
+#pragma unroll
+    for (int i = 0; i < ZPT; ++i) {
+        const int z = tid + i * NT;
+        if (z < Lh) {
+            h[i][0] = H[((int64_t)x * L + y) * Lh + z];
+            h[i][1] = H[((int64_t)xm * L + y) * Lh + z];
+            h[i][2] = H[((int64_t)x * L + ym) * Lh + z];
+            h[i][3] = H[((int64_t)xm * L + ym) * Lh + z];
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+                pw[i][a] = Pf[ax < 0 ? -ax : ax] * Pf[ay < 0 ? -ay : ay] * Pf[az < 0 ? -az : az];
+                nrm[i][a] = sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
+                float2 yy = ylm_f32(A.l, A.m < 0 ? -A.m : A.m, (float)ax, (float)ay, (float)az, A.cnorm);
+                if (A.m < 0) {  // Y_{l,-m} = (-1)^m conj(Y_{l,m})
+                    const float sg = ((-A.m) & 1) ? -1.0f : 1.0f;
+                    yy = make_float2(sg * yy.x, -sg * yy.y);
+                }
+                yv[i][a] = yy;
+            }
+        }
+    }
+    for (int s = 0; s < A.n_shell; ++s) {
+        const double rs = A.radii_over_L[s];
+#pragma unroll
+        for (int i = 0; i < ZPT; ++i) {
+            const int z = tid + i * NT;
+            if (z < Lh) {
+                float2 K = make_float2(0.0f, 0.0f);
+#pragma unroll
+                for (int a = 0; a < 8; ++a) {
+                    if (pw[i][a] == 0.0f) continue;
+                    const float j = jl_f32(A.l, nrm[i][a] * rs) * pw[i][a];
+                    K.x += j * yv[i][a].x;
+                    K.y += j * yv[i][a].y;
+                }
+                const float2 k = make_float2(K.x * A.scale, K.y * A.scale);
+                const float2 kc = conjf2(k);
+                const float sx = A.sx, sz = A.sz;
+                lds[F::addr(0, z)] = cmulf(h[i][0], k);
+                lds[F::addr(1, z)] = cmulf(h[i][1], make_float2(sx * kc.x, sx * kc.y));
+                lds[F::addr(2, z)] = cmulf(h[i][2], kc);
+                lds[F::addr(3, z)] = cmulf(h[i][3], make_float2(sx * k.x, sx * k.y));
+                const int zm = (L - z) & (L - 1);
+                if (zm != z) {
+                    lds[F::addr(0, zm)] = cmulf(conjf2(h[i][3]), make_float2(sz * k.x, sz * k.y));
+                    lds[F::addr(1, zm)] = cmulf(conjf2(h[i][2]), make_float2(sz * sx * kc.x, sz * sx * kc.y));
+                    lds[F::addr(2, zm)] = cmulf(conjf2(h[i][1]), make_float2(sz * kc.x, sz * kc.y));
+                    lds[F::addr(3, zm)] = cmulf(conjf2(h[i][0]), make_float2(sz * sx * k.x, sz * sx * k.y));
+                }
+            }
+        }
+        __syncthreads();
+        F::run(lds, tw, tid);
+        float2* o = out + (int64_t)s * N;
+        for (int i = tid; i < 4 * L; i += NT) {
+            const int row = i >> LOG2L, e = i & (L - 1);
+            const bool dx = row & 1, dy = row & 2;
+            if ((dx && xm == x) || (dy && ym == y)) continue;  // mirror line coincides with line 0 / 2
+            const int rx = dx ? xm : x, ry = dy ? ym : y;
+            o[((int64_t)rx * L + ry) * L + e] = lds[F::addr(row, e)];
+        }
+        __syncthreads();
+    }
+}
+
 __global__ void k_p_to_float(const double* __restrict__ P, int n, float* __restrict__ Pf) {
     for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) Pf[i] = (float)P[i];
 }
@@ -429,10 +520,8 @@ int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const vo
     return 0;
 }
 
-int hermes_multipole_apply_f32(hermes_ctx* ctx, int J, const void* H, const double* P, const double* radii,
-                               int n_shell, int l, int m, double scale, void* out) {
-    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_apply_f32: null argument");
-    HERMES_REQUIRE(J >= 2 && J <= 10, "J=%d unsupported", J);
+static int make_apply_args(hermes_ctx* ctx, int J, const double* P, const double* radii, int n_shell, int l,
+                           int m, double scale, ApplyArgs* A, float** Pf_out) {
     HERMES_REQUIRE(n_shell >= 1 && n_shell <= MAX_SHELLS, "n_shell=%d outside 1..%d", n_shell, MAX_SHELLS);
     HERMES_REQUIRE(l >= 0 && l <= 40 && m >= -l && m <= l, "invalid multipole (l=%d, m=%d)", l, m);
     HERMES_CHECK_HIP(hipSetDevice(ctx->device));
@@ -441,23 +530,62 @@ int hermes_multipole_apply_f32(hermes_ctx* ctx, int J, const void* H, const doub
     HERMES_REQUIRE(Pf, "%s", hermes_last_error());
     k_p_to_float<<<hermes_blocks(L + 1, 256), 256, 0, ctx->stream>>>(P, L + 1, Pf);
     HERMES_LAUNCH_CHECK();
+    A->L = L;
+    A->l = l;
+    A->m = m;
+    A->n_shell = n_shell;
+    const int am = m < 0 ? -m : m;
+    A->cnorm = (float)hermes_ylm_cnorm(l, am);
+    A->scale = (float)scale;
+    A->sx = (am & 1) ? -1.0f : 1.0f;
+    A->sy = 1.0f;
+    A->sz = ((l + am) & 1) ? -1.0f : 1.0f;
+    for (int s = 0; s < MAX_SHELLS; ++s) A->radii_over_L[s] = s < n_shell ? radii[s] / L : 0.0;
+    *Pf_out = Pf;
+    return 0;
+}
+
+int hermes_multipole_apply_f32(hermes_ctx* ctx, int J, const void* H, const double* P, const double* radii,
+                               int n_shell, int l, int m, double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_apply_f32: null argument");
+    HERMES_REQUIRE(J >= 2 && J <= 10, "J=%d unsupported", J);
     ApplyArgs A;
-    A.L = L;
-    A.l = l;
-    A.m = m;
-    A.n_shell = n_shell;
-    int am = m < 0 ? -m : m;
-    A.cnorm = (float)hermes_ylm_cnorm(l, am);
-    A.scale = (float)scale;
-    A.sx = (am & 1) ? -1.0f : 1.0f;
-    A.sy = 1.0f;
-    A.sz = ((l + am) & 1) ? -1.0f : 1.0f;
-    for (int s = 0; s < MAX_SHELLS; ++s) A.radii_over_L[s] = s < n_shell ? radii[s] / L : 0.0;
+    float* Pf = nullptr;
+    int rc = make_apply_args(ctx, J, P, radii, n_shell, l, m, scale, &A, &Pf);
+    if (rc) return rc;
+    const int L = 1 << J;
     const int64_t nrep = (int64_t)(L / 2 + 1) * (L / 2 + 1) * (L / 2 + 1);
     k_multipole_apply<<<hermes_blocks(nrep, 256, 1 << 20), 256, 0, ctx->stream>>>((const float2*)H, Pf, A,
                                                                                    (float2*)out);
     HERMES_LAUNCH_CHECK();
     return 0;
+}
+
+int hermes_multipole_window_ifft_f32(hermes_ctx* ctx, int J, const void* H, const double* P, const double* radii,
+                                     int n_shell, int l, int m, double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_window_ifft_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_window_ifft_f32: J=%d outside 3..10", J);
+    ApplyArgs A;
+    float* Pf = nullptr;
+    int rc = make_apply_args(ctx, J, P, radii, n_shell, l, m, scale, &A, &Pf);
+    if (rc) return rc;
+    const float2* twg = hermes_twiddles(ctx, J, 1);
+    HERMES_REQUIRE(twg, "%s", hermes_last_error());
+    const unsigned nblk = (unsigned)(((1 << J) / 2 + 1) * ((1 << J) / 2 + 1));
+    const float2* h = (const float2*)H;
+    float2* o = (float2*)out;
+    switch (J) {
+        case 3: k_mpwin_zfft<3><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        case 4: k_mpwin_zfft<4><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        case 5: k_mpwin_zfft<5><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        case 6: k_mpwin_zfft<6><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        case 7: k_mpwin_zfft<7><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        case 8: k_mpwin_zfft<8><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        case 9: k_mpwin_zfft<9><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        default: k_mpwin_zfft<10><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+    }
+    HERMES_LAUNCH_CHECK();
+    return hermes_fft3_inverse_yx(ctx, J, n_shell, o);
 }
 
 int hermes_reduce_sum(hermes_ctx* ctx, const void* x, int64_t n, int ncomp, int dbl, double* out_host) {

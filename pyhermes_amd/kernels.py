@@ -106,6 +106,31 @@ def fftn_c2c(x, inverse=False, out=None):
     return out
 
 
+def fft3_c2c_f32(x, inverse=False):
+    """In-place complex64 3-D FFT of (.., L, L, L) by the LDS-tiled passes (unnormalised)."""
+    if x.dtype != torch.complex64 or not x.is_contiguous() or x.ndim < 3:
+        raise ValueError("fft3_c2c_f32 needs a contiguous complex64 (.., L, L, L) tensor")
+    L = x.shape[-1]
+    if x.shape[-3:] != (L, L, L):
+        raise ValueError(f"fft3_c2c_f32: grid shape {tuple(x.shape[-3:])} is not cubic")
+    batch = int(np.prod(x.shape[:-3])) if x.ndim > 3 else 1
+    _lib.call("hermes_fft3_c2c_f32", _lib.context(), _J_of(L), batch, 1 if inverse else 0, ptr(x))
+    return x
+
+
+def fft3_tile_choice(J, inverse=True):
+    """(tile width, threads) the autotuner picked for the strided FFT passes, or (0, 0) if untuned."""
+    tw, nt = ctypes.c_int(0), ctypes.c_int(0)
+    _lib.call("hermes_fft3_tile_choice", _lib.context(), int(J), 1 if inverse else 0, ctypes.byref(tw),
+              ctypes.byref(nt))
+    return tw.value, nt.value
+
+
+def fft3_set_tile(J, tw, threads, inverse=True):
+    """Pin the strided-pass tile shape (one of the autotuner's candidates)."""
+    _lib.call("hermes_fft3_set_tile", _lib.context(), int(J), 1 if inverse else 0, int(tw), int(threads))
+
+
 # ---------------------------------------------------------------- windows
 WIN_CODES = {"sphere": 0, "gaussian": 1, "shell": 2, "thick_shell": 3, "gaussian_shell": 4,
              "inverse_laplacian": 5, "shell_multipole": 6}
@@ -162,13 +187,39 @@ def pair_reduce(F1, F2, P, descs):
 
 def multipole_apply_f32(H, P, radii_grid, l, m, scale, out=None):
     """(S, L, L, L) complex64 spectra H_full * K_{R_s,l,m} * scale from the half spectrum H (complex64)."""
-    L = H.shape[0]
     r = _np_f64(radii_grid)
     S = r.size
+    L = _check_multipole_inputs(H, P, S, out)
     if out is None:
         out = torch.empty((S, L, L, L), dtype=torch.complex64, device=H.device)
     _lib.call("hermes_multipole_apply_f32", _lib.context(), _J_of(L), ptr(H), ptr(P), r.ctypes.data_as(_DP), S,
               int(l), int(m), float(scale), ptr(out))
+    return out
+
+
+def _check_multipole_inputs(H, P, S, out):
+    if H.dtype != torch.complex64 or not H.is_contiguous() or H.ndim != 3:
+        raise ValueError("half spectrum H must be a contiguous complex64 (L, L, L/2+1) tensor")
+    L = H.shape[0]
+    if tuple(H.shape) != (L, L, L // 2 + 1):
+        raise ValueError(f"half spectrum shape {tuple(H.shape)} != {(L, L, L // 2 + 1)}")
+    if P.dtype != torch.float64 or P.numel() < L + 1:
+        raise ValueError("P must hold L + 1 float64 values")
+    if out is not None and (out.dtype != torch.complex64 or not out.is_contiguous()
+                            or tuple(out.shape) != (S, L, L, L)):
+        raise ValueError(f"out must be a contiguous complex64 {(S, L, L, L)} tensor")
+    return L
+
+
+def multipole_window_ifft_f32(H, P, radii_grid, l, m, scale, out=None):
+    """(S, L, L, L) complex64 fields IFFT3(H_full * K_{R_s,l,m} * scale), window and inverse FFT fused."""
+    r = _np_f64(radii_grid)
+    S = r.size
+    L = _check_multipole_inputs(H, P, S, out)
+    if out is None:
+        out = torch.empty((S, L, L, L), dtype=torch.complex64, device=H.device)
+    _lib.call("hermes_multipole_window_ifft_f32", _lib.context(), _J_of(L), ptr(H), ptr(P), r.ctypes.data_as(_DP),
+              S, int(l), int(m), float(scale), ptr(out))
     return out
 
 

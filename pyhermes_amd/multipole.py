@@ -7,9 +7,9 @@ Two paths compute ``ddd_l`` for shell pairs (r12, r13):
   window builds, two forward and two inverse complex FFTs and the FP64
   contraction, exactly as the reference sequences them.
 * ``ddd_multipole_batched_f32`` -- all shell pairs at once: one real FFT per
-  field, one fused multipole window launch per (l, m) writing every shell's
-  spectrum, one batched inverse FFT, and one MFMA Gram contraction (FP32
-  products, FP64 accumulation).  When fields 2 and 3 and their shell lists are
+  field, then per (l, m) one fused window + inverse-FFT call producing every
+  shell's field (window, z pass, y and x passes) and one MFMA Gram contraction
+  (FP32 products, FP64 accumulation).  When fields 2 and 3 and their shell lists are
   identical, F_{l,-m} = (-1)^(l+m) conj(F_{l,m}) removes the second set of FFTs.
 
 Both return ddd_l following combine_multipole_m_terms (corr3pcf_multipoles.py:173-180):
@@ -138,11 +138,8 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     for li, l in enumerate(range(l_min, l_max + 1)):
         v_re = []
         for m in range(l + 1):
-            t = tm.start("window_apply")
-            kernels.multipole_apply_f32(H2, geo.P, r12, l, m, 1.0 / N, out=U)
-            tm.stop(t)
-            t = tm.start("fft_c2c_inv")
-            kernels.fftn_c2c(U, inverse=True, out=U)
+            t = tm.start("window_ifft")
+            kernels.multipole_window_ifft_f32(H2, geo.P, r12, l, m, 1.0 / N, out=U)
             tm.stop(t)
             G = np.zeros((r12.size, r13.size))
             t = tm.start("gram")
@@ -157,11 +154,8 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
                         G[a0:a1, b0:b1] = sign * g
             else:
                 tm.stop(t)
-                t = tm.start("window_apply")
-                kernels.multipole_apply_f32(H3, geo.P, r13, l, -m, 1.0 / N, out=V)
-                tm.stop(t)
-                t = tm.start("fft_c2c_inv")
-                kernels.fftn_c2c(V, inverse=True, out=V)
+                t = tm.start("window_ifft")
+                kernels.multipole_window_ifft_f32(H3, geo.P, r13, l, -m, 1.0 / N, out=V)
                 tm.stop(t)
                 t = tm.start("gram")
                 for a0, a1 in blocks12:

@@ -37,11 +37,11 @@ def test_summand_f64_matches_oracle(phi):
     assert abs(got - ref) < 1e-11 * abs(ref)
 
 
-@pytest.mark.parametrize("cross", [False, True])
-def test_gram_f32_matches_oracle(phi, cross):
+@pytest.mark.parametrize("cross,L", [(False, 16), (True, 16), (False, 32), (False, 64)])
+def test_gram_f32_matches_oracle(phi, cross, L):
     import torch
     from pyhermes_amd import kernels
-    L = 16
+    print(f"gram cross={cross} L={L}", flush=True)
     rng = np.random.default_rng(7)
     d = rng.normal(size=(L, L, L))
     U = np.stack(_fields(L, 5, 3))
