@@ -274,3 +274,31 @@ def test_triplet_estimator_constant_and_bruteforce(phi):
         tot += np.sum(n1 * mc.interpolate_bruteforce(e[1], centers, phi, o[:3])
                       * mc.interpolate_bruteforce(e[2], centers, phi, o[3:]))
     assert est == pytest.approx(tot / (abc.shape[0] * centers.shape[0]), rel=1e-11)
+
+
+def test_mlp_gradient_finite_differences():
+    """The oracle MLP gradient equals central differences of its mean loss (clip inactive)."""
+    from oracle import mlp as omlp
+    g = np.random.default_rng(11)
+    n_par = omlp.H * omlp.IN + omlp.H + omlp.H * omlp.H + omlp.H + omlp.H + 1
+    w = g.normal(0, 0.3, n_par)
+    x = g.normal(size=(40, omlp.IN))
+    y = g.exponential(1.0, 40)
+
+    def loss(wv):
+        s, _ = omlp.forward(wv, x)
+        return float(np.mean(y * np.exp(-s) + s))
+
+    gr, l0 = omlp.grad(w, x, y, clip=1e9)
+    assert abs(l0 - loss(w)) < 1e-12
+    idx = g.choice(n_par, 60, replace=False)
+    h = 1e-6
+    for i in idx:
+        e = np.zeros(n_par)
+        e[i] = h
+        fd = (loss(w + e) - loss(w - e)) / (2 * h)
+        assert abs(fd - gr[i]) <= 1e-6 * max(1.0, abs(fd)), (i, fd, gr[i])
+    # bf16 rounding is round-to-nearest-even on the top 16 bits
+    v = np.array([1.0, 1.00390625, 1.005859375, -3.1415927], dtype=np.float32)
+    r = omlp.bf16_round(v)
+    assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0078125) and r[3] == np.float32(-3.140625)
