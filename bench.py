@@ -33,6 +33,23 @@ def log(msg):
     print(f"[bench] {msg}", file=sys.stderr, flush=True)
 
 
+def gram_traffic_from_profile(workload):
+    """HBM bytes per k_gram_auto launch from the committed rocprofv3 --pmc summary, if it covers this
+    workload (FETCH_SIZE and WRITE_SIZE scaled by their reading for a 1 GiB copy); else None."""
+    path = os.path.join(ROOT, "profiles", "k_gram_auto_traffic.txt")
+    if not os.path.exists(path):
+        return None
+    fields = {}
+    with open(path) as f:
+        for line in f:
+            if ":" in line and not line.startswith("#"):
+                k, v = line.split(":", 1)
+                fields[k.strip()] = v.strip()
+    if fields.get("workload") != workload or "traffic_bytes_per_launch" not in fields:
+        return None
+    return float(fields["traffic_bytes_per_launch"])
+
+
 def cpu_baseline(J_sample, lm_terms, shells_grid_at_sample, box, seed):
     """Time the CPU oracle (C/OpenMP restatement + scipy.fft) on a bounded sample.
 
@@ -132,14 +149,15 @@ def main():
         res32 = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, lmax)
     torch.cuda.synchronize()
     timer = multipole.StageTimer()
+    kernels.profile_gram(True)  # HIP events around the contraction kernel, on the library's stream
     t0 = time.perf_counter()
     for _ in range(args.steps):
         res32 = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, lmax, timer=timer)
     torch.cuda.synchronize()
     t_fp32 = (time.perf_counter() - t0) / args.steps
+    gram_total_ms, gram_calls = kernels.profile_gram(False)
     stages = {k: round(v[0] / args.steps, 3) for k, v in timer.summary().items()}
-    gram_calls = timer.summary()["gram"][1]
-    gram_ms = timer.summary()["gram"][0] / gram_calls
+    gram_ms = gram_total_ms / gram_calls
     log(f"FP32 step {t_fp32 * 1e3:.1f} ms; stages (ms/step) {stages}")
 
     out = {}
@@ -172,10 +190,13 @@ def main():
                          f"({per_term:.2f} s per m-term), scaled x{vol:.0f} to J={J} and x{n_mterms} m-terms"}
         log(f"CPU: {per_term:.2f} s per m-term at J={Js} -> {1.0 / s_per_set:.3e} sets/s at J={J}")
 
-    # roofline of the dominant kernel (MFMA Gram contraction): useful FP32 MFMA flops per launch
+    # roofline of the dominant kernel, the auto Gram contraction k_gram_auto: per launch it forms the
+    # S x S Gram of 18 real products per cell (9 yz slots x re/im against the combined right operand)
     S = args.nshell
-    gram_flops = 2.0 * S * S * 54 * N
+    gram_flops = 2.0 * S * S * 18 * N
     achieved = gram_flops / (gram_ms * 1e-3) / 1e12
+    workload = f"sh3pcf_multipoles_J{J}_lmax{lmax}_{S}shells"
+    traffic = gram_traffic_from_profile(workload)
     summ = timer.summary()
     n_modes = L * L * (L // 2 + 1)
     # fused window + inverse FFT of S shells: the half spectrum (complex64) read once and S complex64
@@ -204,12 +225,13 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 (fp64 accumulation)",
         "data": "synthetic lognormal mock (seeded)",
-        "config": {"workload": f"sh3pcf_multipoles_J{J}_lmax{lmax}_{S}shells", "particles": args.particles,
+        "config": {"workload": workload, "particles": args.particles,
                    "J": J, "l_max": lmax, "n_shell": S, "unique_sets_per_step": n_unique,
                    "ordered_sets_per_step": n_ordered, "m_terms_per_set": n_mterms},
-        "roofline": {"bound": "mfma", "kernel": "k_gram_f32", "achieved": achieved,
+        "roofline": {"bound": "mfma", "kernel": "k_gram_auto", "achieved": achieved,
                      "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MFMA_F32_PEAK_TFLOPS,
-                     "traffic": None, "avg_launch_ms": gram_ms},
+                     "traffic": traffic, "avg_launch_ms": gram_ms,
+                     "algorithmic_bytes": S * N * 8 + N * 4},
         "cpu_baseline": cpu,
         "stages_ms_per_step": stages,
         "projection_ms": proj_ms,

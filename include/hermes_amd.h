@@ -169,6 +169,11 @@ int hermes_multipole_gram_f32(hermes_ctx *ctx, int J, const float *d1, const voi
  * max_abs_err = 0 on a conforming device.  (No counterpart in the reference.) */
 int hermes_mfma_selftest(hermes_ctx *ctx, double *max_abs_err);
 
+/* Measurement hook (no reference counterpart): while enabled, hermes_multipole_gram_f32 records HIP
+ * events on the context stream around its contraction kernel and accumulates the kernel time.
+ * Reports the totals accumulated so far; enable = 1 resets them and starts timing, 0 stops. */
+int hermes_profile_gram(hermes_ctx *ctx, int enable, double *total_ms_out, int64_t *calls_out);
+
 /* ---- Monte-Carlo 3PCF ----
  * Interpolation n(x) = sum_27 grid[(c - s) & (L-1)] phi phi phi at scaled
  * positions spos (n, 3) + (dx, dy, dz), FP64.  Replaces

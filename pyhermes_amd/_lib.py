@@ -33,6 +33,7 @@ _SIGNATURES = {
     "hermes_fft3_c2c_f32": [c_void_p, c_int, c_int, c_int, c_void_p],
     "hermes_fft3_tile_choice": [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "hermes_fft3_set_tile": [c_void_p, c_int, c_int, c_int, c_int],
+    "hermes_profile_gram": [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_i64)],
 }
 
 

@@ -508,20 +508,43 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
     double* ws = (double*)hermes_workspace(
         ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256 + (size_t)RED_MAX_BLOCKS * 256));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
+    if (ctx->time_gram) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[0], ctx->stream));
     if (g.cross)
         k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     else
         k_gram_auto<<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws);
     HERMES_LAUNCH_CHECK();
+    if (ctx->time_gram) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[1], ctx->stream));
     sum_rows(ctx->stream, ws, nblocks, 256, ws + (size_t)nblocks * 256 + 256, ws + (size_t)nblocks * 256);
     HERMES_LAUNCH_CHECK();
     double h[256];
     HERMES_CHECK_HIP(hipMemcpyAsync(h, ws + (size_t)nblocks * 256, sizeof(double) * 256, hipMemcpyDeviceToHost,
                                     ctx->stream));
     HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    if (ctx->time_gram) {
+        float ms = 0.0f;
+        HERMES_CHECK_HIP(hipEventElapsedTime(&ms, ctx->gram_ev[0], ctx->gram_ev[1]));
+        ctx->gram_ms += ms;
+        ctx->gram_calls += 1;
+    }
     for (int i = 0; i < g.S1; ++i)
         for (int j = 0; j < g.S2; ++j)
             out_host[i * g.S2 + j] = g.cross ? h[i * 16 + j] : h[i * 16 + j] + h[j * 16 + i];
+    return 0;
+}
+
+int hermes_profile_gram(hermes_ctx* ctx, int enable, double* total_ms_out, int64_t* calls_out) {
+    HERMES_REQUIRE(ctx, "hermes_profile_gram: null context");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    if (total_ms_out) *total_ms_out = ctx->gram_ms;
+    if (calls_out) *calls_out = ctx->gram_calls;
+    if (enable) {
+        for (int i = 0; i < 2; ++i)
+            if (!ctx->gram_ev[i]) HERMES_CHECK_HIP(hipEventCreate(&ctx->gram_ev[i]));
+        ctx->gram_ms = 0.0;
+        ctx->gram_calls = 0;
+    }
+    ctx->time_gram = enable != 0;
     return 0;
 }
 

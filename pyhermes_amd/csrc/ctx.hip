@@ -64,6 +64,8 @@ int hermes_ctx_destroy(hermes_ctx* ctx) {
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     hermes_fft_cache_destroy(ctx->fft);
     if (ctx->twiddles) (void)hipFree(ctx->twiddles);
+    for (int i = 0; i < 2; ++i)
+        if (ctx->gram_ev[i]) (void)hipEventDestroy(ctx->gram_ev[i]);
     delete ctx;
     return 0;
 }

@@ -21,6 +21,11 @@ struct hermes_ctx {
     hermes_fft_cache* fft = nullptr;
     float2* twiddles = nullptr;  // e^{+-2 pi i k / 2^j} tables for the LDS FFTs (first use)
     int fft_cols_variant[11][2] = {};  // autotuned strided-pass tile per (log2 L, inverse); 0 = not tuned
+    // optional HIP-event timing of the Gram contraction kernel (hermes_profile_gram)
+    bool time_gram = false;
+    hipEvent_t gram_ev[2] = {nullptr, nullptr};
+    double gram_ms = 0.0;
+    int64_t gram_calls = 0;
 };
 
 // Workspace slots (each slot is an independent grow-only buffer).

@@ -257,6 +257,13 @@ def multipole_gram_f32(d1, U, Q, lam, V=None, v_conj=False):
     return out
 
 
+def profile_gram(enable):
+    """Start (True: reset) or stop timing the Gram kernel; returns (total_ms, launches) so far."""
+    ms, n = ctypes.c_double(0.0), ctypes.c_int64(0)
+    _lib.call("hermes_profile_gram", _lib.context(), 1 if enable else 0, ctypes.byref(ms), ctypes.byref(n))
+    return ms.value, n.value
+
+
 def mfma_selftest():
     out = np.zeros(1, dtype=np.float64)
     _lib.call("hermes_mfma_selftest", _lib.context(), out.ctypes.data_as(_DP))
