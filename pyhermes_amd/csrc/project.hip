@@ -3,12 +3,13 @@
 // Deterministic, atomics-free in floating point:
 //   1. count:  each particle is assigned to every output tile its 3x3x3 stencil
 //              touches (1..8 tiles; tiles are 4x8x64 cells, z fastest); integer
-//              counts and the max |w| bit pattern per tile.
+//              counts per tile (wave-aggregated atomics).
 //   2. scan:   exclusive scan of the tile counts.
 //   3. fill:   bucket 16-byte records {cell, frac, w} per tile.
 //   4. accumulate: one workgroup per tile sums the 27 FP64 contributions of its
-//              records into an LDS int64 fixed-point accumulator (scale chosen per
-//              tile from the integer count and max |w|, so no overflow), then
+//              records ((record, tap) pairs spread over the workgroup) into an LDS
+//              int64 fixed-point accumulator (scale chosen per tile from the record
+//              count and the catalogue's max |w|, so no overflow), then
 //              writes the tile once, coalesced, as FP32 and/or FP64.
 // Integer addition is associative, so the result is bit-identical whatever the
 // order in which records were bucketed or accumulated.
@@ -22,6 +23,8 @@ namespace {
 constexpr int MAX_TX = 4, MAX_TY = 8, MAX_TZ = 64;
 constexpr int TILE_CELLS = MAX_TX * MAX_TY * MAX_TZ;
 constexpr int ACC_THREADS = 256;
+constexpr int SCAN_THREADS = 1024, SCAN_ITEMS = 4;  // counts per scan block: 4096
+constexpr int ACC_RCHUNK = 512;                      // records staged per LDS chunk
 
 struct ProjRec {
     uint32_t cell;  // wrapped cell coordinates, 10 bits each (x | y<<10 | z<<20)
@@ -29,14 +32,12 @@ struct ProjRec {
     double w;
 };
 
+// Power-of-two grid and tile extents (log2), tiles z fastest.
 struct TileGeom {
-    int L, tx, ty, tz, ntx, nty, ntz;
+    int L, lgL;
+    int lgtx, lgty, lgtz;
+    int lgnty, lgntz;
 };
-
-__device__ __forceinline__ int wrapi(int c, int L) {
-    int r = c % L;
-    return r < 0 ? r + L : r;
-}
 
 // Cell and fraction index exactly as wavelet_grid.py:32-34 (int32 floor, truncation).
 __device__ __forceinline__ bool cell_frac(const double* pos, int64_t p, double sf, int res, int c[3], int f[3]) {
@@ -56,157 +57,345 @@ __device__ __forceinline__ bool cell_frac(const double* pos, int64_t p, double s
     return ok;
 }
 
-__device__ __forceinline__ int tiles_of(int c, int L, int T, int out[2]) {
-    int t0 = wrapi(c, L) / T;
-    int t1 = wrapi(c - 2, L) / T;
-    out[0] = t0;
-    out[1] = t1;
-    return t0 == t1 ? 1 : 2;
+// Tiles along one axis covered by cells c-2..c (periodic): 1 or 2.
+__device__ __forceinline__ int tiles_of(int c, int m, int lgT, int out[2]) {
+    out[0] = (c & m) >> lgT;
+    out[1] = ((c - 2) & m) >> lgT;
+    return out[0] == out[1] ? 1 : 2;
 }
 
-__global__ void k_proj_count(const double* __restrict__ pos, const double* __restrict__ w, int64_t n, double sf,
-                             int res, TileGeom g, int* __restrict__ tile_count,
-                             unsigned long long* __restrict__ tile_maxw, int32_t* __restrict__ cell_out,
+__device__ __forceinline__ int tile_id(const TileGeom& g, int X, int Y, int Z) {
+    return (X << (g.lgnty + g.lgntz)) | (Y << g.lgntz) | Z;
+}
+
+// Wave-aggregated counter increment: every active lane adds 1 to ctr[key]; one atomic per distinct
+// key in the wave (catalogues are often stored in spatial order, so a wave's particles share tiles).
+// Returns the counter value before this lane's increment (old value + rank among the lanes with the
+// same key).  Must be reached by all lanes of the wave.
+__device__ __forceinline__ int wave_add(int* __restrict__ ctr, int key, bool active) {
+    const int lane = threadIdx.x % 64;
+    unsigned long long pending = __ballot(active);
+    int mine = 0;
+    while (pending) {
+        const int leader = __ffsll((long long)pending) - 1;
+        const int lkey = __shfl(key, leader, 64);
+        const unsigned long long same = __ballot(active && key == lkey) & pending;
+        int base = 0;
+        if (lane == leader) base = atomicAdd(&ctr[lkey], __popcll(same));
+        base = __shfl(base, leader, 64);
+        if ((same >> lane) & 1ull) mine = base + __popcll(same & ((1ull << lane) - 1ull));
+        pending &= ~same;
+    }
+    return mine;
+}
+
+// Replica k (0..7) of a particle: the tile of stencil corner (X[k>>2], Y[(k>>1)&1], Z[k&1]) when that
+// corner is distinct along every axis, else -1.
+__device__ __forceinline__ int replica_tile(const TileGeom& g, const int X[2], const int Y[2], const int Z[2], int nx,
+                                            int ny, int nz, int k) {
+    const int a = k >> 2, b = (k >> 1) & 1, d = k & 1;
+    if (a >= nx || b >= ny || d >= nz) return -1;
+    return tile_id(g, X[a], Y[b], Z[d]);
+}
+
+__global__ void k_proj_count(const double* __restrict__ pos, int64_t n, double sf, int res, TileGeom g,
+                             int* __restrict__ tile_count, int32_t* __restrict__ cell_out,
                              int32_t* __restrict__ frac_out, int* __restrict__ err) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        int c[3], f[3];
-        if (!cell_frac(pos, p, sf, res, c, f)) atomicOr(err, 1);
-        if (cell_out) { cell_out[3 * p] = c[0]; cell_out[3 * p + 1] = c[1]; cell_out[3 * p + 2] = c[2]; }
-        if (frac_out) { frac_out[3 * p] = f[0]; frac_out[3 * p + 1] = f[1]; frac_out[3 * p + 2] = f[2]; }
-        unsigned long long wb = (unsigned long long)__double_as_longlong(fabs(w[p]));
+    const int m = g.L - 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nround = (n + stride - 1) / stride * stride;  // uniform trip count for the wave-wide ops
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nround; p += stride) {
+        const bool act = p < n;
+        int c[3] = {0, 0, 0}, f[3] = {0, 0, 0};
+        if (act) {
+            if (!cell_frac(pos, p, sf, res, c, f)) atomicOr(err, 1);
+            if (cell_out) { cell_out[3 * p] = c[0]; cell_out[3 * p + 1] = c[1]; cell_out[3 * p + 2] = c[2]; }
+            if (frac_out) { frac_out[3 * p] = f[0]; frac_out[3 * p + 1] = f[1]; frac_out[3 * p + 2] = f[2]; }
+        }
         int X[2], Y[2], Z[2];
-        int nx = tiles_of(c[0], g.L, g.tx, X), ny = tiles_of(c[1], g.L, g.ty, Y), nz = tiles_of(c[2], g.L, g.tz, Z);
-        for (int a = 0; a < nx; ++a)
-            for (int b = 0; b < ny; ++b)
-                for (int d = 0; d < nz; ++d) {
-                    int t = (X[a] * g.nty + Y[b]) * g.ntz + Z[d];
-                    atomicAdd(&tile_count[t], 1);
-                    atomicMax(&tile_maxw[t], wb);
-                }
+        const int nx = tiles_of(c[0], m, g.lgtx, X), ny = tiles_of(c[1], m, g.lgty, Y),
+                  nz = tiles_of(c[2], m, g.lgtz, Z);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int t = act ? replica_tile(g, X, Y, Z, nx, ny, nz, k) : -1;
+            (void)wave_add(tile_count, t < 0 ? 0 : t, t >= 0);
+        }
     }
 }
 
-// Exclusive scan of tile counts with one workgroup; also clears the fill cursors.
-__global__ void k_proj_scan(const int* __restrict__ count, int n, int* __restrict__ off, int* __restrict__ cursor) {
-    __shared__ long long part[1024];
-    int tid = threadIdx.x;
-    int chunk = (n + blockDim.x - 1) / blockDim.x;
-    int beg = tid * chunk, end = min(n, beg + chunk);
+// Exclusive scan of the tile counts in three launches: block sums, a scan of the block sums
+// (one workgroup; at most 2^19 tiles -> 128 blocks), then each block's local scan plus its offset.
+// The last launch also clears the fill cursors and writes the total to off[n].
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_reduce(const int* __restrict__ count, int n,
+                                                              long long* __restrict__ bsum) {
+    __shared__ long long sh[SCAN_THREADS / 64];
+    const int base = blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
     long long s = 0;
-    for (int i = beg; i < end; ++i) s += count[i];
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const int k = base + threadIdx.x * SCAN_ITEMS + i;
+        if (k < n) s += count[k];
+    }
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if (threadIdx.x % 64 == 0) sh[threadIdx.x / 64] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        long long t = 0;
+        for (int i = 0; i < SCAN_THREADS / 64; ++i) t += sh[i];
+        bsum[blockIdx.x] = t;
+    }
+}
+
+__global__ void k_scan_blocks(long long* __restrict__ bsum, int nb) {
+    // exclusive, in place; nb is small
+    if (threadIdx.x == 0) {
+        long long run = 0;
+        for (int i = 0; i < nb; ++i) {
+            const long long v = bsum[i];
+            bsum[i] = run;
+            run += v;
+        }
+        bsum[nb] = run;
+    }
+}
+
+__global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* __restrict__ count, int n,
+                                                             const long long* __restrict__ bsum, int nb,
+                                                             int* __restrict__ off, int* __restrict__ cursor) {
+    __shared__ long long part[SCAN_THREADS];
+    const int base = blockIdx.x * SCAN_THREADS * SCAN_ITEMS;
+    const int tid = threadIdx.x;
+    int v[SCAN_ITEMS];
+    long long s = 0;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const int k = base + tid * SCAN_ITEMS + i;
+        v[i] = k < n ? count[k] : 0;
+        s += v[i];
+    }
     part[tid] = s;
     __syncthreads();
-    for (int o = 1; o < (int)blockDim.x; o <<= 1) {
-        long long v = (tid >= o) ? part[tid - o] : 0;
+    for (int o = 1; o < SCAN_THREADS; o <<= 1) {
+        const long long a = tid >= o ? part[tid - o] : 0;
         __syncthreads();
-        part[tid] += v;
+        part[tid] += a;
         __syncthreads();
     }
-    long long run = part[tid] - s;
-    for (int i = beg; i < end; ++i) {
-        off[i] = (int)run;
-        run += count[i];
-        cursor[i] = 0;
+    long long run = bsum[blockIdx.x] + part[tid] - s;
+#pragma unroll
+    for (int i = 0; i < SCAN_ITEMS; ++i) {
+        const int k = base + tid * SCAN_ITEMS + i;
+        if (k < n) {
+            off[k] = (int)run;
+            cursor[k] = 0;
+        }
+        run += v[i];
     }
-    if (tid == blockDim.x - 1) off[n] = (int)part[tid];
+    if (blockIdx.x == 0 && tid == 0) off[n] = (int)bsum[nb];
 }
 
 __global__ void k_proj_fill(const double* __restrict__ pos, const double* __restrict__ w, int64_t n, double sf,
                             int res, TileGeom g, const int* __restrict__ off, int* __restrict__ cursor,
                             ProjRec* __restrict__ rec) {
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x) {
-        int c[3], f[3];
-        cell_frac(pos, p, sf, res, c, f);
+    const int m = g.L - 1;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    const int64_t nround = (n + stride - 1) / stride * stride;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < nround; p += stride) {
+        const bool act = p < n;
+        int c[3] = {0, 0, 0}, f[3] = {0, 0, 0};
         ProjRec r;
-        r.cell = (uint32_t)wrapi(c[0], g.L) | ((uint32_t)wrapi(c[1], g.L) << 10) | ((uint32_t)wrapi(c[2], g.L) << 20);
-        r.frac = (uint32_t)f[0] | ((uint32_t)f[1] << 10) | ((uint32_t)f[2] << 20);
-        r.w = w[p];
+        r.cell = 0;
+        r.frac = 0;
+        r.w = 0.0;
+        if (act) {
+            cell_frac(pos, p, sf, res, c, f);
+            r.cell = (uint32_t)(c[0] & m) | ((uint32_t)(c[1] & m) << 10) | ((uint32_t)(c[2] & m) << 20);
+            r.frac = (uint32_t)f[0] | ((uint32_t)f[1] << 10) | ((uint32_t)f[2] << 20);
+            r.w = w[p];
+        }
         int X[2], Y[2], Z[2];
-        int nx = tiles_of(c[0], g.L, g.tx, X), ny = tiles_of(c[1], g.L, g.ty, Y), nz = tiles_of(c[2], g.L, g.tz, Z);
-        for (int a = 0; a < nx; ++a)
-            for (int b = 0; b < ny; ++b)
-                for (int d = 0; d < nz; ++d) {
-                    int t = (X[a] * g.nty + Y[b]) * g.ntz + Z[d];
-                    int slot = off[t] + atomicAdd(&cursor[t], 1);
-                    rec[slot] = r;
-                }
+        const int nx = tiles_of(c[0], m, g.lgtx, X), ny = tiles_of(c[1], m, g.lgty, Y),
+                  nz = tiles_of(c[2], m, g.lgtz, Z);
+#pragma unroll
+        for (int k = 0; k < 8; ++k) {
+            const int t = act ? replica_tile(g, X, Y, Z, nx, ny, nz, k) : -1;
+            const int slot = wave_add(cursor, t < 0 ? 0 : t, t >= 0);
+            if (t >= 0) rec[off[t] + slot] = r;
+        }
     }
 }
 
+// max |w| over all particles as the bit pattern of a non-negative double (integer max of such bit
+// patterns orders them like the values): deterministic in any order.
+__global__ void k_absmax(const double* __restrict__ w, int64_t n, unsigned long long* __restrict__ out) {
+    double mw = 0.0;
+    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
+        mw = fmax(mw, fabs(w[p]));
+    for (int o = 32; o > 0; o >>= 1) mw = fmax(mw, __shfl_xor(mw, o, 64));
+    if (threadIdx.x % 64 == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mw));
+}
+
+// Persistent workgroups loop over tiles (t = block, block + grid, ...; consecutive workgroups take
+// consecutive tiles, so concurrent stores stay close in memory).  Zeroing and output phases: thread t
+// takes pairs of consecutive z cells (16-byte LDS accesses at unit lane stride: no bank conflicts),
+// stored as 8-byte FP32 / 16-byte FP64 pairs.  Fixed-point scale per tile: every cell receives at most one contribution per record,
+// each bounded by max|w| * max|phi|^3 (global max |w|), so |sum| < 2^61 after scaling.
+// Fixed-point value q * 2^-S as FP32 without the FP64 route: |q| split into 32-bit halves, one fma;
+// within one ulp of the correctly rounded value.  Valid for S <= 100 (2^-S normal in FP32).
+__device__ __forceinline__ float fixed_to_f32(long long q, float two_32mS, float two_mS) {
+    const unsigned long long a = q < 0 ? (unsigned long long)(-q) : (unsigned long long)q;
+    const float f = fmaf((float)(unsigned)(a >> 32), two_32mS, (float)(unsigned)(a & 0xffffffffull) * two_mS);
+    return q < 0 ? -f : f;
+}
+
+// FIXED: the full 4 x 8 x 64 tile (grids of 64^3 and larger) with compile-time extents.
+template <bool COUNTS, bool FIXED>
 __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
-    const ProjRec* __restrict__ rec, const int* __restrict__ off, const unsigned long long* __restrict__ tile_maxw,
-    const double* __restrict__ phi, int res, TileGeom g, double phimax3, float* __restrict__ out32,
+    const ProjRec* __restrict__ rec, const int* __restrict__ off, const double* __restrict__ phi, int res, TileGeom gg,
+    const unsigned long long* __restrict__ wmax_bits, double phimax3, int ntiles, float* __restrict__ out32,
     double* __restrict__ out64, int32_t* __restrict__ counts_out) {
-    __shared__ unsigned long long acc[TILE_CELLS];
-    __shared__ int cnt[TILE_CELLS];
-    const int t = blockIdx.x;
+    TileGeom g = gg;
+    if (FIXED) {
+        g.lgtx = 2;
+        g.lgty = 3;
+        g.lgtz = 6;
+    }
+    __shared__ __align__(16) long long acc[TILE_CELLS];
+    __shared__ __align__(16) int cnt[COUNTS ? TILE_CELLS : 4];
+    __shared__ double sphi[3 * 1024];            // the scaling function, resident for the workgroup's life
+    __shared__ __align__(16) ProjRec srec[ACC_RCHUNK];  // one chunk of the tile's records
     const int tid = threadIdx.x;
-    const int tzid = t % g.ntz, tyid = (t / g.ntz) % g.nty, txid = t / (g.ntz * g.nty);
-    const int ox = txid * g.tx, oy = tyid * g.ty, oz = tzid * g.tz;
-    const int ncell = g.tx * g.ty * g.tz;
-    const int beg = off[t], end = off[t + 1];
+    const int m = g.L - 1;
+    const int tx = 1 << g.lgtx, ty = 1 << g.lgty, tz = 1 << g.lgtz;
+    const int ncell = tx * ty * tz;
     const int64_t L = g.L;
+    const double unit = __longlong_as_double((long long)*wmax_bits) * phimax3;
+    for (int i = tid; i < 3 * res; i += ACC_THREADS) sphi[i] = phi[i];
 
-    if (beg == end) {
-        for (int i = tid; i < ncell; i += blockDim.x) {
-            int lz = i % g.tz, ly = (i / g.tz) % g.ty, lx = i / (g.tz * g.ty);
-            int64_t gi = ((ox + lx) * L + (oy + ly)) * L + (oz + lz);
-            if (out32) out32[gi] = 0.0f;
-            if (out64) out64[gi] = 0.0;
-            if (counts_out) counts_out[gi] = 0;
-        }
-        return;
-    }
-    // Fixed-point scale: every cell receives at most one contribution per record,
-    // each bounded by max|w| * max|phi|^3, so |sum| < 2^61 after scaling.
-    double bound = (double)(end - beg) * __longlong_as_double((long long)tile_maxw[t]) * phimax3;
-    int e = 0;
-    frexp(bound, &e);
-    int S = 61 - e;
-    if (S > 1000) S = 1000;  // keep 2^S finite for vanishing weights
-    const double mult = ldexp(1.0, S), inv = ldexp(1.0, -S);
-
-    for (int i = tid; i < ncell; i += blockDim.x) {
-        acc[i] = 0ull;
-        cnt[i] = 0;
-    }
-    __syncthreads();
-    for (int r = beg + tid; r < end; r += blockDim.x) {
-        ProjRec R = rec[r];
-        int cx = R.cell & 1023, cy = (R.cell >> 10) & 1023, cz = (R.cell >> 20) & 1023;
-        int fx = R.frac & 1023, fy = (R.frac >> 10) & 1023, fz = (R.frac >> 20) & 1023;
-        for (int sx = 0; sx < 3; ++sx) {
-            int lx = wrapi(cx - sx, g.L) - ox;
-            if (lx < 0 || lx >= g.tx) continue;
-            double phx = R.w * phi[fx + res * sx];
-            for (int sy = 0; sy < 3; ++sy) {
-                int ly = wrapi(cy - sy, g.L) - oy;
-                if (ly < 0 || ly >= g.ty) continue;
-                double phxy = phx * phi[fy + res * sy];
-                for (int sz = 0; sz < 3; ++sz) {
-                    int lz = wrapi(cz - sz, g.L) - oz;
-                    if (lz < 0 || lz >= g.tz) continue;
-                    double v = phxy * phi[fz + res * sz];
-                    long long q = __double2ll_rn(v * mult);
-                    atomicAdd(&acc[(lx * g.ty + ly) * g.tz + lz], (unsigned long long)q);
+    // Software pipeline over this workgroup's tiles t, t+G, t+2G, ...: while tile t is processed, the
+    // first record chunk of tile t+G is in flight into registers and the offsets of tile t+2G too.
+    const int G = gridDim.x;
+    constexpr int RPT = ACC_RCHUNK / ACC_THREADS;  // records per thread per chunk
+    int t = blockIdx.x;
+    int beg = t < ntiles ? off[t] : 0, end = t < ntiles ? off[t + 1] : 0;
+    int nbeg = t + G < ntiles ? off[t + G] : 0, nend = t + G < ntiles ? off[t + G + 1] : 0;
+    ProjRec pref[RPT];
+#pragma unroll
+    for (int k = 0; k < RPT; ++k)
+        if (beg + tid + k * ACC_THREADS < end) pref[k] = rec[beg + tid + k * ACC_THREADS];
+    for (; t < ntiles; t += G) {
+        const int tn2 = t + 2 * G;
+        const int nnbeg = tn2 < ntiles ? off[tn2] : 0, nnend = tn2 < ntiles ? off[tn2 + 1] : 0;
+        const int tzid = t & ((1 << g.lgntz) - 1), tyid = (t >> g.lgntz) & ((1 << g.lgnty) - 1);
+        const int txid = t >> (g.lgntz + g.lgnty);
+        const int ox = txid << g.lgtx, oy = tyid << g.lgty, oz = tzid << g.lgtz;
+        auto cell_base = [&](int i2) -> int64_t {
+            const int lz = i2 & (tz - 1), ly = (i2 >> g.lgtz) & (ty - 1), lx = i2 >> (g.lgtz + g.lgty);
+            return ((int64_t)(ox + lx) * L + (oy + ly)) * L + (oz + lz);
+        };
+        __syncthreads();  // the previous tile's taps and output phase are done with srec and acc
+        // this tile's first chunk: prefetched registers -> LDS; then prefetch the next tile's first chunk
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (beg + tid + k * ACC_THREADS < end) srec[tid + k * ACC_THREADS] = pref[k];
+#pragma unroll
+        for (int k = 0; k < RPT; ++k)
+            if (nbeg + tid + k * ACC_THREADS < nend) pref[k] = rec[nbeg + tid + k * ACC_THREADS];
+        if (beg == end) {
+            for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
+                const int64_t gi = cell_base(i2);
+                if (out32) *(float2*)(out32 + gi) = make_float2(0.f, 0.f);
+                if (out64) *(double2*)(out64 + gi) = make_double2(0.0, 0.0);
+                if (COUNTS) *(int2*)(counts_out + gi) = make_int2(0, 0);
+            }
+        } else {
+            const double bound = (double)(end - beg) * unit;
+            int e = 0;
+            frexp(bound, &e);
+            int S = 61 - e;
+            if (S > 1000) S = 1000;  // keep 2^S finite for vanishing weights
+            const double mult = ldexp(1.0, S), inv = ldexp(1.0, -S);
+            const bool f32_direct = out32 && !out64 && S <= 100;
+            const float two_32mS = ldexpf(1.0f, 32 - S), two_mS = ldexpf(1.0f, -S);
+            for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
+                *(longlong2*)&acc[i2] = make_longlong2(0, 0);
+                if (COUNTS) *(int2*)&cnt[i2] = make_int2(0, 0);
+            }
+            __syncthreads();
+            // (record, tap) pairs spread over the workgroup, product order ((w phi_x) phi_y) phi_z; records
+            // beyond the first chunk (rare) are staged synchronously
+            for (int rb = beg; rb < end; rb += ACC_RCHUNK) {
+                const int nr = end - rb < ACC_RCHUNK ? end - rb : ACC_RCHUNK;
+                if (rb != beg) {
+                    for (int r = tid; r < nr; r += ACC_THREADS) srec[r] = rec[rb + r];
+                    __syncthreads();
+                }
+                // items (record, x tap); y and z taps limited to the tile; 2^S folded into w (exact), so
+                // each contribution is round((((w 2^S) phi_x) phi_y) phi_z)
+                for (int it = tid; it < nr * 3; it += ACC_THREADS) {
+                    const int r = it / 3, sx = it - 3 * r;
+                    const ProjRec R = srec[r];
+                    const int cx = R.cell & 1023;
+                    const int lx = ((cx - sx) & m) - ox;
+                    if (lx < 0 || lx >= tx) continue;
+                    const int cy = (R.cell >> 10) & 1023, cz = (R.cell >> 20) & 1023;
+                    const int fx = R.frac & 1023, fy = (R.frac >> 10) & 1023, fz = (R.frac >> 20) & 1023;
+                    const double phx = (R.w * mult) * sphi[fx + res * sx];
+                    const int rowx = lx << g.lgty;
+#pragma unroll
+                    for (int sy = 0; sy < 3; ++sy) {
+                        const int ly = ((cy - sy) & m) - oy;
+                        if (ly < 0 || ly >= ty) continue;
+                        const double phxy = phx * sphi[fy + res * sy];
+                        const int rowxy = (rowx + ly) << g.lgtz;
+#pragma unroll
+                        for (int sz = 0; sz < 3; ++sz) {
+                            const int lz = ((cz - sz) & m) - oz;
+                            if (lz < 0 || lz >= tz) continue;
+                            const long long q = __double2ll_rn(phxy * sphi[fz + res * sz]);
+                            atomicAdd((unsigned long long*)&acc[rowxy + lz], (unsigned long long)q);
+                        }
+                    }
+                }
+                if (COUNTS) {
+                    for (int r = tid; r < nr; r += ACC_THREADS) {
+                        const ProjRec R = srec[r];
+                        const int hx = (int)(R.cell & 1023) - ox, hy = (int)((R.cell >> 10) & 1023) - oy;
+                        const int hz = (int)((R.cell >> 20) & 1023) - oz;
+                        if (hx >= 0 && hx < tx && hy >= 0 && hy < ty && hz >= 0 && hz < tz)
+                            atomicAdd(&cnt[(((hx << g.lgty) + hy) << g.lgtz) + hz], 1);
+                    }
+                }
+                __syncthreads();  // chunk consumed; after the last chunk the accumulators are complete
+            }
+            if (f32_direct && !COUNTS) {
+                for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
+                    const longlong2 a = *(const longlong2*)&acc[i2];
+                    *(float2*)(out32 + cell_base(i2)) =
+                        make_float2(fixed_to_f32(a.x, two_32mS, two_mS), fixed_to_f32(a.y, two_32mS, two_mS));
+                }
+            } else {
+                for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
+                    const int64_t gi = cell_base(i2);
+                    const longlong2 a = *(const longlong2*)&acc[i2];
+                    const double v0 = (double)a.x * inv, v1 = (double)a.y * inv;
+                    if (out32) *(float2*)(out32 + gi) = make_float2((float)v0, (float)v1);
+                    if (out64) *(double2*)(out64 + gi) = make_double2(v0, v1);
+                    if (COUNTS) *(int2*)(counts_out + gi) = *(const int2*)&cnt[i2];
                 }
             }
         }
-        if (counts_out) {
-            int hx = cx - ox, hy = cy - oy, hz = cz - oz;
-            if (hx >= 0 && hx < g.tx && hy >= 0 && hy < g.ty && hz >= 0 && hz < g.tz)
-                atomicAdd(&cnt[(hx * g.ty + hy) * g.tz + hz], 1);
-        }
+        beg = nbeg;
+        end = nend;
+        nbeg = nnbeg;
+        nend = nnend;
     }
-    __syncthreads();
-    for (int i = tid; i < ncell; i += blockDim.x) {
-        int lz = i % g.tz, ly = (i / g.tz) % g.ty, lx = i / (g.tz * g.ty);
-        int64_t gi = ((ox + lx) * L + (oy + ly)) * L + (oz + lz);
-        double v = (double)(long long)acc[i] * inv;
-        if (out32) out32[gi] = (float)v;
-        if (out64) out64[gi] = v;
-        if (counts_out) counts_out[gi] = cnt[i];
-    }
+}
+
+int ilog2(int v) {
+    int r = 0;
+    while ((1 << r) < v) ++r;
+    return r;
 }
 
 }  // namespace
@@ -215,7 +404,7 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
                               const double* phi, int phi_len, int phi_res, double phi_abs_max, float* eps32,
                               double* eps64, int32_t* cell_out, int32_t* frac_out, int32_t* cell_counts) {
     HERMES_REQUIRE(ctx, "null context");
-    HERMES_REQUIRE(J >= 2 && J <= 10, "hermes_project: J=%d outside the supported range 2..10", J);
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_project: J=%d outside the supported range 3..10", J);
     HERMES_REQUIRE(phi_res == 1024 && phi_len == 3 * phi_res,
                    "hermes_project: expects a support-3 scaling function sampled at 1024 (got %d samples, res %d)",
                    phi_len, phi_res);
@@ -226,32 +415,43 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
     const int L = 1 << J;
     TileGeom g;
     g.L = L;
-    g.tx = L < MAX_TX ? L : MAX_TX;
-    g.ty = L < MAX_TY ? L : MAX_TY;
-    g.tz = L < MAX_TZ ? L : MAX_TZ;
-    g.ntx = L / g.tx;
-    g.nty = L / g.ty;
-    g.ntz = L / g.tz;
-    const int ntiles = g.ntx * g.nty * g.ntz;
+    g.lgL = J;
+    g.lgtx = ilog2(L < MAX_TX ? L : MAX_TX);
+    g.lgty = ilog2(L < MAX_TY ? L : MAX_TY);
+    g.lgtz = ilog2(L < MAX_TZ ? L : MAX_TZ);
+    g.lgnty = J - g.lgty;
+    g.lgntz = J - g.lgtz;
+    const int ntiles = 1 << (3 * J - g.lgtx - g.lgty - g.lgtz);
     const double sf = (double)L / box;
     hipStream_t st = ctx->stream;
 
-    // tile workspace: count | maxw(u64) | off(ntiles+1) | cursor | err + total readback
-    size_t tile_bytes = (size_t)ntiles * 4 * 3 + 64 + (size_t)ntiles * 8 + 64;
+    // tile workspace: (unused u64 slot) | count | off(ntiles+1) | cursor | err | scan block sums (i64)
+    const int nscan = (ntiles + SCAN_THREADS * SCAN_ITEMS - 1) / (SCAN_THREADS * SCAN_ITEMS);
+    size_t tile_bytes = (size_t)ntiles * 8 + (size_t)ntiles * 4 * 3 + 64 + (size_t)(nscan + 1) * 8 + 64;
     char* tw = (char*)hermes_workspace(ctx, WS_PROJ_TILE, tile_bytes);
     HERMES_REQUIRE(tw, "%s", hermes_last_error());
-    unsigned long long* maxw = (unsigned long long*)tw;
     int* count = (int*)(tw + (size_t)ntiles * 8);
     int* off = count + ntiles;
     int* cursor = off + ntiles + 1;
     int* err = cursor + ntiles;
-    HERMES_CHECK_HIP(hipMemsetAsync(tw, 0, (size_t)ntiles * 8 + (size_t)ntiles * 4, st));
+    long long* bsum = (long long*)(tw + (((size_t)ntiles * 8 + (size_t)ntiles * 4 * 3 + 16 + 15) / 16) * 16);
+    HERMES_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)ntiles * 4, st));
+    unsigned long long* wmax = (unsigned long long*)tw;  // first 8 bytes of the u64 slot
+    HERMES_CHECK_HIP(hipMemsetAsync(wmax, 0, sizeof(unsigned long long), st));
+    if (n > 0) {
+        k_absmax<<<hermes_blocks(n, 256, 1024), 256, 0, st>>>(w, n, wmax);
+        HERMES_LAUNCH_CHECK();
+    }
     HERMES_CHECK_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
 
-    k_proj_count<<<hermes_blocks(n, 256, 4096), 256, 0, st>>>(pos, w, n, sf, phi_res, g, count, maxw, cell_out,
-                                                              frac_out, err);
+    k_proj_count<<<hermes_blocks(n, 256, 4096), 256, 0, st>>>(pos, n, sf, phi_res, g, count, cell_out, frac_out,
+                                                              err);
     HERMES_LAUNCH_CHECK();
-    k_proj_scan<<<1, 1024, 0, st>>>(count, ntiles, off, cursor);
+    k_scan_reduce<<<nscan, SCAN_THREADS, 0, st>>>(count, ntiles, bsum);
+    HERMES_LAUNCH_CHECK();
+    k_scan_blocks<<<1, 64, 0, st>>>(bsum, nscan);
+    HERMES_LAUNCH_CHECK();
+    k_scan_apply<<<nscan, SCAN_THREADS, 0, st>>>(count, ntiles, bsum, nscan, off, cursor);
     HERMES_LAUNCH_CHECK();
     int host[2] = {0, 0};
     HERMES_CHECK_HIP(hipMemcpyAsync(&host[0], err, sizeof(int), hipMemcpyDeviceToHost, st));
@@ -266,8 +466,25 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
         HERMES_LAUNCH_CHECK();
     }
     const double phimax3 = phi_abs_max * phi_abs_max * phi_abs_max * (1.0 + 1e-12);
-    k_proj_accumulate<<<ntiles, ACC_THREADS, 0, st>>>(rec, off, maxw, phi, phi_res, g, phimax3, eps32, eps64,
-                                                      cell_counts);
+    int n_cu = 256;
+    (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
+    const int acc_grid = ntiles < 3 * n_cu ? ntiles : 3 * n_cu;
+    const bool full = g.lgtx == 2 && g.lgty == 3 && g.lgtz == 6;
+    if (cell_counts) {
+        if (full)
+            k_proj_accumulate<true, true><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax, phimax3,
+                                                                             ntiles, eps32, eps64, cell_counts);
+        else
+            k_proj_accumulate<true, false><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax, phimax3,
+                                                                              ntiles, eps32, eps64, cell_counts);
+    } else {
+        if (full)
+            k_proj_accumulate<false, true><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax, phimax3,
+                                                                              ntiles, eps32, eps64, nullptr);
+        else
+            k_proj_accumulate<false, false><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax,
+                                                                               phimax3, ntiles, eps32, eps64, nullptr);
+    }
     HERMES_LAUNCH_CHECK();
     return 0;
 }
