@@ -84,8 +84,114 @@ def cpu_baseline(J_sample, lm_terms, shells_grid_at_sample, box, seed):
     return dt / len(lm_terms), threads
 
 
+def mc_cpu_baseline(delta, phi, bins, n, seed):
+    """Time the C/OpenMP oracle sampler (plain draws) on n triangles; returns (triangles/s, threads)."""
+    from oracle import mc as omc, native
+    t0 = time.perf_counter()
+    omc.binned_sampler(delta, delta, delta, phi, bins.shape, bins.r1, bins.r2, n, seed)
+    return n / (time.perf_counter() - t0), native.num_threads()
+
+
+def run_mc_workload(args):
+    """Monte-Carlo 3PCF with the learned proposal against uniform MC at equal triangle count.
+
+    The J=8 field of the 2,000,000-particle mock (delta = eps / mean - 1, FP64), triangles
+    binned in 15 x 15 (r1, r2) shells over 10-120 Mpc/h x 20 cos(theta) bins.  The MLP is
+    trained on the records of a plain run; then plain and learned runs of the same size
+    are compared bin by bin.  value = median over bins of the per-triangle variance ratio
+    (triangles the uniform sampler needs for the learned sampler's variance).
+    """
+    import numpy as np
+    import torch
+
+    from pyhermes_amd import kernels, mcsample
+    from pyhermes_amd.mocks import lognormal_mock
+    from pyhermes_amd.utils.wavelet_grid import sample_scaling_function
+
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    torch.cuda.set_device(0)
+    box, J = 1000.0, args.mc_J
+    L = 1 << J
+    log(f"mock: {args.particles} particles, ng=256, box={box}; projection J={J}")
+    pos, _ = lognormal_mock(args.particles, 256, box, 3000)
+    phi = sample_scaling_function()
+    proj = kernels.project(kernels.as_device(pos, torch.float64),
+                           kernels.as_device(np.ones(pos.shape[0]), torch.float64), J, box, phi,
+                           want32=False, want64=True)
+    eps = proj["eps64"]
+    delta = (eps / eps.mean() - 1.0).contiguous()
+    del proj, eps
+    phi_d = kernels.as_device(phi, torch.float64)
+    s = L / box
+    bins = mcsample.TriangleBins(15, 15, 20, 10.0 * s, 120.0 * s)
+    n_train, n_run = int(args.mc_train), int(args.mc_triangles)
+
+    def timed(fn):
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        r = fn()
+        torch.cuda.synchronize()
+        return r, time.perf_counter() - t0
+
+    mcsample.run_mc(delta, delta, delta, phi_d, bins, 1 << 20, 1)  # warm
+    prop, t_buckets = timed(lambda: mcsample.LearnedProposal(bins, delta, phi_d, nbucket=args.mc_buckets,
+                                                             alpha=args.mc_alpha, sub=args.mc_sub,
+                                                             anchors=args.mc_anchors, seed=1))
+    log(f"proposal buckets on the {L * args.mc_sub}^3 sub-cell grid: {t_buckets:.2f} s")
+    rec = torch.empty((n_train, 8), dtype=torch.float32, device=delta.device)
+    pilot, t_pilot = timed(lambda: mcsample.run_mc(delta, delta, delta, phi_d, bins, n_train, 11,
+                                                   proposal=prop.recording, record=rec))
+    log(f"plain pilot with records: {n_train:.2e} triangles in {t_pilot:.2f} s")
+    losses, t_fit = timed(lambda: prop.fit(rec, pilot, epochs=args.mc_epochs, log=log))
+    _, t_build = timed(lambda: prop.build(args.mc_allocation))
+    del rec
+    log(f"MLP trained in {t_fit:.2f} s (losses {[round(v, 4) for v in losses]}), tables in {t_build:.2f} s")
+    plain, t_plain = timed(lambda: mcsample.run_mc(delta, delta, delta, phi_d, bins, n_run, 12))
+    log(f"uniform MC: {n_run:.2e} triangles in {t_plain:.2f} s")
+    imp, t_imp = timed(lambda: mcsample.run_mc(delta, delta, delta, phi_d, bins, n_run, 13, proposal=prop))
+    log(f"learned MC: {n_run:.2e} triangles in {t_imp:.2f} s")
+    cmp = mcsample.compare(plain, imp)
+    ok = (plain.variance > 0) & (imp.variance > 0)
+    gain = (plain.variance[ok] / imp.variance[ok]) * (imp.n / plain.n)
+    log(f"compare: {cmp}")
+    cpu = None
+    if not args.skip_cpu:
+        n_cpu = int(args.mc_cpu_triangles)
+        log(f"CPU oracle sampler: {n_cpu:.1e} plain triangles")
+        rate, threads = mc_cpu_baseline(delta.cpu().numpy(), phi, bins, n_cpu, 12)
+        cpu = {"value": rate, "unit": "triangles/s", "cores": threads, "kind": "port",
+               "sample": f"C/OpenMP oracle binned sampler, {n_cpu:.0e} uniform triangles on the same field"}
+    line = {
+        "metric": "MC 3PCF triangles saved by the learned proposal at equal variance (median over bins)",
+        "value": float(np.median(gain)),
+        "unit": "x fewer triangles",
+        "n_gpus": 1,
+        "higher_is_better": True,
+        "dtype": "f64 sampling, bf16/fp32 MLP",
+        "data": "synthetic lognormal mock (seeded)",
+        "config": {"workload": f"mc3pcf_J{J}_{n_run:.0e}_triangles_15x15x20", "particles": args.particles,
+                   "J": J, "bins": [15, 15, 20], "r_range_mpc": [10.0, 120.0], "train_records": n_train,
+                   "triangles_per_run": n_run, "nbucket": args.mc_buckets, "alpha": args.mc_alpha,
+                   "sub": args.mc_sub, "anchors": args.mc_anchors, "allocation": args.mc_allocation,
+                   "epochs": args.mc_epochs},
+        "gain": {"median": float(np.median(gain)), "mean": cmp["mean_gain"], "total": cmp["total_gain"],
+                 "p10": float(np.percentile(gain, 10)), "p90": float(np.percentile(gain, 90)),
+                 "frac_bins_ge_3": float(np.mean(gain >= 3.0)), "bins": int(ok.sum())},
+        "bias": {"z_mean": cmp["z_mean"], "z_rms": cmp["z_rms"]},
+        "triangles_per_s": {"uniform": n_run / t_plain, "learned": n_run / t_imp,
+                            "pilot_with_records": n_train / t_pilot},
+        "seconds": {"buckets": t_buckets, "pilot": t_pilot, "train": t_fit, "build": t_build, "uniform": t_plain,
+                    "learned": t_imp},
+        "mlp_losses": [float(v) for v in losses],
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", choices=["sh3pcf", "mc"], default="sh3pcf",
+                    help="sh3pcf: the multipole sets/s headline; mc: learned vs uniform Monte-Carlo 3PCF")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -97,10 +203,23 @@ def main():
     ap.add_argument("--skip-cpu", action="store_true")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (HBM counter calibration)")
+    ap.add_argument("--mc-J", type=int, default=8)
+    ap.add_argument("--mc-triangles", type=float, default=1e8)
+    ap.add_argument("--mc-train", type=float, default=1e7)
+    ap.add_argument("--mc-epochs", type=int, default=3)
+    ap.add_argument("--mc-buckets", type=int, default=64)
+    ap.add_argument("--mc-alpha", type=float, default=0.05)
+    ap.add_argument("--mc-sub", type=int, default=4)
+    ap.add_argument("--mc-anchors", type=int, choices=[1, 3], default=3)
+    ap.add_argument("--mc-allocation", choices=["uniform", "relative"], default="uniform")
+    ap.add_argument("--mc-cpu-triangles", type=float, default=1e8)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1", "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.workload == "mc":
+        run_mc_workload(args)
+        return
 
     import numpy as np
     import torch

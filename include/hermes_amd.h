@@ -189,36 +189,57 @@ int hermes_mc_triplet_box(hermes_ctx *ctx, int J, double R1, double R2, double m
                           const double *n1, int64_t n, int n_rot, const double *abc_host, const double *g2,
                           const double *g3, const double *phi, int phi_res, double *out_host);
 
-/* Importance proposal tables for hermes_mc_binned (device arrays): cells are
- * grouped into nbucket density buckets; bucket k is drawn with CDF bucket_cdf,
- * a cell uniformly from bucket_cells[bucket_off[k] .. bucket_off[k+1]), a point
- * uniformly in the cell, and the (r1, r2, mu) bin from bin_cdf[k]; the density
- * relative to the uniform one is bucket_density[k] * bin_pdf[k][bin].  With
- * probability alpha a sample is drawn from the uniform proposal instead. */
+/* Importance proposal tables for hermes_mc_binned (device arrays).  The box is
+ * cut into Ls^3 sub-cells (Ls = L * sub) grouped into nbucket density buckets.
+ * A proposal draw picks bucket k with CDF bucket_cdf, a sub-cell uniformly from
+ * bucket_cells[bucket_off[k] .. bucket_off[k+1]), a point p uniformly in it and
+ * the (r1, r2, mu) bin from bin_cdf[k]; p becomes the apex (anchors = 1) or a
+ * uniformly chosen vertex (anchors = 3) of the triangle.  Relative to the uniform
+ * density an anchor proposes R(k, b) = bucket_density[k] * bin_pdf[k][b] with k
+ * the bucket of that vertex; the mixture weight is
+ * w = 1 / (alpha + (1 - alpha) * mean over anchors of R).  With probability
+ * alpha a sample is drawn uniformly instead (alpha = 1: plain draws that still
+ * record each vertex's bucket). */
 typedef struct {
-    const uint8_t *cell_bucket;     /* (L^3) bucket of each cell */
-    const int32_t *bucket_cells;    /* (L^3) cells ordered by bucket */
+    const uint8_t *cell_bucket;     /* (Ls^3) bucket of each sub-cell */
+    const int32_t *bucket_cells;    /* (Ls^3) sub-cells ordered by bucket */
     const int64_t *bucket_off;      /* (nbucket + 1) */
     const float *bucket_cdf;        /* (nbucket) */
-    const double *bucket_density;   /* (nbucket) V B P(k) / n_k */
+    const double *bucket_density;   /* (nbucket) Ls^3 B P(k) / n_k */
     const float *bin_cdf;           /* (nbucket, B) */
-    const float *bin_pdf;           /* (nbucket, B), sums to 1 over bins */
-    int32_t nbucket;
-    int32_t pad;
-    double alpha;
+    const float *bin_pdf;           /* (nbucket, B) */
+    int32_t nbucket;                /* 1..256 */
+    int32_t sub;                    /* 1, 2, 4 or 8; L * sub <= 1024 */
+    double alpha;                   /* (0, 1] */
+    int32_t anchors;                /* 1 or 3 */
+    int32_t reserved;
 } hermes_mc_proposal;
+
+/* Proposal buckets on the sub-cell grid (no reference counterpart): bucket of
+ * sub-cell s = number of edges_host[0 .. nbucket-2] (ascending) that are <= the
+ * interpolated field (wavelet_grid.py:143-186 evaluation) at the centre of s;
+ * cell_bucket (Ls^3), bucket_cells (Ls^3, sub-cells ascending within each
+ * bucket) and bucket_off (nbucket + 1) are device outputs, counts_host the
+ * sub-cells per bucket.  Deterministic. */
+int hermes_mc_bucketize(hermes_ctx *ctx, int J, const double *grid, const double *phi, int phi_res, int sub,
+                        const double *edges_host, int nbucket, uint8_t *cell_bucket, int32_t *bucket_cells,
+                        int64_t *bucket_off, int64_t *counts_host);
 
 /* Binned Monte-Carlo triangle sampler (no reference counterpart: the reference
  * estimates one fixed triangle per call).  Triangles (r1, r2, mu) in
  * nr1 x nr2 x nmu uniform bins over [rmin, rmax) (grid units) and [-1, 1), centres
  * in the box, orientations uniform; f = n1 n2 n3 from grids g1, g2, g3.  prop == NULL
  * draws everything uniformly; otherwise the importance mixture above with weights
- * w = p / q.  Outputs per bin: sum w f, sum (w f)^2 and counts (deterministic).
- * wf_bound must bound |w f| for every sample.  record (n_record x 6 floats:
- * r1, r2, mu, g1[cell], f, bin) is optional. */
+ * w = p / q.  Outputs per bin: sum w f, sum (w f)^2 and counts, accumulated in
+ * int64 fixed point (deterministic).  The fixed-point scale comes from a cap on
+ * |w f|: wf_cap > 0 gives it, wf_cap == 0 takes the maximum over the first 2^20
+ * samples.  Samples above the cap are summed in a second pass at the scale of
+ * the overall maximum, so no cap changes which samples count.  record
+ * (n_record x 8 floats: r1, r2, mu, f, bin and the proposal bucket of each
+ * vertex, -1 without tables) is optional. */
 int hermes_mc_binned(hermes_ctx *ctx, int J, const double *g1, const double *g2, const double *g3, const double *phi,
                      int phi_res, int nr1, int nr2, int nmu, double rmin1, double rmax1, double rmin2, double rmax2,
-                     int64_t n_samples, uint64_t seed, double wf_bound, const hermes_mc_proposal *prop, float *record,
+                     int64_t n_samples, uint64_t seed, double wf_cap, const hermes_mc_proposal *prop, float *record,
                      int64_t n_record, double *sum_wf_host, double *sum_wf2_host, int64_t *count_host);
 
 /* ---- learned importance proposal: 3-layer MLP (4 -> 64 -> 64 -> 1, SiLU),

@@ -402,33 +402,43 @@ void oracle_summand(const double *data, const double *F1, const double *F2, cons
  * corr3pcf_kernels.py:21-63 (generate_triangle_offsets), :100-140 (box-random
  * centres).  The per-rotation uniforms (a, b, c) are drawn by the caller.
  * ------------------------------------------------------------------------- */
+/* One point.  A coordinate in (-2^-54, 0) rounds s - floor(s) to 1.0; the
+ * fraction index is clamped to phi_res - 1 there (the reference indexes past
+ * its table in that case). */
+static double interp_one(const double *grid, const double *phi, int L, int phi_res, int support,
+                         double sx, double sy, double sz) {
+    const int64_t mask = L - 1;
+    int64_t xc = (int64_t)floor(sx), yc = (int64_t)floor(sy), zc = (int64_t)floor(sz);
+    int64_t xf = (int64_t)((sx - (double)xc) * phi_res);
+    int64_t yf = (int64_t)((sy - (double)yc) * phi_res);
+    int64_t zf = (int64_t)((sz - (double)zc) * phi_res);
+    if (xf > phi_res - 1) xf = phi_res - 1;
+    if (yf > phi_res - 1) yf = phi_res - 1;
+    if (zf > phi_res - 1) zf = phi_res - 1;
+    double acc = 0.0;
+    for (int a = 0; a < support; ++a) {
+        int64_t xg = (xc - a) & mask;
+        double px = phi[xf + a * phi_res];
+        for (int b = 0; b < support; ++b) {
+            int64_t yg = (yc - b) & mask;
+            double py = phi[yf + b * phi_res];
+            for (int c = 0; c < support; ++c) {
+                int64_t zg = (zc - c) & mask;
+                double pz = phi[zf + c * phi_res];
+                acc += grid[(xg * L + yg) * L + zg] * px * py * pz;
+            }
+        }
+    }
+    return acc;
+}
+
 void oracle_interpolate(double *out, const double *spos, int64_t n, const double *grid,
                         const double *phi, int L, int phi_res, int support,
                         double dx, double dy, double dz) {
-    const int64_t mask = L - 1;
 #pragma omp parallel for schedule(static)
-    for (int64_t p = 0; p < n; ++p) {
-        double sx = spos[3 * p] + dx, sy = spos[3 * p + 1] + dy, sz = spos[3 * p + 2] + dz;
-        int64_t xc = (int64_t)floor(sx), yc = (int64_t)floor(sy), zc = (int64_t)floor(sz);
-        int64_t xf = (int64_t)((sx - (double)xc) * phi_res);
-        int64_t yf = (int64_t)((sy - (double)yc) * phi_res);
-        int64_t zf = (int64_t)((sz - (double)zc) * phi_res);
-        double acc = 0.0;
-        for (int a = 0; a < support; ++a) {
-            int64_t xg = (xc - a) & mask;
-            double px = phi[xf + a * phi_res];
-            for (int b = 0; b < support; ++b) {
-                int64_t yg = (yc - b) & mask;
-                double py = phi[yf + b * phi_res];
-                for (int c = 0; c < support; ++c) {
-                    int64_t zg = (zc - c) & mask;
-                    double pz = phi[zf + c * phi_res];
-                    acc += grid[(xg * L + yg) * L + zg] * px * py * pz;
-                }
-            }
-        }
-        out[p] = acc;
-    }
+    for (int64_t p = 0; p < n; ++p)
+        out[p] = interp_one(grid, phi, L, phi_res, support, spos[3 * p] + dx, spos[3 * p + 1] + dy,
+                            spos[3 * p + 2] + dz);
 }
 
 void oracle_triangle_offsets(double R1, double R2, double mu, double phi, double ct1, double alpha,
@@ -475,4 +485,143 @@ double oracle_triplet_box_random(double R1, double R2, double mu, const double *
     }
     free(n1); free(n2); free(n3);
     return total / ((double)n_rot * (double)n);
+}
+
+/* ---------------------------------------------------------------------------
+ * Binned Monte-Carlo triangle sampler.  No reference counterpart as a whole:
+ * it bins the reference's triangle products n1 n2 n3 (geometry
+ * corr3pcf_kernels.py:21-63, interpolation wavelet_grid.py:143-186) over
+ * (r1, r2, cos theta) bins, with uniforms from Philox4x32-10 (Salmon, Moraes,
+ * Dror, Shaw, "Parallel random numbers: as easy as 1, 2, 3", SC'11).
+ * Sample i, stream s: counter (i_lo, i_hi, 0x9E3779B9, 0x85EBCA6B), key
+ * seed ^ (0x9E3779B97F4A7C15 * (s + 1)); uniform = 53 bits from two words.
+ * ------------------------------------------------------------------------- */
+void oracle_philox4x32_10(const uint32_t *ctr, const uint32_t *key, uint32_t *out) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3], k0 = key[0], k1 = key[1];
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0, p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0, n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1, n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+static void mc_uniforms2(uint64_t i, uint64_t seed, uint64_t stream, double *a, double *b) {
+    const uint64_t key = seed ^ (0x9E3779B97F4A7C15ULL * (stream + 1));
+    const uint32_t ctr[4] = {(uint32_t)i, (uint32_t)(i >> 32), 0x9E3779B9u, 0x85EBCA6Bu};
+    const uint32_t k[2] = {(uint32_t)key, (uint32_t)(key >> 32)};
+    uint32_t r[4];
+    oracle_philox4x32_10(ctr, k, r);
+    *a = ((double)(r[0] >> 5) * 67108864.0 + (double)(r[1] >> 6)) / 9007199254740992.0;
+    *b = ((double)(r[2] >> 5) * 67108864.0 + (double)(r[3] >> 6)) / 9007199254740992.0;
+}
+
+/* first index with cdf[i] > u, clamped to n - 1 */
+static int mc_upper_bound(const float *cdf, int n, float u) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (cdf[mid] > u) hi = mid; else lo = mid + 1;
+    }
+    return lo < n ? lo : n - 1;
+}
+
+static double mc_interp(const double *g, const double *phi, int L, int res, double sx, double sy, double sz) {
+    return interp_one(g, phi, L, res, 3, sx, sy, sz);
+}
+
+/* Plain estimator when cell_bucket is NULL; otherwise the defensive-mixture
+ * importance proposal on the sub-cell grid (side Ls = L sub): with probability
+ * alpha a uniform draw, else bucket k ~ bucket_cdf, a uniform sub-cell of
+ * bucket k, a uniform point p in it, bin ~ bin_cdf[k] and an anchor vertex a
+ * (the apex when anchors = 1, else uniform over the three) placed at p.  With
+ * R(k, b) = bucket_density[k] bin_pdf[k][b] and k_a the bucket of vertex a,
+ * w = 1 / (alpha + (1 - alpha) mean_a R(k_a, bin)) (apex only when anchors = 1).
+ * Outputs per-bin sum w f, sum (w f)^2 and counts (B = nr1 nr2 nmu,
+ * bin = (ir1 nr2 + ir2) nmu + imu). */
+void oracle_mc_binned(const double *g1, const double *g2, const double *g3, const double *phi, int L,
+                      int phi_res, int nr1, int nr2, int nmu, double rmin1, double rmax1, double rmin2,
+                      double rmax2, int64_t n, uint64_t seed, const uint8_t *cell_bucket,
+                      const int32_t *bucket_cells, const int64_t *bucket_off, const float *bucket_cdf,
+                      const double *bucket_density, const float *bin_cdf, const float *bin_pdf, int nbucket,
+                      double alpha, int sub, int anchors, double *sum_wf, double *sum_wf2, int64_t *count) {
+    const int B = nr1 * nr2 * nmu;
+    const int64_t Ls = (int64_t)L * sub, lm = Ls - 1;
+    const int imp = cell_bucket != NULL;
+    const double dr1 = (rmax1 - rmin1) / nr1, dr2 = (rmax2 - rmin2) / nr2, dmu = 2.0 / nmu;
+    memset(sum_wf, 0, sizeof(double) * B);
+    memset(sum_wf2, 0, sizeof(double) * B);
+    memset(count, 0, sizeof(int64_t) * B);
+#pragma omp parallel
+    {
+        double *s1 = (double *)calloc(B, sizeof(double));
+        double *s2 = (double *)calloc(B, sizeof(double));
+        int64_t *cn = (int64_t *)calloc(B, sizeof(int64_t));
+#pragma omp for schedule(static)
+        for (int64_t i = 0; i < n; ++i) {
+            double ux, uy, uz, ubin, upick, umix, ubin2, w1, w2, w3, oph, oct, oal, uanc;
+            mc_uniforms2((uint64_t)i, seed, 0, &ux, &uy);
+            mc_uniforms2((uint64_t)i, seed, 1, &uz, &ubin);
+            mc_uniforms2((uint64_t)i, seed, 2, &upick, &umix);
+            mc_uniforms2((uint64_t)i, seed, 3, &ubin2, &w1);
+            mc_uniforms2((uint64_t)i, seed, 4, &w2, &w3);
+            mc_uniforms2((uint64_t)i, seed, 5, &oph, &oct);
+            mc_uniforms2((uint64_t)i, seed, 6, &oal, &uanc);
+            double px, py, pz;
+            int bin, anchor = 0;
+            if (!imp || umix < alpha) {
+                px = ux * L; py = uy * L; pz = uz * L;
+                bin = (int)(ubin * B);
+                if (bin >= B) bin = B - 1;
+            } else {
+                const int k = mc_upper_bound(bucket_cdf, nbucket, (float)ubin);
+                const int64_t c0 = bucket_off[k], c1 = bucket_off[k + 1];
+                int64_t pick = c0 + (int64_t)(upick * (double)(c1 - c0));
+                if (pick >= c1) pick = c1 - 1;
+                const int64_t sc = bucket_cells[pick];
+                px = (double)(sc / (Ls * Ls) + ux) / sub;
+                py = (double)((sc / Ls) % Ls + uy) / sub;
+                pz = (double)(sc % Ls + uz) / sub;
+                bin = mc_upper_bound(bin_cdf + (int64_t)k * B, B, (float)ubin2);
+                if (anchors == 3) { anchor = (int)(uanc * 3.0); if (anchor > 2) anchor = 2; }
+            }
+            const int imu = bin % nmu, ir2 = (bin / nmu) % nr2, ir1 = bin / (nmu * nr2);
+            const double r1 = rmin1 + (ir1 + w1) * dr1, r2 = rmin2 + (ir2 + w2) * dr2;
+            const double mu = -1.0 + (imu + w3) * dmu;
+            double o[6];
+            oracle_triangle_offsets(r1, r2, mu, 2.0 * M_PI * oph, 2.0 * oct - 1.0, 2.0 * M_PI * oal, o);
+            double cx = px, cy = py, cz = pz;
+            if (anchor == 1) { cx = px - o[0]; cy = py - o[1]; cz = pz - o[2]; }
+            if (anchor == 2) { cx = px - o[3]; cy = py - o[4]; cz = pz - o[5]; }
+            const double v[3][3] = {{cx, cy, cz}, {cx + o[0], cy + o[1], cz + o[2]},
+                                    {cx + o[3], cy + o[4], cz + o[5]}};
+            const double f = mc_interp(g1, phi, L, phi_res, v[0][0], v[0][1], v[0][2]) *
+                             mc_interp(g2, phi, L, phi_res, v[1][0], v[1][1], v[1][2]) *
+                             mc_interp(g3, phi, L, phi_res, v[2][0], v[2][1], v[2][2]);
+            double w = 1.0;
+            if (imp) {
+                double rel = 0.0;
+                const int na = anchors == 3 ? 3 : 1;
+                for (int a = 0; a < na; ++a) {
+                    const int64_t s3 = ((((int64_t)floor(v[a][0] * sub)) & lm) * Ls +
+                                        (((int64_t)floor(v[a][1] * sub)) & lm)) * Ls +
+                                       (((int64_t)floor(v[a][2] * sub)) & lm);
+                    const int k = cell_bucket[s3];
+                    rel += bucket_density[k] * (double)bin_pdf[(int64_t)k * B + bin];
+                }
+                if (na == 3) rel *= (1.0 / 3.0);
+                w = 1.0 / (alpha + (1.0 - alpha) * rel);
+            }
+            s1[bin] += w * f;
+            s2[bin] += (w * f) * (w * f);
+            cn[bin] += 1;
+        }
+#pragma omp critical
+        {
+            for (int b = 0; b < B; ++b) { sum_wf[b] += s1[b]; sum_wf2[b] += s2[b]; count[b] += cn[b]; }
+        }
+        free(s1); free(s2); free(cn);
+    }
 }

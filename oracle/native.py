@@ -50,6 +50,14 @@ def lib():
         L.oracle_triplet_box_random.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double, _dp,
                                                 ctypes.c_int64, ctypes.c_int, _dp, _dp, _dp, _dp, _dp,
                                                 ctypes.c_int, ctypes.c_int, ctypes.c_int]
+        u32p = ctypes.POINTER(ctypes.c_uint32)
+        L.oracle_philox4x32_10.argtypes = [u32p, u32p, u32p]
+        L.oracle_mc_binned.argtypes = ([_dp] * 4 + [ctypes.c_int] * 5 + [ctypes.c_double] * 4 +
+                                       [ctypes.c_int64, ctypes.c_uint64, ctypes.POINTER(ctypes.c_uint8), _i32p,
+                                        ctypes.POINTER(ctypes.c_int64), ctypes.POINTER(ctypes.c_float), _dp,
+                                        ctypes.POINTER(ctypes.c_float), ctypes.POINTER(ctypes.c_float),
+                                        ctypes.c_int, ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp, _dp,
+                                        ctypes.POINTER(ctypes.c_int64)])
         _lib = L
     return _lib
 

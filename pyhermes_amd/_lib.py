@@ -66,6 +66,8 @@ _SIGNATURES.update({
     "hermes_mc_binned": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, c_void_p, c_int, c_int, c_int, c_int, c_double,
                          c_double, c_double, c_double, c_i64, ctypes.c_uint64, c_double, c_void_p, c_void_p, c_i64,
                          _dpp, _dpp, ctypes.POINTER(ctypes.c_int64)],
+    "hermes_mc_bucketize": [c_void_p, c_int, c_void_p, c_void_p, c_int, c_int, _dpp, c_int, c_void_p, c_void_p,
+                            c_void_p, ctypes.POINTER(ctypes.c_int64)],
     "hermes_mlp_nparam": [],
     "hermes_mlp_to_bf16": [c_void_p, c_void_p, c_void_p],
     "hermes_mlp_forward": [c_void_p, c_void_p, c_void_p, c_int, c_void_p],
@@ -79,8 +81,8 @@ class MCProposal(ctypes.Structure):
     """hermes_mc_proposal (include/hermes_amd.h)."""
     _fields_ = [("cell_bucket", c_void_p), ("bucket_cells", c_void_p), ("bucket_off", c_void_p),
                 ("bucket_cdf", c_void_p), ("bucket_density", c_void_p), ("bin_cdf", c_void_p),
-                ("bin_pdf", c_void_p), ("nbucket", ctypes.c_int32), ("pad", ctypes.c_int32),
-                ("alpha", c_double)]
+                ("bin_pdf", c_void_p), ("nbucket", ctypes.c_int32), ("sub", ctypes.c_int32),
+                ("alpha", c_double), ("anchors", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 _RESTYPES = {"hermes_last_error": ctypes.c_char_p}
 
 

@@ -295,8 +295,11 @@ def mc_triplet_box(R1, R2, mu, spos, n1, abc, grid2, grid3, phi):
     return float(out[0])
 
 
-def mc_binned(g1, g2, g3, phi, nbins, rrange1, rrange2, n_samples, seed, wf_bound, proposal=None, record=None):
-    """Binned triangle sampler; returns (sum_wf, sum_wf2, counts) as (nr1, nr2, nmu) host arrays."""
+def mc_binned(g1, g2, g3, phi, nbins, rrange1, rrange2, n_samples, seed, wf_cap=0.0, proposal=None, record=None):
+    """Binned triangle sampler; returns (sum_wf, sum_wf2, counts) as (nr1, nr2, nmu) host arrays.
+
+    ``wf_cap`` sets the fixed-point scale of the sums (0: measured on a pilot pass).
+    """
     L = g1.shape[0]
     phi_t = phi if isinstance(phi, torch.Tensor) else as_device(np.asarray(phi, dtype=np.float64), torch.float64)
     nr1, nr2, nmu = (int(v) for v in nbins)
@@ -305,14 +308,35 @@ def mc_binned(g1, g2, g3, phi, nbins, rrange1, rrange2, n_samples, seed, wf_boun
     swf2 = np.zeros(B)
     cnt = np.zeros(B, dtype=np.int64)
     prop = ctypes.byref(proposal) if proposal is not None else None
+    if record is not None and (record.dtype != torch.float32 or record.ndim != 2 or record.shape[1] != 8
+                               or not record.is_contiguous()):
+        raise ValueError("record must be a contiguous (n, 8) float32 tensor")
     rec_ptr = ptr(record) if record is not None else None
     n_rec = int(record.shape[0]) if record is not None else 0
     _lib.call("hermes_mc_binned", _lib.context(), _J_of(L), ptr(g1), ptr(g2), ptr(g3), ptr(phi_t), 1024, nr1, nr2,
               nmu, float(rrange1[0]), float(rrange1[1]), float(rrange2[0]), float(rrange2[1]), int(n_samples),
-              int(seed) & 0xFFFFFFFFFFFFFFFF, float(wf_bound), prop, rec_ptr, n_rec, swf.ctypes.data_as(_DP),
+              int(seed) & 0xFFFFFFFFFFFFFFFF, float(wf_cap), prop, rec_ptr, n_rec, swf.ctypes.data_as(_DP),
               swf2.ctypes.data_as(_DP), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
     shape = (nr1, nr2, nmu)
     return swf.reshape(shape), swf2.reshape(shape), cnt.reshape(shape)
+
+
+def mc_bucketize(grid, phi, sub, edges):
+    """Proposal buckets on the (L*sub)^3 sub-cell grid: (cell_bucket u8, bucket_cells i32, bucket_off i64)
+    device tensors and the per-bucket sub-cell counts (host int64)."""
+    L = grid.shape[0]
+    Ls = L * int(sub)
+    phi_t = phi if isinstance(phi, torch.Tensor) else as_device(np.asarray(phi, dtype=np.float64), torch.float64)
+    e = np.ascontiguousarray(edges, dtype=np.float64)
+    nb = e.size + 1
+    cb = torch.empty(Ls ** 3, dtype=torch.uint8, device=grid.device)
+    cells = torch.empty(Ls ** 3, dtype=torch.int32, device=grid.device)
+    off = torch.empty(nb + 1, dtype=torch.int64, device=grid.device)
+    counts = np.zeros(nb, dtype=np.int64)
+    _lib.call("hermes_mc_bucketize", _lib.context(), _J_of(L), ptr(grid), ptr(phi_t), 1024, int(sub),
+              e.ctypes.data_as(_DP), nb, ptr(cb), ptr(cells), ptr(off),
+              counts.ctypes.data_as(ctypes.POINTER(ctypes.c_int64)))
+    return cb, cells, off, counts
 
 
 # ---------------------------------------------------------------- proposal MLP

@@ -9,11 +9,11 @@ The learned proposal q(centre, bin) reweights the same estimator (w = p / q, so
 it stays unbiased): a 3-layer MLP (4 -> 64 -> 64 -> 1, bf16 weights, FP32
 accumulation, trained on the GPU by the HIP kernels in csrc/mlp.hip) maps
 (r1, r2, cos theta, local density) to log E[f^2 | ...], fitted on the plain
-estimator's own triangle weights.  The proposal is
-q(c, b) ~ sqrt(E[f^2 | b, rho(c)]) / |zeta_b(pilot)| -- the allocation that
-minimises the summed relative variance -- discretised on density buckets of the
-centre cell and mixed with the uniform proposal (weight alpha) so that
-w <= 1 / alpha.
+estimator's own triangle weights.  Within a bin the proposal places centres
+with density ~ sqrt(E[f^2 | b, rho(c)]) -- the variance-minimising choice given
+rho(c) -- discretised on density buckets of the centre cell; across bins it
+keeps the uniform share (default) or weights by 1 / |zeta_b(pilot)|.  It is
+mixed with the uniform proposal (weight alpha) so that w <= 1 / alpha.
 """
 
 import ctypes
@@ -54,24 +54,11 @@ class MCResult:
         self.variance = (B / self.n) ** 2 * np.maximum(sum_wf2 - sum_wf ** 2 / self.n, 0.0)
 
 
-def _phi_bound(phi, res=1024):
-    """max over fractions of sum_s |phi[f + res s]|: |n(x)| <= max|eps| * bound^3."""
-    return float(np.abs(np.asarray(phi)).reshape(-1, res).sum(axis=0).max())
-
-
-def _wf_bound(grids, phi, weight_max=1.0):
-    a = _phi_bound(phi) ** 3
-    b = 1.0
-    for g in grids:
-        b *= float(g.abs().max()) * a
-    return max(b * weight_max * 1.0000001, 1e-300)
-
-
 def run_mc(g1, g2, g3, phi, bins, n_samples, seed, proposal=None, record=None):
-    wmax = 1.0 / proposal.alpha if proposal is not None else 1.0
-    bound = _wf_bound((g1, g2, g3), phi, wmax)
-    s1, s2, cnt = kernels.mc_binned(g1, g2, g3, phi, bins.shape, bins.r1, bins.r2, n_samples, seed, bound,
-                                    proposal=proposal.struct if proposal is not None else None, record=record)
+    """proposal: a built LearnedProposal, a hermes_mc_proposal struct, or None (plain draws)."""
+    st = getattr(proposal, "struct", proposal)
+    s1, s2, cnt = kernels.mc_binned(g1, g2, g3, phi, bins.shape, bins.r1, bins.r2, n_samples, seed,
+                                    proposal=st, record=record)
     return MCResult(s1, s2, cnt, n_samples, bins.B)
 
 
@@ -97,10 +84,13 @@ class ProposalMLP:
     def forward(self, x):
         return kernels.mlp_forward(self.wb, x)
 
-    def train(self, x, y, epochs=3, batch=8192, lr=3e-3, clip=50.0, seed=0, log=None):
+    def train(self, x, y, epochs=3, batch=8192, lr=3e-3, lr_final=1.5e-4, clip=50.0, seed=0, log=None):
+        """Adam on mini-batches; the step size decays from lr to lr_final on a cosine."""
         n = int(x.shape[0])
         gen = torch.Generator(device=x.device).manual_seed(seed)
         losses = []
+        total = epochs * ((n + batch - 1) // batch)
+        it = 0
         for ep in range(epochs):
             perm = torch.randperm(n, device=x.device, generator=gen)
             tot = 0.0
@@ -109,7 +99,9 @@ class ProposalMLP:
                 xb, yb = x[idx].contiguous(), y[idx].contiguous()
                 kernels.mlp_grad(self.wb, xb, yb, clip, self.grad)
                 self.step += 1
-                kernels.mlp_adam(self.w, self.m, self.v, self.grad, self.wb, lr, self.step)
+                rate = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * it / max(total - 1, 1)))
+                it += 1
+                kernels.mlp_adam(self.w, self.m, self.v, self.grad, self.wb, float(rate), self.step)
             # epoch loss on a fixed subsample
             sub = min(n, 65536)
             kernels.mlp_grad(self.wb, x[:sub].contiguous(), y[:sub].contiguous(), clip, self.grad)
@@ -121,35 +113,50 @@ class ProposalMLP:
 
 
 class LearnedProposal:
-    """Fits the proposal MLP on plain-MC records and builds the sampling tables."""
+    """Fits the proposal MLP on plain-MC records and builds the sampling tables.
 
-    def __init__(self, bins, grid, nbucket=32, alpha=0.1, seed=0):
+    The proposal lives on the sub-cell grid of side L * sub: the local density of a
+    sub-cell is the interpolated field at its centre, and sub-cells are grouped into
+    ``nbucket`` quantile buckets of it (hermes_mc_bucketize).  With ``anchors`` = 3
+    a proposal draw places any of the three vertices (chosen uniformly) in a
+    favoured sub-cell and weights by the balance heuristic over the three.
+    """
+
+    def __init__(self, bins, grid, phi, nbucket=64, alpha=0.05, sub=4, anchors=3, seed=0, n_quantile=1 << 22):
         self.bins = bins
         self.grid = grid
-        self.L = grid.shape[0]
+        self.L = int(grid.shape[0])
         self.nbucket = int(nbucket)
         self.alpha = float(alpha)
+        self.sub = int(sub)
+        self.anchors = int(anchors)
+        self.phi = phi if isinstance(phi, torch.Tensor) else kernels.as_device(np.asarray(phi, np.float64),
+                                                                               torch.float64)
         self.mlp = ProposalMLP(seed=seed)
-        flat = grid.reshape(-1)
-        self.rho_scale = float(flat.std()) or 1.0
-        t = torch.asinh(flat / self.rho_scale)
-        qs = torch.linspace(0, 1, self.nbucket + 1, device=grid.device, dtype=torch.float64)[1:-1]
-        sample = t[torch.randint(0, t.numel(), (min(t.numel(), 1 << 22),), device=grid.device,
-                                 generator=torch.Generator(device=grid.device).manual_seed(seed))]
-        self.edges = torch.quantile(sample.float(), qs.float()).double()
-        self.cell_bucket = torch.bucketize(t, self.edges).to(torch.uint8)
-        order = torch.sort(self.cell_bucket.to(torch.int32), stable=True)
-        self.bucket_cells = order.indices.to(torch.int32)
-        counts = torch.bincount(self.cell_bucket.to(torch.int64), minlength=self.nbucket)
-        self.n_k = counts.cpu().numpy().astype(np.float64)
-        self.bucket_off = torch.cat([torch.zeros(1, dtype=torch.int64, device=grid.device),
-                                     torch.cumsum(counts, 0)]).contiguous()
-        # representative feature value of each bucket: mean of t over its cells
-        tsum = torch.zeros(self.nbucket, dtype=torch.float64, device=grid.device)
-        tsum.index_add_(0, self.cell_bucket.to(torch.int64), t)
-        self.t_k = (tsum.cpu().numpy() / np.maximum(self.n_k, 1.0))
+        Ls = self.L * self.sub
+        # bucket edges: quantiles of the field at a random sample of sub-cell centres
+        rng = np.random.default_rng(seed)
+        idx = rng.integers(0, Ls ** 3, size=min(int(n_quantile), Ls ** 3))
+        centres = (np.stack([idx // (Ls * Ls), (idx // Ls) % Ls, idx % Ls], axis=1) + 0.5) / self.sub
+        v = kernels.interpolate(grid, kernels.as_device(centres, torch.float64), self.phi).cpu().numpy()
+        self.edges = np.quantile(v, np.linspace(0.0, 1.0, self.nbucket + 1)[1:-1])
+        self.cell_bucket, self.bucket_cells, self.bucket_off, counts = kernels.mc_bucketize(
+            grid, self.phi, self.sub, self.edges)
+        self.n_k = counts.astype(np.float64)
+        # representative feature of each bucket: mean of asinh(v / scale) over its sampled sub-cells
+        self.rho_scale = float(v.std()) or 1.0
+        t = np.arcsinh(v / self.rho_scale)
+        kv = np.searchsorted(self.edges, v, side="right")
+        cnt = np.bincount(kv, minlength=self.nbucket).astype(np.float64)
+        t_k = np.bincount(kv, weights=t, minlength=self.nbucket) / np.maximum(cnt, 1.0)
+        te = np.arcsinh(np.asarray(self.edges) / self.rho_scale)
+        for k in np.nonzero(cnt == 0)[0]:  # unsampled bucket: its edge interval
+            t_k[k] = te[min(k, te.size - 1)] if te.size else 0.0
+        self.t_k = t_k
         self.t_scale = float(max(np.abs(self.t_k).max(), 1e-6))
         self.zeta_pilot = None
+        self._recording = self._struct(np.ones(self.nbucket) / self.nbucket,
+                                       np.full((self.nbucket, bins.B), 1.0 / bins.B), 1.0)
 
     def features(self, r1, r2, mu, t):
         b = self.bins
@@ -161,18 +168,37 @@ class LearnedProposal:
         ], dim=1)
         return x.to(torch.float32).contiguous()
 
-    def fit(self, records, pilot, epochs=3, log=None):
-        """records: (n, 6) float32 device tensor from a plain run; pilot: its MCResult."""
-        r1, r2, mu, rho, f = (records[:, i].double() for i in range(5))
-        t = torch.asinh(rho / self.rho_scale)
-        x = self.features(r1, r2, mu, t)
-        y = (f * f)
+    @property
+    def recording(self):
+        """Proposal struct for plain draws (alpha = 1) that record every vertex's bucket."""
+        return self._recording
+
+    def fit(self, records, pilot, epochs=3, log=None, lr=3e-3, batch=16384):
+        """records: (n, 8) float32 device tensor from a plain run with ``recording``;
+        pilot: its MCResult.  One training pair per record and anchor vertex."""
+        r1, r2, mu, f = (records[:, i].double() for i in range(4))
+        tk = torch.from_numpy(self.t_k).to(records.device)
+        xs = []
+        for a in range(self.anchors):
+            k = records[:, 5 + a].long()
+            if bool((k < 0).any()):
+                raise ValueError("records lack proposal buckets: record a plain run with .recording")
+            xs.append(self.features(r1, r2, mu, tk[k]))
+        x = torch.cat(xs).contiguous()
+        y = (f * f).repeat(self.anchors)
         y = (y / y.mean()).to(torch.float32).contiguous()
         self.zeta_pilot = np.abs(pilot.estimate).ravel()
-        return self.mlp.train(x, y, epochs=epochs, log=log)
+        return self.mlp.train(x, y, epochs=epochs, batch=batch, lr=lr, log=log)
 
-    def build(self):
-        """Evaluate the MLP on (bucket, bin) centres and upload the proposal tables."""
+    def joint(self, allocation="uniform"):
+        """Proposal mass Q[k, b] of density bucket k and triangle bin b (sums to 1).
+
+        Within a bin the centre density follows sqrt(E[f^2 | bin, bucket]) from the
+        MLP.  ``allocation`` spreads the mass over bins: "uniform" keeps every
+        bin's expected share at 1/B, as in the plain estimator, so each bin gains
+        on its own; "relative" weights bin b by 1/|zeta_b(pilot)|, minimising the
+        summed relative variance over bins.
+        """
         b = self.bins
         B = b.B
         r1c, r2c, muc = b.centres()
@@ -184,22 +210,41 @@ class LearnedProposal:
         x = self.features(*(torch.from_numpy(a).to(dev) for a in (R1, R2, MU, T)))
         s = self.mlp.forward(x).double().cpu().numpy().reshape(self.nbucket, B)
         s = np.clip(s - s.max(), -60.0, 0.0)
-        score = np.exp(0.5 * s)
-        if self.zeta_pilot is not None:
+        mass = self.n_k[:, None] * np.exp(0.5 * s)
+        if allocation == "uniform":
+            Q = mass / np.maximum(mass.sum(axis=0, keepdims=True), 1e-300) / B
+        elif allocation == "relative":
+            if self.zeta_pilot is None:
+                raise ValueError("relative allocation needs a pilot run (fit)")
             z = np.maximum(self.zeta_pilot, 1e-3 * self.zeta_pilot.max() + 1e-300)
-            score = score / z[None, :]
-        g_k = score.sum(axis=1) / B
-        bin_pdf = score / score.sum(axis=1, keepdims=True)
-        P = self.n_k * g_k
-        P = P / P.sum()
-        V = float(self.L) ** 3
-        dens = np.where(self.n_k > 0, V * B * P / np.maximum(self.n_k, 1.0), 0.0)
-        cdf = np.cumsum(P).astype(np.float32)
-        last = int(np.nonzero(P > 0)[0].max())
+            Q = mass / z[None, :]
+            Q = Q / Q.sum()
+        else:
+            raise ValueError(f"unknown allocation {allocation!r}")
+        return Q
+
+    def build(self, allocation="uniform"):
+        """Evaluate the MLP on (bucket, bin) centres and upload the proposal tables."""
+        B = self.bins.B
+        Q = self.joint(allocation)
+        P = Q.sum(axis=1)
+        bin_pdf = Q / np.maximum(P[:, None], 1e-300)
+        bin_pdf[P <= 0] = 1.0 / B
+        self.struct = self._struct(P, bin_pdf, self.alpha)
+        return self
+
+    def _struct(self, P, bin_pdf, alpha):
+        V = float(self.L * self.sub) ** 3
+        B = self.bins.B
+        Pm = np.where(self.n_k > 0, P, 0.0)
+        Pm = Pm / Pm.sum()
+        dens = np.where(self.n_k > 0, V * B * Pm / np.maximum(self.n_k, 1.0), 0.0)
+        cdf = np.cumsum(Pm).astype(np.float32)
+        last = int(np.nonzero(Pm > 0)[0].max())
         cdf[last:] = 2.0
         bcdf = np.cumsum(bin_pdf, axis=1).astype(np.float32)
         bcdf[:, -1] = 2.0
-        self._tables = {
+        tables = {
             "bucket_cdf": kernels.as_device(cdf, torch.float32),
             "bucket_density": kernels.as_device(dens, torch.float64),
             "bin_cdf": kernels.as_device(bcdf, torch.float32),
@@ -209,14 +254,16 @@ class LearnedProposal:
         st.cell_bucket = self.cell_bucket.data_ptr()
         st.bucket_cells = self.bucket_cells.data_ptr()
         st.bucket_off = self.bucket_off.data_ptr()
-        st.bucket_cdf = self._tables["bucket_cdf"].data_ptr()
-        st.bucket_density = self._tables["bucket_density"].data_ptr()
-        st.bin_cdf = self._tables["bin_cdf"].data_ptr()
-        st.bin_pdf = self._tables["bin_pdf"].data_ptr()
+        st.bucket_cdf = tables["bucket_cdf"].data_ptr()
+        st.bucket_density = tables["bucket_density"].data_ptr()
+        st.bin_cdf = tables["bin_cdf"].data_ptr()
+        st.bin_pdf = tables["bin_pdf"].data_ptr()
         st.nbucket = self.nbucket
-        st.alpha = self.alpha
-        self.struct = st
-        return self
+        st.sub = self.sub
+        st.alpha = float(alpha)
+        st.anchors = self.anchors
+        st._tables = tables  # keeps the device arrays alive with the struct
+        return st
 
 
 def compare(plain, imp):

@@ -56,6 +56,95 @@ def test_binned_sampler_constant_field_and_counts(phi):
     assert np.array_equal(again.sum_wf, res.sum_wf) and np.array_equal(again.counts, res.counts)
 
 
+def _device_tables(tab):
+    """hermes_mc_proposal over device copies of numpy proposal tables (kept alive on the struct)."""
+    import torch
+    from pyhermes_amd import _lib, kernels
+    keep = {k: kernels.as_device(np.ascontiguousarray(tab[k]), dt) for k, dt in
+            (("cell_bucket", torch.uint8), ("bucket_cells", torch.int32), ("bucket_off", torch.int64),
+             ("bucket_cdf", torch.float32), ("bucket_density", torch.float64), ("bin_cdf", torch.float32),
+             ("bin_pdf", torch.float32))}
+    st = _lib.MCProposal()
+    for k, t in keep.items():
+        setattr(st, k, t.data_ptr())
+    st.nbucket = int(tab["nbucket"])
+    st.alpha = float(tab["alpha"])
+    st.sub = int(tab.get("sub", 1))
+    st.anchors = int(tab.get("anchors", 1))
+    st._keep = keep
+    return st
+
+
+@pytest.mark.parametrize("importance,sub,anchors", [(False, 1, 1), (True, 1, 1), (True, 2, 3), (True, 4, 3),
+                                                    (True, 2, 1)])
+def test_binned_sampler_matches_oracle(phi, importance, sub, anchors):
+    """Same Philox streams: bit-exact per-bin counts, sums to the fixed-point resolution."""
+    import torch
+    from pyhermes_amd import kernels
+    from tests.test_oracle import _synthetic_tables
+    L, n, seed = 32, 300000, 4242
+    nbins, r1r, r2r = (3, 4, 5), (1.5, 9.0), (2.0, 11.0)
+    g = [_grid(L, s) - 1.0 for s in (6, 7, 8)]
+    tab = _synthetic_tables(L, nbins, 8, 0.2, 9, sub, anchors) if importance else None
+    print(f"mc_binned vs oracle importance={importance} sub={sub} anchors={anchors}", flush=True)
+    ref = omc.binned_sampler(*g, phi, nbins, r1r, r2r, n, seed, tables=tab)
+    gd = [kernels.as_device(a, torch.float64) for a in g]
+    prop = _device_tables(tab) if importance else None
+    got = kernels.mc_binned(*gd, phi, nbins, r1r, r2r, n, seed, proposal=prop)
+    assert np.array_equal(got[2], ref[2])
+    for a, b in zip(got[:2], ref[:2]):
+        np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-9 * np.abs(b).max())
+
+
+def _edges_between_values(v, nbucket):
+    """Quantile edges moved to midpoints between neighbouring sample values."""
+    vs = np.sort(v)
+    edges = []
+    for q in np.linspace(0, 1, nbucket + 1)[1:-1]:
+        j = min(max(int(q * vs.size), 1), vs.size - 1)
+        while j < vs.size - 1 and vs[j] - vs[j - 1] < 1e-9:
+            j += 1
+        edges.append(0.5 * (vs[j - 1] + vs[j]))
+    return np.array(edges)
+
+
+@pytest.mark.parametrize("sub,nbucket", [(1, 7), (2, 7), (4, 7), (2, 1), (2, 256)])
+def test_bucketize_matches_oracle(phi, sub, nbucket):
+    """Bucket of every sub-cell and the stable per-bucket listing, bit for bit."""
+    import torch
+    from pyhermes_amd import kernels
+    L = 16
+    g = _grid(L, 21) - 1.0
+    _, _, _, v = omc.bucketize(g, phi, sub, [])
+    edges = _edges_between_values(v, nbucket)
+    print(f"bucketize sub={sub} nbucket={nbucket}", flush=True)
+    cb, cells, off, _ = omc.bucketize(g, phi, sub, edges)
+    gcb, gcells, goff, counts = kernels.mc_bucketize(kernels.as_device(g, torch.float64), phi, sub, edges)
+    assert np.array_equal(gcb.cpu().numpy(), cb)
+    assert np.array_equal(gcells.cpu().numpy(), cells)
+    assert np.array_equal(goff.cpu().numpy(), off)
+    assert np.array_equal(counts, np.diff(off))
+
+
+def test_binned_sampler_cap_paths_agree(phi):
+    """Pilot cap, a small cap (overflow list + second pass) and a cap so small that the
+    list overflows (rerun at the exact maximum) give the same sums and counts."""
+    import torch
+    from pyhermes_amd import kernels
+    L, n, seed = 32, 1_200_000, 99
+    nbins, rr = (2, 2, 3), (1.0, 7.0)
+    g = kernels.as_device(_grid(L, 12) - 1.0, torch.float64)
+    runs = []
+    for cap in (0.0, 0.5, 1e-6):
+        print(f"mc_binned cap={cap}", flush=True)
+        runs.append(kernels.mc_binned(g, g, g, phi, nbins, rr, rr, n, seed, wf_cap=cap))
+    for r in runs[1:]:
+        assert np.array_equal(r[2], runs[0][2])
+        for a, b in zip(r[:2], runs[0][:2]):
+            np.testing.assert_allclose(a, b, rtol=1e-9, atol=1e-12 * np.abs(b).max())
+    assert runs[0][2].sum() == n
+
+
 def test_mlp_gradient_matches_oracle():
     import torch
     from pyhermes_amd import kernels, mcsample
@@ -89,9 +178,9 @@ def test_learned_proposal_is_unbiased_and_reduces_variance(phi):
     eps, _, _ = oproj.project_c(pos, np.full(pos.shape[0], 1.0), phi, J, box)
     g = kernels.as_device(eps, torch.float64)
     bins = mcsample.TriangleBins(4, 4, 6, 1.5, 9.0)
-    rec = torch.zeros((400000, 6), dtype=torch.float32, device=g.device)
-    pilot = mcsample.run_mc(g, g, g, phi, bins, 400000, 11, record=rec)
-    prop = mcsample.LearnedProposal(bins, g, nbucket=16, alpha=0.2, seed=1)
+    rec = torch.zeros((400000, 8), dtype=torch.float32, device=g.device)
+    prop = mcsample.LearnedProposal(bins, g, phi, nbucket=16, alpha=0.2, sub=2, anchors=3, seed=1)
+    pilot = mcsample.run_mc(g, g, g, phi, bins, 400000, 11, proposal=prop.recording, record=rec)
     prop.fit(rec, pilot, epochs=4)
     prop.build()
     plain = mcsample.run_mc(g, g, g, phi, bins, 2_000_000, 12)

@@ -302,3 +302,145 @@ def test_mlp_gradient_finite_differences():
     v = np.array([1.0, 1.00390625, 1.005859375, -3.1415927], dtype=np.float32)
     r = omlp.bf16_round(v)
     assert r[0] == 1.0 and r[1] == 1.0 and r[2] == np.float32(1.0078125) and r[3] == np.float32(-3.140625)
+
+
+# ---------------------------------------------------------------- binned MC sampler
+_M64 = (1 << 64) - 1
+
+
+def test_philox_known_answer_vectors():
+    """Random123's published philox4x32-10 known-answer vectors."""
+    kat = [
+        ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+        ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+        ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+         [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+    ]
+    for ctr, key, want in kat:
+        assert [int(v) for v in mc.philox4x32_10(ctr, key)] == want
+
+
+def _u2(i, seed, stream):
+    key = (seed ^ ((0x9E3779B97F4A7C15 * (stream + 1)) & _M64)) & _M64
+    r = [int(v) for v in mc.philox4x32_10([i & 0xFFFFFFFF, i >> 32, 0x9E3779B9, 0x85EBCA6B],
+                                          [key & 0xFFFFFFFF, key >> 32])]
+    return (((r[0] >> 5) * 67108864 + (r[1] >> 6)) / 2.0 ** 53, ((r[2] >> 5) * 67108864 + (r[3] >> 6)) / 2.0 ** 53)
+
+
+def test_binned_sampler_matches_per_sample_bruteforce(phi):
+    """Each sample rebuilt from its Philox streams with brute-force interpolation."""
+    L, n, seed = 16, 40, 987654321
+    rng = np.random.default_rng(5)
+    g = [rng.normal(size=(L, L, L)) for _ in range(3)]
+    nr1, nr2, nmu, r1r, r2r = 3, 2, 4, (1.5, 5.0), (2.0, 6.5)
+    s1, s2, cnt = mc.binned_sampler(*g, phi, (nr1, nr2, nmu), r1r, r2r, n, seed)
+    B = nr1 * nr2 * nmu
+    e1, e2, e3 = np.zeros(B), np.zeros(B), np.zeros(B, dtype=np.int64)
+    for i in range(n):
+        ux, uy = _u2(i, seed, 0)
+        uz, ubin = _u2(i, seed, 1)
+        w1 = _u2(i, seed, 3)[1]
+        w2, w3 = _u2(i, seed, 4)
+        oph, oct_ = _u2(i, seed, 5)
+        oal = _u2(i, seed, 6)[0]
+        b = min(int(ubin * B), B - 1)
+        imu, ir2, ir1 = b % nmu, (b // nmu) % nr2, b // (nmu * nr2)
+        r1 = r1r[0] + (ir1 + w1) * (r1r[1] - r1r[0]) / nr1
+        r2 = r2r[0] + (ir2 + w2) * (r2r[1] - r2r[0]) / nr2
+        mu = -1.0 + (imu + w3) * 2.0 / nmu
+        o = mc.triangle_offsets(r1, r2, mu, 2 * np.pi * oph, 2 * oct_ - 1, 2 * np.pi * oal)
+        assert abs(np.dot(o[:3], o[3:]) - r1 * r2 * mu) < 1e-9 * r1 * r2
+        c = np.array([[ux * L, uy * L, uz * L]])
+        f = (mc.interpolate_bruteforce(g[0], c, phi)[0] * mc.interpolate_bruteforce(g[1], c + o[:3], phi)[0]
+             * mc.interpolate_bruteforce(g[2], c + o[3:], phi)[0])
+        e1[b] += f
+        e2[b] += f * f
+        e3[b] += 1
+    assert np.array_equal(cnt.ravel(), e3)
+    np.testing.assert_allclose(s1.ravel(), e1, rtol=1e-12, atol=1e-12 * np.abs(e1).max())
+    np.testing.assert_allclose(s2.ravel(), e2, rtol=1e-12)
+
+
+def test_binned_sampler_uniform_bins_and_constant_field(phi):
+    """Plain draws: bin counts uniform (chi-square); a constant field gives f = c^3 exactly."""
+    L, n, c = 16, 200000, 1.3
+    g = np.full((L, L, L), c)
+    s1, s2, cnt = mc.binned_sampler(g, g, g, phi, (3, 4, 5), (1.0, 6.0), (1.0, 6.0), n, 17)
+    B = cnt.size
+    assert cnt.sum() == n
+    chi2 = (((cnt - n / B) ** 2) / (n / B)).sum()
+    assert chi2 < B + 6 * np.sqrt(2 * B), chi2
+    np.testing.assert_allclose(s1, cnt * c ** 3, rtol=1e-12)
+
+
+def _synthetic_tables(L, nbins, nbucket, alpha, seed, sub=1, anchors=1):
+    """Importance tables from an arbitrary bucket assignment and proposal mass."""
+    rng = np.random.default_rng(seed)
+    B = int(np.prod(nbins))
+    Ls = L * sub
+    cell_bucket = rng.integers(0, nbucket, size=Ls ** 3).astype(np.uint8)
+    order = np.argsort(cell_bucket, kind="stable").astype(np.int32)
+    n_k = np.bincount(cell_bucket, minlength=nbucket).astype(np.float64)
+    off = np.concatenate([[0], np.cumsum(n_k)]).astype(np.int64)
+    Q = rng.uniform(0.05, 1.0, size=(nbucket, B)) * rng.uniform(0.1, 3.0, size=(nbucket, 1))
+    Q /= Q.sum()
+    P = Q.sum(axis=1)
+    pdf = Q / P[:, None]
+    cdf = np.cumsum(P).astype(np.float32)
+    cdf[-1] = 2.0
+    bcdf = np.cumsum(pdf, axis=1).astype(np.float32)
+    bcdf[:, -1] = 2.0
+    return {"cell_bucket": cell_bucket, "bucket_cells": order, "bucket_off": off, "bucket_cdf": cdf,
+            "bucket_density": float(Ls) ** 3 * B * P / n_k, "bin_cdf": bcdf, "bin_pdf": pdf.astype(np.float32),
+            "nbucket": nbucket, "alpha": alpha, "sub": sub, "anchors": anchors}
+
+
+@pytest.mark.parametrize("sub,anchors", [(1, 1), (2, 3), (4, 3), (2, 1)])
+def test_importance_weights_are_unbiased_on_constant_field(phi, sub, anchors):
+    """E_q[w 1{bin b}] = 1/B for any proposal: per-bin (B/n) sum w f -> c^3 within the errors."""
+    L, n, c = 16, 400000, 0.9
+    nbins = (2, 3, 4)
+    tab = _synthetic_tables(L, nbins, 6, 0.25, 3, sub, anchors)
+    g = np.full((L, L, L), c)
+    s1, s2, cnt = mc.binned_sampler(g, g, g, phi, nbins, (1.0, 6.0), (1.0, 6.0), n, 23, tables=tab)
+    B = cnt.size
+    est = B / n * s1.ravel()
+    var = (B / n) ** 2 * (s2.ravel() - s1.ravel() ** 2 / n)
+    z = (est - c ** 3) / np.sqrt(var)
+    assert np.abs(z).max() < 5.0, z
+    assert 0.5 < np.sqrt(np.mean(z ** 2)) < 1.6
+    # the proposal really moved the counts away from uniform
+    assert cnt.max() > 1.5 * cnt.min()
+
+
+def test_anchored_proposal_is_unbiased_on_clustered_field(phi):
+    """Three-anchor importance on sub-cells against plain draws on a lognormal field."""
+    L, n = 16, 600000
+    nbins, rr = (2, 2, 3), (1.5, 6.0)
+    g = np.random.default_rng(31).lognormal(0.0, 0.8, size=(L, L, L)) - 1.0
+    tab = _synthetic_tables(L, nbins, 8, 0.2, 4, sub=2, anchors=3)
+    p1, p2, _ = mc.binned_sampler(g, g, g, phi, nbins, rr, rr, n, 41)
+    i1, i2, _ = mc.binned_sampler(g, g, g, phi, nbins, rr, rr, n, 42, tables=tab)
+    B = p1.size
+    est = [B / n * a.ravel() for a in (p1, i1)]
+    var = [(B / n) ** 2 * (b.ravel() - a.ravel() ** 2 / n) for a, b in ((p1, p2), (i1, i2))]
+    z = (est[1] - est[0]) / np.sqrt(var[0] + var[1])
+    assert np.abs(z).max() < 4.5, z
+    assert np.sqrt(np.mean(z ** 2)) < 1.7
+
+
+def test_bucketize_oracle_matches_definition(phi):
+    """Buckets by edge count; the listing is a permutation sorted by (bucket, sub-cell)."""
+    L, sub = 8, 2
+    g = np.random.default_rng(8).normal(size=(L, L, L))
+    edges = [-0.5, 0.0, 0.7]
+    cb, cells, off, v = mc.bucketize(g, phi, sub, edges)
+    Ls = L * sub
+    idx = np.arange(Ls ** 3)
+    c = (np.stack([idx // (Ls * Ls), (idx // Ls) % Ls, idx % Ls], axis=1) + 0.5) / sub
+    np.testing.assert_allclose(v, mc.interpolate_bruteforce(g, c, phi), rtol=1e-12, atol=1e-13)
+    assert np.array_equal(cb, (v[:, None] >= np.array(edges)[None, :]).sum(axis=1))
+    assert np.array_equal(np.sort(cells), idx)
+    key = cb[cells].astype(np.int64) * Ls ** 3 + cells
+    assert np.all(np.diff(key) > 0)
+    assert off[-1] == Ls ** 3 and np.array_equal(np.diff(off), np.bincount(cb, minlength=4))
