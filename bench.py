@@ -188,10 +188,214 @@ def run_mc_workload(args):
     print(json.dumps(line))
 
 
+def _events():
+    import torch
+    return torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+
+def _time_events(fn, reps):
+    """Mean HIP-event milliseconds of fn() on torch's current stream (the library's stream)."""
+    import torch
+    e0, e1 = _events()
+    torch.cuda.synchronize()
+    e0.record()
+    for _ in range(reps):
+        out = fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps, out
+
+
+def run_2pcf_workload(args):
+    """Isotropic 2PCF, FP64 throughout: 100,000-particle lognormal mock (P(k) ~ k^-1.5 on 128^3, sigma_LN = 1),
+    box 1000 Mpc/h, unit weights, projection at J=7, 20 linear shell bins over 5-150 Mpc/h.
+
+    Times the drop-in API (SFCProjection.run + Corr_2PCF.run, host catalogue in, xi out) and the device-resident
+    kernels, and runs the full CPU oracle on the same catalogue (the configuration is CPU-sized) for parity.
+    """
+    import numpy as np
+    import torch
+
+    from oracle import native, projection as oproj, scaling, windows as ow
+    from pyhermes_amd import kernels
+    from pyhermes_amd.base import SFCProjection
+    from pyhermes_amd.mocks import lognormal_mock
+    from pyhermes_amd.theory import Corr_2PCF
+    from pyhermes_amd.utils.wavelet_grid import sample_scaling_function
+
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    torch.cuda.set_device(0)
+    box, J, n = 1000.0, 7, 100_000
+    L = 1 << J
+    pos, _ = lognormal_mock(n, 128, box, 1000)
+    w = np.ones(pos.shape[0])
+    s_edges = np.linspace(5.0, 150.0, 20)
+
+    def api():
+        f = SFCProjection({"SFCProjection": {"J": J, "box_size": box, "particle_pos": pos, "catalog_weight": w,
+                                             "weight_normalization": "catalog", "fout_path": ""}}).run(
+            save_result=False)
+        return Corr_2PCF({"Corr_2PCF": {"sfc_field": f, "random": "uniform",
+                                        "sampling": {"s": {"min": 5.0, "max": 150.0, "n": 20}},
+                                        "products": ["xi"], "fout_path": ""}}).run(save_result=False)
+
+    api()
+    torch.cuda.synchronize()
+    reps = max(args.steps, 3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        res = api()
+    torch.cuda.synchronize()
+    t_api = (time.perf_counter() - t0) / reps
+    log(f"API projection + 2PCF: {t_api * 1e3:.2f} ms")
+    # device-resident kernels: projection, r2c FFT, fused 20-bin pair reduction
+    phi = sample_scaling_function()
+    pos_d = kernels.as_device(pos, torch.float64)
+    pw = kernels.as_device(w / w.sum(), torch.float64)
+    ms_proj, proj = _time_events(lambda: kernels.project(pos_d, pw, J, box, phi, want32=False, want64=True), reps)
+    delta = (proj["eps64"] - 1.0 / L ** 3).contiguous()
+    ms_fft, F = _time_events(lambda: kernels.rfftn(delta), reps)
+    Pd = kernels.as_device(scaling.fourier_power_spectrum(phi, 0, 1, L, 1024), torch.float64)
+    descs = [kernels.window_desc("shell", sv * L / box) for sv in s_edges]
+    ms_pair, _ = _time_events(lambda: kernels.pair_reduce(F, F, Pd, descs), reps)
+    log(f"kernels: projection {ms_proj:.3f} ms, rfft {ms_fft:.3f} ms, pair reduce {ms_pair:.3f} ms")
+    # full CPU oracle
+    threads = native.num_threads()
+    t0 = time.perf_counter()
+    pwn, _ = oproj.projection_weights(w, np.ones_like(w), "catalog")
+    eps, _, _ = oproj.project_c(pos, pwn, phi, J, box)
+    rho = eps.sum() / L ** 3
+    d = eps - rho
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    xi_cpu = np.array([ow.pair_product(d, d, ow.octant_rfft_kernel(L, P, "shell", sv * L / box)) for sv in s_edges])
+    xi_cpu = xi_cpu / rho ** 2
+    t_cpu = time.perf_counter() - t0
+    err = float(np.max(np.abs(res.xi - xi_cpu)) / np.max(np.abs(xi_cpu)))
+    log(f"CPU oracle: {t_cpu:.2f} s ({threads} threads); max |xi_gpu - xi_cpu| / max |xi| = {err:.2e}")
+    line = {
+        "metric": "isotropic 2PCF (20 shell bins) per catalogue, FP64, projection + correlation",
+        "value": 1.0 / t_api, "unit": "catalogues/s", "n_gpus": 1, "higher_is_better": True,
+        "ms_per_catalogue": t_api * 1e3, "dtype": "f64", "data": "synthetic lognormal mock (seeded)",
+        "config": {"workload": "2pcf_J7_100k_20bins", "particles": n, "J": J, "bins": 20, "s_range_mpc": [5, 150]},
+        "kernels_ms": {"projection": ms_proj, "rfft": ms_fft, "pair_reduce_20_bins": ms_pair},
+        "parity": {"xi_max_abs_err_over_max_xi": err, "bar": 1e-5},
+        "cpu_baseline": {"value": 1.0 / t_cpu, "unit": "catalogues/s", "cores": threads, "kind": "port",
+                         "sample": "the whole configuration: C/OpenMP projection, octant shell kernels and "
+                                   "scipy.fft pair products for all 20 bins"},
+    }
+    print(json.dumps(line))
+
+
+def run_xi_ell_workload(args):
+    """Anisotropic 2PCF multipoles: 1,000,000-particle lognormal mock (256^3 recipe) with plane-parallel
+    redshift-space shifts (sigma_v = 300 km/s along z) and lognormal weights (mean 1, sigma 0.3), box
+    1000 Mpc/h, projection at J=9; 30 shell bins over 5-150 Mpc/h x l = 0, 2, 4 in one fused pair reduction.
+    FP32 coefficients/spectra/windows with FP64 normalisation and accumulators, against the FP64 path.
+    """
+    import numpy as np
+    import torch
+
+    from oracle import native, projection as oproj, scaling, windows as ow
+    from pyhermes_amd import kernels
+    from pyhermes_amd.base import SFCProjection
+    from pyhermes_amd.mocks import lognormal_mock
+    from pyhermes_amd.theory import Corr_2PCF
+    from pyhermes_amd.utils.wavelet_grid import sample_scaling_function
+
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    torch.cuda.set_device(0)
+    box, J, n = 1000.0, 9, 1_000_000
+    L = 1 << J
+    pos, w = lognormal_mock(n, 256, box, 2000, rsd_sigma_kms=300.0, weight_sigma=0.3)
+    ells = [0, 2, 4]
+    nbin = 30
+
+    def api(precision):
+        f = SFCProjection({"SFCProjection": {"J": J, "box_size": box, "particle_pos": pos, "catalog_weight": w,
+                                             "weight_normalization": "catalog", "precision": precision,
+                                             "fout_path": ""}}).run(save_result=False)
+        return Corr_2PCF({"Corr_2PCF": {"sfc_field": f, "random": "uniform",
+                                        "sampling": {"s": {"min": 5.0, "max": 150.0, "n": nbin}},
+                                        "products": ["xi"], "multipoles": ells, "fout_path": ""}}).run(
+            save_result=False)
+
+    api("fp32")
+    torch.cuda.synchronize()
+    reps = max(args.steps, 3)
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        r32 = api("fp32")
+    torch.cuda.synchronize()
+    t32 = (time.perf_counter() - t0) / reps
+    t0 = time.perf_counter()
+    r64 = api("fp64")
+    torch.cuda.synchronize()
+    t64 = time.perf_counter() - t0
+    errs = {}
+    for j, l in enumerate(ells):
+        a, b = r32.xi_l[j], r64.xi_l[j]
+        errs[f"xi_{l}"] = {"max_abs_over_max": float(np.max(np.abs(a - b)) / np.max(np.abs(b))),
+                           "rel_l2": float(np.linalg.norm(a - b) / np.linalg.norm(b))}
+    log(f"API FP32 {t32 * 1e3:.1f} ms, FP64 {t64 * 1e3:.1f} ms; errors {errs}")
+    # device-resident kernels of the FP32 path
+    phi = sample_scaling_function()
+    pos_d = kernels.as_device(pos, torch.float64)
+    pw = kernels.as_device(w / w.sum(), torch.float64)
+    ms_proj, proj = _time_events(lambda: kernels.project(pos_d, pw, J, box, phi, want32=True, want64=False), reps)
+    d32 = (proj["eps32"] - np.float32(1.0 / L ** 3)).contiguous()
+    ms_fft, F = _time_events(lambda: kernels.rfftn(d32), reps)
+    Pd = kernels.as_device(scaling.fourier_power_spectrum(phi, 0, 1, L, 1024), torch.float64)
+    s = np.linspace(5.0, 150.0, nbin)
+    descs = [kernels.window_desc("shell", sv * L / box) for sv in s]
+    descs += [kernels.window_desc("shell_multipole", sv * L / box, l=l, m=0) for l in ells for sv in s]
+    ms_pair, _ = _time_events(lambda: kernels.pair_reduce(F, F, Pd, descs), reps)
+    half_bytes = L * L * (L // 2 + 1) * 8
+    log(f"kernels: projection {ms_proj:.3f} ms, rfft {ms_fft:.3f} ms, pair reduce ({len(descs)} windows) "
+        f"{ms_pair:.3f} ms")
+    cpu = None
+    if not args.skip_cpu:
+        # bounded sample: the oracle's pair products for one radius (shell + l = 0, 2, 4) at J=8, scaled by
+        # the grid volume (x8) and the window count (x30 radii)
+        Js = 8
+        Ls = 1 << Js
+        threads = native.num_threads()
+        rng = np.random.default_rng(0)
+        idx = rng.choice(pos.shape[0], 250_000, replace=False)
+        pwn, _ = oproj.projection_weights(w[idx], np.ones(idx.size), "catalog")
+        eps, _, _ = oproj.project_c(pos[idx], pwn, phi, Js, box)
+        d = eps - eps.sum() / Ls ** 3
+        P = scaling.fourier_power_spectrum(phi, 0, 1, Ls, 1024)
+        R = 60.0 * Ls / box
+        t0 = time.perf_counter()
+        ow.pair_product(d, d, ow.octant_rfft_kernel(Ls, P, "shell", R))
+        for l in ells:
+            K = ow.radial_multipole_window(Ls, P, R, l, 0)
+            np.sum(np.real(ow.convolve_complex(d, K)) * d)
+        t_sample = time.perf_counter() - t0
+        t_full = t_sample * 8.0 * nbin
+        cpu = {"value": 1.0 / t_full, "unit": "catalogues/s", "cores": threads, "kind": "port",
+               "sample": f"C/OpenMP + scipy.fft oracle pair products for one radius (shell + l=0,2,4) at J={Js} "
+                         f"({t_sample:.2f} s), scaled x8 (J=9 volume) and x{nbin} radii; projection not included"}
+    line = {
+        "metric": "2PCF multipoles xi_0,2,4 (30 bins) per catalogue, FP32 path vs FP64",
+        "value": 1.0 / t32, "unit": "catalogues/s", "n_gpus": 1, "higher_is_better": True,
+        "ms_per_catalogue": t32 * 1e3, "fp64_ms_per_catalogue": t64 * 1e3,
+        "dtype": "f32 (f64 normalisation and accumulators)", "data": "synthetic lognormal mock (seeded), RSD + weights",
+        "config": {"workload": "xi_ell_J9_1M_rsd_30bins_l024", "particles": n, "J": J, "bins": nbin,
+                   "multipoles": ells, "windows_per_launch": len(descs)},
+        "kernels_ms": {"projection_fp32": ms_proj, "rfft_fp32": ms_fft, "pair_reduce": ms_pair},
+        "pair_reduce_hbm_frac": half_bytes / (ms_pair * 1e-3) / 1e9 / HBM_PEAK_GBS,
+        "fp32_vs_fp64": errs,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["sh3pcf", "mc"], default="sh3pcf",
-                    help="sh3pcf: the multipole sets/s headline; mc: learned vs uniform Monte-Carlo 3PCF")
+    ap.add_argument("--workload", choices=["sh3pcf", "mc", "2pcf", "xi_ell"], default="sh3pcf",
+                    help="sh3pcf: the multipole sets/s headline; mc: learned vs uniform Monte-Carlo 3PCF; "
+                         "2pcf: FP64 isotropic 2PCF at J=7; xi_ell: FP32 2PCF multipoles with RSD at J=9")
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -207,8 +411,8 @@ def main():
     ap.add_argument("--mc-triangles", type=float, default=1e8)
     ap.add_argument("--mc-train", type=float, default=1e7)
     ap.add_argument("--mc-epochs", type=int, default=3)
-    ap.add_argument("--mc-buckets", type=int, default=64)
-    ap.add_argument("--mc-alpha", type=float, default=0.05)
+    ap.add_argument("--mc-buckets", type=int, default=128)
+    ap.add_argument("--mc-alpha", type=float, default=0.02)
     ap.add_argument("--mc-sub", type=int, default=4)
     ap.add_argument("--mc-anchors", type=int, choices=[1, 3], default=3)
     ap.add_argument("--mc-allocation", choices=["uniform", "relative"], default="uniform")
@@ -217,8 +421,8 @@ def main():
     if args.gpus != 1:
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1", "n_gpus": args.gpus}))
         sys.exit(2)
-    if args.workload == "mc":
-        run_mc_workload(args)
+    if args.workload != "sh3pcf":
+        {"mc": run_mc_workload, "2pcf": run_2pcf_workload, "xi_ell": run_xi_ell_workload}[args.workload](args)
         return
 
     import numpy as np

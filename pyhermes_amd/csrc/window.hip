@@ -177,6 +177,146 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __r
     }
 }
 
+// j_0 .. j_LMAX at q = 2 pi t (t in turns), FP32.  One hardware sin/cos and the
+// upward recurrence when q >= LMAX + 1 (stable for every order there); otherwise
+// each order on its own (jl_f32: recurrence where q >= l + 1, else the series).
+template <int LMAX>
+__device__ __forceinline__ void radial_jl(double t, float* j) {
+    if (t == 0.0) {
+        j[0] = 1.0f;
+#pragma unroll
+        for (int l = 1; l <= LMAX; ++l) j[l] = 0.0f;
+        return;
+    }
+    const float q = (float)(2.0 * HERMES_PI * t);
+    if (LMAX == 0 || q >= (float)LMAX + 1.0f) {
+        const float fr = (float)(t - floor(t));
+        const float sn = __builtin_amdgcn_sinf(fr), cs = __builtin_amdgcn_cosf(fr);
+        const float iq = __builtin_amdgcn_rcpf(q);
+        j[0] = sn * iq;
+        if (LMAX >= 1) j[1] = (j[0] - cs) * iq;
+#pragma unroll
+        for (int n = 1; n < LMAX; ++n) j[n + 1] = (float)(2 * n + 1) * iq * j[n] - j[n - 1];
+    } else {
+#pragma unroll
+        for (int l = 0; l <= LMAX; ++l) j[l] = jl_f32(l, t);
+    }
+}
+
+// FP64 counterpart, j_0 .. j_LMAX at x = 2 pi t: one sincos and the upward
+// recurrence of jl_f64 when x >= LMAX + 1 (jl_f64 takes the same branch for
+// every order there); otherwise jl_f64 order by order.
+template <int LMAX>
+__device__ __forceinline__ void radial_jl(double t, double* j) {
+    const double x = 2.0 * HERMES_PI * t;
+    if (LMAX >= 1 && x >= (double)LMAX + 1.0) {
+        double sx, cx;
+        sincos(x, &sx, &cx);
+        j[0] = sx / x;
+        j[1] = sx / (x * x) - cx / x;
+#pragma unroll
+        for (int n = 1; n < LMAX; ++n) j[n + 1] = ((2.0 * n + 1.0) / x) * j[n] - j[n - 1];
+    } else {
+#pragma unroll
+        for (int l = 0; l <= LMAX; ++l) j[l] = jl_f64(l, x);
+    }
+}
+
+// Pair reduction for thin shells and m = 0 shell multipoles, grouped by radius.
+// For octant representative (x, y, z) and its 8 aliases a, the window of radius R
+// and order l is sum_a P(a) j_l(2 pi |a| R / L) P_l(a_z / |a|) (times the
+// descriptor's constant: 1 for a shell, N_l0 for a multipole).  The alias geometry
+// (P product, |a|, Legendre values) is computed once per representative and one
+// sin/cos per (alias, radius) serves every order.  Channel c = (radius u, order l)
+// accumulates sum_k K_c(k) Re F1 conj F2 in FP64; chan[u * (LMAX + 1) + l] is the
+// channel of (u, l) or -1.
+// T = float: FP32 windows (FP64 power and accumulation); T = double: all FP64.
+template <int LMAX, typename T, typename C>
+__global__ __launch_bounds__(RED_THREADS) void k_pair_reduce_radial(int L, const C* __restrict__ F1,
+                                                                    const C* __restrict__ F2,
+                                                                    const double* __restrict__ P,
+                                                                    const double* __restrict__ radii_over_L, int nu,
+                                                                    const int* __restrict__ chan, int nch,
+                                                                    double* __restrict__ partial) {
+    extern __shared__ double wacc[];  // [waves][nch]
+    const int Lh = L / 2 + 1;
+    const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
+    const int nwaves = blockDim.x / 64;
+    for (int i = threadIdx.x; i < nwaves * nch; i += blockDim.x) wacc[i] = 0.0;
+    __syncthreads();
+    const int64_t nrep = (int64_t)Lh * Lh * Lh;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int64_t base = blockIdx.x * (int64_t)blockDim.x; base < nrep; base += stride) {
+        const int64_t r = base + threadIdx.x;
+        const bool active = r < nrep;
+        double power = 0.0;
+        int x = 0, y = 0, z = 0;
+        if (active) {
+            z = (int)(r % Lh);
+            y = (int)((r / Lh) % Lh);
+            x = (int)(r / ((int64_t)Lh * Lh));
+            const int xm = (L - x) % L, ym = (L - y) % L;
+            const int nxm = (xm != x) ? 2 : 1, nym = (ym != y) ? 2 : 1;
+            const double wz = (z == 0 || z == L / 2) ? 1.0 : 2.0;
+            for (int a = 0; a < nxm; ++a)
+                for (int b = 0; b < nym; ++b) {
+                    const int xx = a ? xm : x, yy = b ? ym : y;
+                    const int64_t idx = ((int64_t)xx * L + yy) * Lh + z;
+                    const C f1 = F1[idx], f2 = F2[idx];
+                    power += wz * ((double)f1.x * (double)f2.x + (double)f1.y * (double)f2.y);
+                }
+        }
+        double sa[8];
+        T wl[8][LMAX + 1];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+            const int ix = ax < 0 ? -ax : ax, iy = ay < 0 ? -ay : ay, iz = az < 0 ? -az : az;
+            const T p = (T)(P[ix] * P[iy] * P[iz]);
+            const int n2 = ax * ax + ay * ay + az * az;
+            sa[a] = sqrt((double)n2);
+            const T mu = n2 ? (T)((double)az / sa[a]) : (T)0;
+            T pm = 1, pc = mu;  // P_0, P_1
+            wl[a][0] = p;
+            if (LMAX >= 1) wl[a][1] = n2 ? p * mu : (T)0;
+#pragma unroll
+            for (int n = 1; n < LMAX; ++n) {
+                const T pn = ((T)(2 * n + 1) * mu * pc - (T)n * pm) / (T)(n + 1);
+                pm = pc;
+                pc = pn;
+                wl[a][n + 1] = n2 ? p * pn : (T)0;
+            }
+        }
+        for (int u = 0; u < nu; ++u) {
+            const double R = radii_over_L[u];
+            T kb[LMAX + 1];
+#pragma unroll
+            for (int l = 0; l <= LMAX; ++l) kb[l] = 0;
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                T j[LMAX + 1];
+                radial_jl<LMAX>(sa[a] * R, j);
+#pragma unroll
+                for (int l = 0; l <= LMAX; ++l) kb[l] += j[l] * wl[a][l];
+            }
+#pragma unroll
+            for (int l = 0; l <= LMAX; ++l) {
+                const int c = chan[u * (LMAX + 1) + l];
+                if (c >= 0) {
+                    const double v = wave_sum(power * (double)kb[l]);
+                    if (lane == 0) wacc[wave * nch + c] += v;
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nch; b += blockDim.x) {
+        double s = 0.0;
+        for (int w = 0; w < nwaves; ++w) s += wacc[w * nch + b];
+        partial[(int64_t)blockIdx.x * nch + b] = s;
+    }
+}
+
 __global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, int nd, double* __restrict__ out) {
     // one thread per output; fixed order -> deterministic
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nd; b += gridDim.x * blockDim.x) {
@@ -476,6 +616,88 @@ int hermes_spectrum_mul(hermes_ctx* ctx, int J, int half, int dbl, void* spec, c
     return 0;
 }
 
+// The radial path when every descriptor is a thin shell or an m = 0 shell
+// multipole of order <= 10 (FP32) or <= 4 (FP64).  Returns false (rc untouched)
+// when it does not apply; true with rc = the status otherwise.
+static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
+                               const hermes_window_desc* descs, int n_desc, int nblocks, double* out_host, int& rc) {
+    int lmax = 0;
+    for (int b = 0; b < n_desc; ++b) {
+        const hermes_window_desc& d = descs[b];
+        if (d.type == HERMES_WIN_SHELL) continue;
+        if (d.type != HERMES_WIN_SHELL_MULTIPOLE || d.m != 0 || d.l > (dbl ? 4 : 10)) return false;
+        lmax = d.l > lmax ? d.l : lmax;
+    }
+    const int LM = lmax == 0 ? 0 : (lmax <= 4 ? 4 : 10);
+    // channels: unique (radius, order); each descriptor = its channel times a constant
+    std::vector<double> radii;
+    std::vector<int> chan, ch_of(n_desc);
+    std::vector<double> factor(n_desc);
+    int nch = 0;
+    const int L = 1 << J;
+    for (int b = 0; b < n_desc; ++b) {
+        const hermes_window_desc& d = descs[b];
+        const double R = d.a0 / L;
+        size_t u = 0;
+        while (u < radii.size() && radii[u] != R) ++u;
+        if (u == radii.size()) {
+            radii.push_back(R);
+            chan.insert(chan.end(), LM + 1, -1);
+        }
+        const int l = d.type == HERMES_WIN_SHELL ? 0 : d.l;
+        int& c = chan[u * (LM + 1) + l];
+        if (c < 0) c = nch++;
+        ch_of[b] = c;
+        factor[b] = d.type == HERMES_WIN_SHELL ? 1.0 : hermes_ylm_cnorm(d.l, 0);
+    }
+    const int nu = (int)radii.size();
+    const size_t shmem = sizeof(double) * (RED_THREADS / 64) * nch;
+    if (shmem > 64 * 1024) return false;
+    rc = 0;
+    const size_t off_chan = sizeof(double) * nu;
+    const size_t off_part = ((off_chan + sizeof(int) * chan.size() + 255) / 256) * 256;
+    const size_t need = off_part + sizeof(double) * ((size_t)nblocks * nch + nch);
+    char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);
+    if (!ws) {
+        rc = 1;
+        return true;
+    }
+    double* d_radii = (double*)ws;
+    int* d_chan = (int*)(ws + off_chan);
+    double* partial = (double*)(ws + off_part);
+    double* d_out = partial + (size_t)nblocks * nch;
+    auto fail = [&](hipError_t e) {
+        hermes_set_error("pair reduction: %s", hipGetErrorString(e));
+        rc = 1;
+        return true;
+    };
+    hipError_t e = hipMemcpyAsync(d_radii, radii.data(), sizeof(double) * nu, hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return fail(e);
+    e = hipMemcpyAsync(d_chan, chan.data(), sizeof(int) * chan.size(), hipMemcpyHostToDevice, ctx->stream);
+    if (e != hipSuccess) return fail(e);
+#define HERMES_RADIAL(LMV, T, C)                                                                      \
+    k_pair_reduce_radial<LMV, T, C><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(                    \
+        L, (const C*)F1, (const C*)F2, P, d_radii, nu, d_chan, nch, partial)
+    if (dbl) {
+        if (LM == 0) HERMES_RADIAL(0, double, double2);
+        else HERMES_RADIAL(4, double, double2);
+    } else {
+        if (LM == 0) HERMES_RADIAL(0, float, float2);
+        else if (LM == 4) HERMES_RADIAL(4, float, float2);
+        else HERMES_RADIAL(10, float, float2);
+    }
+#undef HERMES_RADIAL
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    k_sum_partials<<<hermes_blocks(nch, 64), 64, 0, ctx->stream>>>(partial, nblocks, nch, d_out);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    std::vector<double> ch(nch);
+    e = hipMemcpyAsync(ch.data(), d_out, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream);
+    if (e != hipSuccess) return fail(e);
+    if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
+    for (int b = 0; b < n_desc; ++b) out_host[b] = factor[b] * ch[ch_of[b]];
+    return true;
+}
+
 int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
                        const hermes_window_desc* descs, int n_desc, double* out_host) {
     HERMES_REQUIRE(ctx && F1 && F2 && P && descs && out_host, "hermes_pair_reduce: null argument");
@@ -485,12 +707,20 @@ int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const vo
         if (int rc = check_desc(descs[b])) return rc;
         HERMES_REQUIRE(descs[b].type != HERMES_WIN_SHELL_MULTIPOLE || descs[b].m == 0,
                        "pair reduction supports only m = 0 multipole windows");
+        // the half-spectrum sum assumes W(-k) = W(k); an odd-l window is odd in k and its
+        // pair product of real fields vanishes identically
+        HERMES_REQUIRE(descs[b].type != HERMES_WIN_SHELL_MULTIPOLE || descs[b].l % 2 == 0,
+                       "pair reduction supports only even-l multipole windows (odd l gives 0 for real fields)");
     }
     HERMES_CHECK_HIP(hipSetDevice(ctx->device));
     const int L = 1 << J;
     const int Lh = L / 2 + 1;
     const int64_t nrep = (int64_t)Lh * Lh * Lh;
     const int nblocks = (int)hermes_blocks(nrep, RED_THREADS, 2048);
+    {
+        int rc = -1;
+        if (pair_reduce_radial(ctx, J, dbl, F1, F2, P, descs, n_desc, nblocks, out_host, rc)) return rc;
+    }
     size_t need = sizeof(hermes_window_desc) * n_desc + sizeof(float) * n_desc + 64 +
                   sizeof(double) * ((size_t)nblocks * n_desc + n_desc);
     char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);

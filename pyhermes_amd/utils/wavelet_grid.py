@@ -56,8 +56,22 @@ def sample_scaling_function(wavelet_name="db2", level=10):
     return phi[:-1]
 
 
+_POWER_CACHE = {}
+
+
 def fourier_power_spectrum(values, k_min, k_max, k_count, phi_resolution):
-    """|sum_i phi_i dx exp(-2 pi i k x_i)|^2 at k = k_min + j (k_max - k_min) / k_count, j = 0..k_count."""
+    """|sum_i phi_i dx exp(-2 pi i k x_i)|^2 at k = k_min + j (k_max - k_min) / k_count, j = 0..k_count.
+
+    A dense (k_count + 1) x len(phi) DFT on the host (about 30 ms at k_count = 512), so results are
+    kept per (phi samples, k range, resolution); callers get a read-only array.
+    """
+    values = np.ascontiguousarray(values, dtype=np.float64)
+    key = (hash(values.tobytes()), values.shape[0], float(k_min), float(k_max), int(k_count), int(phi_resolution))
+    hit = _POWER_CACHE.get(key)
+    if hit is not None:
+        return hit
+    if len(_POWER_CACHE) > 64:
+        _POWER_CACHE.clear()
     n = values.shape[0]
     dx = (n / phi_resolution) / n
     dk = (k_max - k_min) / k_count
@@ -66,7 +80,10 @@ def fourier_power_spectrum(values, k_min, k_max, k_count, phi_resolution):
     xg, kg = np.meshgrid(x, k)
     re = np.sum(values * dx * np.cos(-2 * np.pi * kg * xg), axis=1)
     im = np.sum(values * dx * np.sin(-2 * np.pi * kg * xg), axis=1)
-    return re ** 2 + im ** 2
+    out = re ** 2 + im ** 2
+    out.setflags(write=False)
+    _POWER_CACHE[key] = out
+    return out
 
 
 def cal_gamma(phi_array, phi_support, phi_resolution):

@@ -73,6 +73,56 @@ def test_pair_reduce_matches_convolution_oracle(phi, dbl):
         assert abs(g - ref) <= tol * scale, (d.type, d.a0, d.l, g, ref)
 
 
+@pytest.mark.parametrize("case", ["shells", "ell4", "ell10"])
+@pytest.mark.parametrize("J", [5, 6])
+def test_pair_reduce_radial_path(phi, case, J):
+    """Shell / m = 0 multipole lists take the radius-grouped FP32 kernel: it matches the
+    convolution oracle, the generic FP32 kernel and the FP64 kernel."""
+    import torch
+    from pyhermes_amd import kernels
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    rng = np.random.default_rng(30 + J)
+    f1 = rng.normal(size=(L, L, L))
+    f2 = f1 + 0.5 * rng.normal(size=(L, L, L))
+    radii = (0.7, 1.5, 3.0, 0.45 * L)
+    if case == "shells":
+        descs = [kernels.window_desc("shell", R) for R in radii]
+    elif case == "ell4":
+        descs = [kernels.window_desc("shell", R) for R in radii]
+        descs += [kernels.window_desc("shell_multipole", R, l=l) for l in (0, 2, 4) for R in radii]
+    else:
+        descs = [kernels.window_desc("shell_multipole", R, l=l) for l in (2, 6, 8, 10) for R in radii[1:]]
+    print(f"radial pair reduce J={J} case={case}", flush=True)
+    F1 = kernels.rfftn(kernels.as_device(f1, torch.float32))
+    F2 = kernels.rfftn(kernels.as_device(f2, torch.float32))
+    got = kernels.pair_reduce(F1, F2, Pd, descs)
+    generic = kernels.pair_reduce(F1, F2, Pd, descs + [kernels.window_desc("sphere", 2.0)])[:-1]
+    G1, G2 = F1.to(torch.complex128), F2.to(torch.complex128)
+    fp64 = kernels.pair_reduce(G1, G2, Pd, descs)
+    fp64_generic = kernels.pair_reduce(G1, G2, Pd, descs + [kernels.window_desc("sphere", 2.0)])[:-1]
+    scale = np.sqrt(np.mean(f1 ** 2) * np.mean(f2 ** 2)) * L ** 6
+    assert np.abs(fp64 - fp64_generic).max() <= 1e-11 * scale
+    assert np.abs(got - generic).max() <= 1e-5 * scale
+    assert np.abs(got - fp64).max() <= 1e-5 * scale
+    for d, g in zip(descs[::3], got[::3]):
+        K = ow.radial_multipole_window(L, P, d.a0, d.l, 0) if d.type == 6 else ow.octant_rfft_kernel(
+            L, P, "shell", d.a0, 0.0)
+        ref = (float(np.sum(np.real(ow.convolve_complex(f1, K)) * f2) / L ** 3) if d.type == 6
+               else ow.pair_product(f1, f2, K))
+        assert abs(g / L ** 6 - ref) <= 2e-5 * scale / L ** 6, (d.type, d.a0, d.l)
+
+
+def test_pair_reduce_rejects_odd_multipoles():
+    import torch
+    from pyhermes_amd import kernels
+    L = 16
+    F = torch.zeros((L, L, L // 2 + 1), dtype=torch.complex64, device="cuda")
+    Pd = kernels.as_device(np.ones(L + 1), torch.float64)
+    with pytest.raises(RuntimeError):
+        kernels.pair_reduce(F, F, Pd, [kernels.window_desc("shell_multipole", 2.0, l=3)])
+
+
 @pytest.mark.parametrize("l,m", [(0, 0), (2, 1), (3, -2), (6, 6)])
 def test_multipole_apply_matches_oracle(phi, l, m):
     import torch
