@@ -267,7 +267,8 @@ def run_2pcf_workload(args):
     rho = eps.sum() / L ** 3
     d = eps - rho
     P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
-    xi_cpu = np.array([ow.pair_product(d, d, ow.octant_rfft_kernel(L, P, "shell", sv * L / box)) for sv in s_edges])
+    xi_cpu = np.array([ow.pair_product(d, d, ow.octant_rfft_kernel(L, P, "shell", sv * L / box), threads)
+                       for sv in s_edges])
     xi_cpu = xi_cpu / rho ** 2
     t_cpu = time.perf_counter() - t0
     err = float(np.max(np.abs(res.xi - xi_cpu)) / np.max(np.abs(xi_cpu)))
@@ -367,10 +368,10 @@ def run_xi_ell_workload(args):
         P = scaling.fourier_power_spectrum(phi, 0, 1, Ls, 1024)
         R = 60.0 * Ls / box
         t0 = time.perf_counter()
-        ow.pair_product(d, d, ow.octant_rfft_kernel(Ls, P, "shell", R))
+        ow.pair_product(d, d, ow.octant_rfft_kernel(Ls, P, "shell", R), threads)
         for l in ells:
             K = ow.radial_multipole_window(Ls, P, R, l, 0)
-            np.sum(np.real(ow.convolve_complex(d, K)) * d)
+            np.sum(np.real(ow.convolve_complex(d, K, threads)) * d)
         t_sample = time.perf_counter() - t0
         t_full = t_sample * 8.0 * nbin
         cpu = {"value": 1.0 / t_full, "unit": "catalogues/s", "cores": threads, "kind": "port",
@@ -391,11 +392,130 @@ def run_xi_ell_workload(args):
     print(json.dumps(line))
 
 
+def run_batch_workload(args):
+    """Uneven batch: 64 seeded lognormal sub-box catalogues, sizes log-uniform in 1e4..2e6 particles, box
+    sides 250-1000 Mpc/h, each with its own J in {6, 7, 8}; every catalogue runs the isotropic 2PCF (20 shell
+    bins, 10-100 Mpc/h) and the auto SH-3PCF multipoles (8 shells, 10-80 Mpc/h, l <= 6).  The whole batch is
+    captured once into a hipGraph and replayed; value = catalogues/s of the replay.  Per-catalogue latency is
+    the replay time of a graph of that catalogue alone (p50/p99 over the 64).  Also reported: the same batch
+    enqueued eagerly, the warm-up (FFT tables, tuned tiles, workspaces) and the capture time.
+    """
+    import numpy as np
+    import torch
+
+    from oracle import native, projection as oproj, scaling, windows as ow
+    from pyhermes_amd import kernels
+    from pyhermes_amd.batch import Catalogue, CatalogueBatch, timed_replays, wall
+    from pyhermes_amd.mocks import lognormal_mock
+    from pyhermes_amd.utils.wavelet_grid import sample_scaling_function
+
+    assert torch.cuda.is_available(), "bench needs a HIP device"
+    torch.cuda.set_device(0)
+    rng = np.random.default_rng(5000)
+    ncat = args.batch_catalogues
+    sizes = np.round(10.0 ** rng.uniform(4.0, np.log10(2e6), ncat)).astype(np.int64)
+    boxes = rng.uniform(250.0, 1000.0, ncat)
+    Js = rng.choice([6, 7, 8], ncat)
+    log(f"batch: {ncat} catalogues, {int(sizes.sum())} particles, J counts "
+        f"{ {int(j): int((Js == j).sum()) for j in (6, 7, 8)} }")
+    cats = []
+    for i in range(ncat):
+        pos, w = lognormal_mock(int(sizes[i]), 64, float(boxes[i]), 5000 + 10 * i)
+        cats.append(Catalogue(pos, w, float(boxes[i]), int(Js[i])))
+    phi = sample_scaling_function()
+    s_bins = np.linspace(10.0, 100.0, 20)
+    shells = np.linspace(10.0, 80.0, 8)
+    lmax = 6
+    b = CatalogueBatch(cats, s_bins, shells, lmax, phi)
+    stream = torch.cuda.Stream()
+    with torch.cuda.stream(stream):
+        _, t_warm = wall(b.warmup)
+        res_eager, t_eager = wall(b.run_eager)
+        log(f"warm-up {t_warm:.2f} s; eager batch {t_eager * 1e3:.1f} ms")
+        t0 = time.perf_counter()
+        g_all = b.capture()
+        t_capture = time.perf_counter() - t0
+        b.replay(g_all)  # first replay (uploads the graph)
+        torch.cuda.synchronize()
+        reps = max(args.steps, 3)
+        ms_all = timed_replays(b, [g_all] * reps, stream)
+        b.check_errors()
+        res_graph = b.results()
+        same = all(np.array_equal(r["xi"], q["xi"]) and np.array_equal(r["zeta_l"], q["zeta_l"])
+                   for r, q in zip(res_eager, res_graph))
+        log(f"capture {t_capture * 1e3:.1f} ms; graph replay {np.mean(ms_all):.1f} ms; identical to eager: {same}")
+        graphs = [b.capture([i]) for i in range(ncat)]
+        timed_replays(b, graphs, stream)  # first replay of each
+        lat = np.array(timed_replays(b, graphs, stream))
+        eager_lat = []
+        for i in range(ncat):
+            _, t = wall(lambda: b.enqueue(i))
+            eager_lat.append(t * 1e3)
+        for g in graphs + [g_all]:
+            kernels.graph_destroy(g)
+    t_graph = float(np.mean(ms_all)) * 1e-3
+    by_J = {int(j): {"catalogues": int((Js == j).sum()),
+                     "p50_ms": float(np.percentile(lat[Js == j], 50)) if (Js == j).any() else None}
+            for j in (6, 7, 8)}
+    cpu = None
+    if not args.skip_cpu:
+        # bounded sample: the oracle's 20 shell-bin pair products and 3 (l, m) terms of one shell pair for one
+        # J=6 catalogue, scaled to the batch: x 8^(J-6) volume per catalogue, 20 bins and 28 m-terms x 36 pairs
+        i6 = int(np.nonzero(Js == 6)[0][0]) if (Js == 6).any() else 0
+        c = cats[i6]
+        L = 64
+        threads = native.num_threads()
+        pwn, _ = oproj.projection_weights(c.weight, np.ones(c.n), "catalog")
+        eps, _, _ = oproj.project_c(c.pos, pwn, phi, 6, c.box)
+        d = eps - eps.sum() / L ** 3
+        P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+        gamma = scaling.cal_gamma(phi, 3, 1024)
+        from oracle import multipole as om
+        t0 = time.perf_counter()
+        for sv in s_bins:
+            ow.pair_product(d, d, ow.octant_rfft_kernel(L, P, "shell", sv * L / c.box), threads)
+        t_2pcf = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        terms = [(0, 0), (2, 1), (4, 3)]
+        for (l, m) in terms:
+            F1 = ow.convolve_complex(d, ow.radial_multipole_window(L, P, shells[2] * L / c.box, l, m), threads)
+            F2 = ow.convolve_complex(d, ow.radial_multipole_window(L, P, shells[5] * L / c.box, l, -m), threads)
+            om.summand_c(d, F1, F2, gamma)
+        t_term = (time.perf_counter() - t0) / len(terms)
+        n_pairs = len(shells) * (len(shells) + 1) // 2
+        n_m = (lmax + 1) * (lmax + 2) // 2
+        t_cpu = sum(8.0 ** (int(j) - 6) * (t_2pcf + t_term * n_m * n_pairs) for j in Js)
+        cpu = {"value": ncat / t_cpu, "unit": "catalogues/s", "cores": threads, "kind": "port",
+               "sample": f"C/OpenMP + scipy.fft oracle on one J=6 catalogue: 20 shell-bin pair products "
+                         f"({t_2pcf:.2f} s) and 3 (l,m) terms of one shell pair ({t_term:.2f} s each), scaled by "
+                         f"8^(J-6) per catalogue, {n_m} m-terms x {n_pairs} pairs"}
+    line = {
+        "metric": "uneven catalogue batch (2PCF 20 bins + SH-3PCF l<=6, 8 shells) catalogues/s from one hipGraph",
+        "value": ncat / t_graph, "unit": "catalogues/s", "n_gpus": 1, "higher_is_better": True,
+        "ms_per_batch": t_graph * 1e3, "dtype": "f32 (f64 normalisation and accumulators)",
+        "data": "synthetic lognormal mocks (seeded)",
+        "config": {"workload": f"batch_{ncat}_catalogues_J678_2pcf20_sh3pcf_l6_8shells",
+                   "particles_total": int(sizes.sum()), "sizes_range": [int(sizes.min()), int(sizes.max())],
+                   "J_counts": {str(j): int((Js == j).sum()) for j in (6, 7, 8)}},
+        "latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                       "max": float(lat.max()), "by_J": by_J},
+        "eager": {"catalogues_per_s": ncat / t_eager, "p50_ms": float(np.percentile(eager_lat, 50)),
+                  "p99_ms": float(np.percentile(eager_lat, 99))},
+        "graph_identical_to_eager": bool(same),
+        "setup_s": {"warmup": t_warm, "capture_batch": t_capture},
+        "cpu_baseline": cpu,
+    }
+    b.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=["sh3pcf", "mc", "2pcf", "xi_ell"], default="sh3pcf",
+    ap.add_argument("--workload", choices=["sh3pcf", "mc", "2pcf", "xi_ell", "batch"], default="sh3pcf",
                     help="sh3pcf: the multipole sets/s headline; mc: learned vs uniform Monte-Carlo 3PCF; "
-                         "2pcf: FP64 isotropic 2PCF at J=7; xi_ell: FP32 2PCF multipoles with RSD at J=9")
+                         "2pcf: FP64 isotropic 2PCF at J=7; xi_ell: FP32 2PCF multipoles with RSD at J=9; "
+                         "batch: 64 uneven catalogues from one hipGraph")
+    ap.add_argument("--batch-catalogues", type=int, default=64)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
@@ -422,7 +542,8 @@ def main():
         print(json.dumps({"error": "this build measures one GPU only; run with --gpus 1", "n_gpus": args.gpus}))
         sys.exit(2)
     if args.workload != "sh3pcf":
-        {"mc": run_mc_workload, "2pcf": run_2pcf_workload, "xi_ell": run_xi_ell_workload}[args.workload](args)
+        {"mc": run_mc_workload, "2pcf": run_2pcf_workload, "xi_ell": run_xi_ell_workload,
+         "batch": run_batch_workload}[args.workload](args)
         return
 
     import numpy as np

@@ -39,6 +39,26 @@ int hermes_ctx_destroy(hermes_ctx *ctx);
 int hermes_ctx_set_stream(hermes_ctx *ctx, void *hip_stream);
 int hermes_sync(hermes_ctx *ctx);
 
+/* ---- graphs of whole batches (no reference counterpart; the reference runs each
+ * catalogue eagerly).  hermes_capture_begin starts capturing the context's stream
+ * (install an explicit stream first; the null stream cannot be captured); until
+ * hermes_capture_end, the capturable entry points below only enqueue kernels:
+ * hermes_project, hermes_delta_f32, hermes_half_spectrum_f32, hermes_fft3_c2c_f32,
+ * hermes_multipole_window_ifft_f32, hermes_pair_reduce_planned,
+ * hermes_multipole_gram_f32_dev, hermes_clear_errors.  They fail (status 2) instead
+ * of allocating: run the same work once eagerly first, so workspaces, FFT tables
+ * and tuned tiles exist, and call hermes_project_reserve.  Invalid particle
+ * positions found while capturing or replaying set the context's sticky error word
+ * (hermes_read_errors; hermes_clear_errors resets it on the stream).
+ * hermes_capture_end instantiates the graph; *exec_out is replayed on the context
+ * stream by hermes_graph_launch and released by hermes_graph_destroy. */
+int hermes_capture_begin(hermes_ctx *ctx);
+int hermes_capture_end(hermes_ctx *ctx, void **exec_out);
+int hermes_graph_launch(hermes_ctx *ctx, void *exec);
+int hermes_graph_destroy(void *exec);
+int hermes_clear_errors(hermes_ctx *ctx);
+int hermes_read_errors(hermes_ctx *ctx, int *out_host);
+
 /* ---- projection: replaces project_scaling_grid_numba, wavelet_grid.py:17-70
  *      (called from SFCProjection.run, sfc_projection.py:317-324).
  * pos (n,3) f64 and w (n,) f64 projection weights; phi (phi_len,) f64 samples,
@@ -50,6 +70,20 @@ int hermes_sync(hermes_ctx *ctx);
 int hermes_project(hermes_ctx *ctx, const double *pos, const double *w, int64_t n, int J, double box,
                    const double *phi, int phi_len, int phi_res, double phi_abs_max, float *eps32,
                    double *eps64, int32_t *cell_out, int32_t *frac_out, int32_t *cell_counts);
+
+/* Grows the projection workspaces to what a captured hermes_project of up to n_max
+ * particles at level J needs (8 tile records per particle at most).  No counterpart. */
+int hermes_project_reserve(hermes_ctx *ctx, int J, int64_t n_max);
+
+/* delta = (float)(a * eps - b) over n cells, written as a real grid (d_out) and/or as
+ * the real part of a complex64 grid (dc_out) for hermes_fft3_c2c_f32; with a = 1,
+ * b = rho it is the field minus the uniform random density of
+ * _prepare_product_fields (corr3pcf_multipole.py:1363-1384). */
+int hermes_delta_f32(hermes_ctx *ctx, int64_t n, const float *eps, double a, double b, float *d_out, void *dc_out);
+
+/* (L, L, L/2+1) half spectrum of a real field from its full (L, L, L) complex64
+ * transform: the rfftn layout the window and pair-product entry points take. */
+int hermes_half_spectrum_f32(hermes_ctx *ctx, int J, const void *full, void *half);
 
 
 /* ---- 3-D FFTs (rocFFT), replacing scipy.fft rfftn / irfftn / fftn / ifftn in
@@ -120,6 +154,17 @@ int hermes_spectrum_mul(hermes_ctx *ctx, int J, int half, int dbl, void *spec, c
 int hermes_pair_reduce(hermes_ctx *ctx, int J, int dbl, const void *F1, const void *F2, const double *P,
                        const hermes_window_desc *descs, int n_desc, double *out_host);
 
+/* The same reduction with its window tables prepared once and the result left on
+ * the device (capturable): a plan takes thin shells and even-l, m = 0 shell
+ * multipoles (l <= 10 in FP32, l <= 4 in FP64) for level J.  out_dev receives
+ * n_desc doubles.  Plans own their device memory (hermes_pair_plan_destroy). */
+typedef struct hermes_pair_plan hermes_pair_plan;
+int hermes_pair_plan_create(hermes_ctx *ctx, int J, int dbl, const hermes_window_desc *descs, int n_desc,
+                            hermes_pair_plan **out);
+int hermes_pair_plan_destroy(hermes_pair_plan *plan);
+int hermes_pair_reduce_planned(hermes_ctx *ctx, const hermes_pair_plan *plan, const void *F1, const void *F2,
+                               const double *P, double *out_dev);
+
 /* Fused multipole window stage of the batched SH-3PCF path (FP32): from the half
  * spectrum H (L, L, L/2+1) complex64 of a real field, writes for every shell
  * radius R_s (grid units) the full spectrum
@@ -164,6 +209,13 @@ int hermes_multipole_summand_f64(hermes_ctx *ctx, int J, const double *d1, const
 int hermes_multipole_gram_f32(hermes_ctx *ctx, int J, const float *d1, const void *U, int n_u, const void *V,
                               int n_v, int v_conj, const double *Q3x3_host, const double *lam3_host,
                               double *out_host);
+
+/* The same contraction with the raw 16 x 16 block of sums left on the device
+ * (capturable): out_dev[i * 16 + j] for i < n_u, j < n_v (V given) or j < n_u
+ * (V = NULL, where the result above is out_dev[i*16+j] + out_dev[j*16+i]). */
+int hermes_multipole_gram_f32_dev(hermes_ctx *ctx, int J, const float *d1, const void *U, int n_u, const void *V,
+                                  int n_v, int v_conj, const double *Q3x3_host, const double *lam3_host,
+                                  double *out_dev);
 
 /* Checks the lane layout of the 16x16x4 FP32 matrix instruction used above;
  * max_abs_err = 0 on a conforming device.  (No counterpart in the reference.) */

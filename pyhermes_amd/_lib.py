@@ -34,6 +34,15 @@ _SIGNATURES = {
     "hermes_fft3_tile_choice": [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "hermes_fft3_set_tile": [c_void_p, c_int, c_int, c_int, c_int],
     "hermes_profile_gram": [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_i64)],
+    "hermes_capture_begin": [c_void_p],
+    "hermes_capture_end": [c_void_p, ctypes.POINTER(c_void_p)],
+    "hermes_graph_launch": [c_void_p, c_void_p],
+    "hermes_graph_destroy": [c_void_p],
+    "hermes_clear_errors": [c_void_p],
+    "hermes_read_errors": [c_void_p, ctypes.POINTER(c_int)],
+    "hermes_project_reserve": [c_void_p, c_int, c_i64],
+    "hermes_delta_f32": [c_void_p, c_i64, c_void_p, c_double, c_double, c_void_p, c_void_p],
+    "hermes_half_spectrum_f32": [c_void_p, c_int, c_void_p, c_void_p],
 }
 
 
@@ -58,6 +67,11 @@ _SIGNATURES.update({
     "hermes_multipole_summand_f64": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, _dpp, _dpp],
     "hermes_multipole_gram_f32": [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, _dpp, _dpp,
                                   _dpp],
+    "hermes_multipole_gram_f32_dev": [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, _dpp,
+                                      _dpp, c_void_p],
+    "hermes_pair_plan_create": [c_void_p, c_int, c_int, ctypes.POINTER(WindowDesc), c_int, ctypes.POINTER(c_void_p)],
+    "hermes_pair_plan_destroy": [c_void_p],
+    "hermes_pair_reduce_planned": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "hermes_mfma_selftest": [c_void_p, _dpp],
     "hermes_interpolate": [c_void_p, c_int, c_void_p, c_void_p, c_i64, c_void_p, c_int, c_double, c_double, c_double,
                            c_void_p],

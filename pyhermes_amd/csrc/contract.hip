@@ -474,9 +474,11 @@ int hermes_multipole_summand_f64(hermes_ctx* ctx, int J, const double* d1, const
     return 0;
 }
 
-int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u, const void* V, int n_v,
-                              int v_conj, const double* Q3x3_host, const double* lam3_host, double* out_host) {
-    HERMES_REQUIRE(ctx && d1 && U && Q3x3_host && lam3_host && out_host, "hermes_multipole_gram_f32: null argument");
+// Launches the Gram kernel and the fixed-order row reduction; the 16 x 16 raw sums land in
+// out_dev (256 doubles; row i, column j at i * 16 + j).  No host synchronisation.
+static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u, const void* V, int n_v,
+                       int v_conj, const double* Q3x3_host, const double* lam3_host, double* out_dev, GramArgs* gout) {
+    HERMES_REQUIRE(ctx && d1 && U && Q3x3_host && lam3_host && out_dev, "hermes_multipole_gram_f32: null argument");
     HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_gram_f32: J=%d unsupported (needs L >= 8)", J);
     HERMES_REQUIRE(n_u >= 1 && n_u <= 16, "n_u=%d outside 1..16", n_u);
     HERMES_REQUIRE(V == nullptr || (n_v >= 1 && n_v <= 16), "n_v=%d outside 1..16", n_v);
@@ -508,18 +510,30 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
     double* ws = (double*)hermes_workspace(
         ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256 + (size_t)RED_MAX_BLOCKS * 256));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
-    if (ctx->time_gram) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[0], ctx->stream));
+    const bool timed = ctx->time_gram && !ctx->capturing;
+    if (timed) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[0], ctx->stream));
     if (g.cross)
         k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     else
         k_gram_auto<<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws);
     HERMES_LAUNCH_CHECK();
-    if (ctx->time_gram) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[1], ctx->stream));
-    sum_rows(ctx->stream, ws, nblocks, 256, ws + (size_t)nblocks * 256 + 256, ws + (size_t)nblocks * 256);
+    if (timed) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[1], ctx->stream));
+    sum_rows(ctx->stream, ws, nblocks, 256, ws + (size_t)nblocks * 256 + 256, out_dev);
     HERMES_LAUNCH_CHECK();
+    if (gout) *gout = g;
+    return 0;
+}
+
+int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u, const void* V, int n_v,
+                              int v_conj, const double* Q3x3_host, const double* lam3_host, double* out_host) {
+    HERMES_REQUIRE(ctx && out_host, "hermes_multipole_gram_f32: null argument");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_multipole_gram_f32 returns host data; capture hermes_multipole_gram_f32_dev");
+    double* dst = (double*)hermes_workspace(ctx, WS_MISC, sizeof(double) * 256);
+    HERMES_REQUIRE(dst, "%s", hermes_last_error());
+    GramArgs g;
+    if (int rc = gram_launch(ctx, J, d1, U, n_u, V, n_v, v_conj, Q3x3_host, lam3_host, dst, &g)) return rc;
     double h[256];
-    HERMES_CHECK_HIP(hipMemcpyAsync(h, ws + (size_t)nblocks * 256, sizeof(double) * 256, hipMemcpyDeviceToHost,
-                                    ctx->stream));
+    HERMES_CHECK_HIP(hipMemcpyAsync(h, dst, sizeof(double) * 256, hipMemcpyDeviceToHost, ctx->stream));
     HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     if (ctx->time_gram) {
         float ms = 0.0f;
@@ -531,6 +545,12 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
         for (int j = 0; j < g.S2; ++j)
             out_host[i * g.S2 + j] = g.cross ? h[i * 16 + j] : h[i * 16 + j] + h[j * 16 + i];
     return 0;
+}
+
+int hermes_multipole_gram_f32_dev(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u, const void* V,
+                                  int n_v, int v_conj, const double* Q3x3_host, const double* lam3_host,
+                                  double* out_dev) {
+    return gram_launch(ctx, J, d1, U, n_u, V, n_v, v_conj, Q3x3_host, lam3_host, out_dev, nullptr);
 }
 
 int hermes_profile_gram(hermes_ctx* ctx, int enable, double* total_ms_out, int64_t* calls_out) {

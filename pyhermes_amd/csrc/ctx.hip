@@ -14,9 +14,28 @@ void hermes_set_error(const char* fmt, ...) {
     va_end(ap);
 }
 
+__global__ void k_fill_u32(uint32_t* __restrict__ p, size_t n, uint32_t v) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+hipError_t hermes_fill_u32(hipStream_t st, void* p, size_t n, uint32_t v) {
+    if (n == 0) return hipSuccess;
+    size_t nb = (n + 255) / 256;
+    if (nb > 4096) nb = 4096;
+    k_fill_u32<<<(unsigned)nb, 256, 0, st>>>((uint32_t*)p, n, v);
+    return hipGetLastError();
+}
+
 void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes) {
     if (bytes == 0) bytes = 256;
     if (ctx->ws_size[slot] >= bytes) return ctx->ws[slot];
+    if (ctx->capturing) {
+        hermes_set_error("workspace slot %d would have to grow to %zu bytes during stream capture; run the same "
+                         "work once before capturing it",
+                         slot, bytes);
+        return nullptr;
+    }
     if (ctx->ws[slot]) {
         (void)hipStreamSynchronize(ctx->stream);
         (void)hipFree(ctx->ws[slot]);
@@ -64,6 +83,7 @@ int hermes_ctx_destroy(hermes_ctx* ctx) {
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     hermes_fft_cache_destroy(ctx->fft);
     if (ctx->twiddles) (void)hipFree(ctx->twiddles);
+    if (ctx->dev_err) (void)hipFree(ctx->dev_err);
     for (int i = 0; i < 2; ++i)
         if (ctx->gram_ev[i]) (void)hipEventDestroy(ctx->gram_ev[i]);
     delete ctx;
@@ -72,7 +92,71 @@ int hermes_ctx_destroy(hermes_ctx* ctx) {
 
 int hermes_ctx_set_stream(hermes_ctx* ctx, void* stream) {
     HERMES_REQUIRE(ctx != nullptr, "null context");
+    HERMES_REQUIRE(!ctx->capturing || (hipStream_t)stream == ctx->stream,
+                   "hermes_ctx_set_stream: another stream is being captured");
     ctx->stream = (hipStream_t)stream;
+    return 0;
+}
+
+int hermes_capture_begin(hermes_ctx* ctx) {
+    HERMES_REQUIRE(ctx != nullptr, "null context");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_capture_begin: already capturing");
+    HERMES_REQUIRE(ctx->stream != nullptr,
+                   "hermes_capture_begin: install an explicit stream first (the null stream cannot be captured)");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    if (!ctx->dev_err) {
+        HERMES_CHECK_HIP(hipMalloc(&ctx->dev_err, 256));
+        HERMES_CHECK_HIP(hermes_fill_u32(ctx->stream, ctx->dev_err, 64, 0u));
+        HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    }
+    HERMES_CHECK_HIP(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+    ctx->capturing = true;
+    return 0;
+}
+
+int hermes_capture_end(hermes_ctx* ctx, void** exec_out) {
+    HERMES_REQUIRE(ctx != nullptr && exec_out != nullptr, "hermes_capture_end: null argument");
+    HERMES_REQUIRE(ctx->capturing, "hermes_capture_end: not capturing");
+    *exec_out = nullptr;
+    ctx->capturing = false;
+    hipGraph_t graph = nullptr;
+    HERMES_CHECK_HIP(hipStreamEndCapture(ctx->stream, &graph));
+    hipGraphExec_t exec = nullptr;
+    const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    HERMES_CHECK_HIP(e);
+    *exec_out = (void*)exec;
+    return 0;
+}
+
+int hermes_graph_launch(hermes_ctx* ctx, void* exec) {
+    HERMES_REQUIRE(ctx != nullptr && exec != nullptr, "hermes_graph_launch: null argument");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_graph_launch: the stream is being captured");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    HERMES_CHECK_HIP(hipGraphLaunch((hipGraphExec_t)exec, ctx->stream));
+    return 0;
+}
+
+int hermes_graph_destroy(void* exec) {
+    if (exec) HERMES_CHECK_HIP(hipGraphExecDestroy((hipGraphExec_t)exec));
+    return 0;
+}
+
+int hermes_clear_errors(hermes_ctx* ctx) {
+    HERMES_REQUIRE(ctx != nullptr, "null context");
+    HERMES_REQUIRE(ctx->dev_err != nullptr, "hermes_clear_errors: no error word yet (capture once first)");
+    HERMES_CHECK_HIP(hermes_fill_u32(ctx->stream, ctx->dev_err, 1, 0u));
+    return 0;
+}
+
+int hermes_read_errors(hermes_ctx* ctx, int* out_host) {
+    HERMES_REQUIRE(ctx != nullptr && out_host != nullptr, "hermes_read_errors: null argument");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_read_errors: the stream is being captured");
+    *out_host = 0;
+    if (!ctx->dev_err) return 0;
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    HERMES_CHECK_HIP(hipMemcpyAsync(out_host, ctx->dev_err, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
 }
 

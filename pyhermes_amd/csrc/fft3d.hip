@@ -17,6 +17,10 @@ using namespace hermes_fft;
 
 // e^{dir 2 pi i k / 2^j} for j = 0..10, k < 2^j, computed in FP64 and rounded once.
 const float2* hermes_twiddles(hermes_ctx* ctx, int log2L, int dir) {
+    if (!ctx->twiddles && ctx->capturing) {
+        hermes_set_error("FFT twiddle tables are built on first use; run an FFT once before capturing");
+        return nullptr;
+    }
     if (!ctx->twiddles) {
         std::vector<float2> h((size_t)11 * 2 * 1024);
         for (int j = 0; j <= 10; ++j)
@@ -161,6 +165,8 @@ int tuned_variant(hermes_ctx* ctx, const float2* twg, int* out) {
         *out = 0;
         return 0;
     }
+    HERMES_REQUIRE(!ctx->capturing, "the FFT tile for L=%d is tuned on first use; run it once before capturing",
+                   1 << LOG2L);
     constexpr int L = 1 << LOG2L;
     const size_t bytes = sizeof(float2) * (size_t)L * L * L;
     float2* scratch = nullptr;
@@ -243,6 +249,27 @@ int dispatch_fft3(hermes_ctx* ctx, int J, int batch, float2* data, bool z_pass) 
     return 2;
 }
 
+// d = (float)(a * eps - b) as a real FP32 grid and as the real part of a complex one.
+__global__ void k_delta_f32(const float* __restrict__ eps, int64_t n, double a, double b, float* __restrict__ d,
+                            float2* __restrict__ dc) {
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const float v = (float)(a * (double)eps[i] - b);
+        if (d) d[i] = v;
+        if (dc) dc[i] = make_float2(v, 0.0f);
+    }
+}
+
+// (L, L, L/2+1) half spectrum of a real field from its full (L, L, L) transform.
+__global__ void k_half_spectrum(const float2* __restrict__ full, int L, float2* __restrict__ half) {
+    const int Lh = L / 2 + 1;
+    const int64_t n = (int64_t)L * L * Lh;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = i / Lh;
+        const int z = (int)(i - row * Lh);
+        half[i] = full[row * L + z];
+    }
+}
+
 }  // namespace
 
 // Inverse strided passes only: completes a batch whose z lines were already transformed
@@ -282,6 +309,26 @@ int hermes_fft3_set_tile(hermes_ctx* ctx, int J, int inverse, int tw, int thread
     HERMES_REQUIRE(v >= 0, "hermes_fft3_set_tile: (tw=%d, threads=%d) is not a candidate tile for J=%d", tw, threads,
                    J);
     ctx->fft_cols_variant[J][inverse ? 1 : 0] = v + 1;
+    return 0;
+}
+
+int hermes_delta_f32(hermes_ctx* ctx, int64_t n, const float* eps, double a, double b, float* d_out,
+                     void* dc_out) {
+    HERMES_REQUIRE(ctx && eps && (d_out || dc_out) && n >= 1, "hermes_delta_f32: invalid argument");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    k_delta_f32<<<hermes_blocks(n, 256, 8192), 256, 0, ctx->stream>>>(eps, n, a, b, d_out, (float2*)dc_out);
+    HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+int hermes_half_spectrum_f32(hermes_ctx* ctx, int J, const void* full, void* half) {
+    HERMES_REQUIRE(ctx && full && half, "hermes_half_spectrum_f32: null argument");
+    HERMES_REQUIRE(J >= 1 && J <= 10, "hermes_half_spectrum_f32: J=%d unsupported", J);
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int L = 1 << J;
+    const int64_t n = (int64_t)L * L * (L / 2 + 1);
+    k_half_spectrum<<<hermes_blocks(n, 256, 8192), 256, 0, ctx->stream>>>((const float2*)full, L, (float2*)half);
+    HERMES_LAUNCH_CHECK();
     return 0;
 }
 

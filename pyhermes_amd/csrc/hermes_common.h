@@ -26,6 +26,10 @@ struct hermes_ctx {
     hipEvent_t gram_ev[2] = {nullptr, nullptr};
     double gram_ms = 0.0;
     int64_t gram_calls = 0;
+    // stream capture (hermes_capture_begin/end): while set, nothing may allocate, copy to or
+    // from the host, memset or synchronise; kernels report input errors into dev_err
+    bool capturing = false;
+    int* dev_err = nullptr;  // sticky device error word (see hermes_read_errors)
 };
 
 // Workspace slots (each slot is an independent grow-only buffer).
@@ -41,6 +45,8 @@ enum hermes_ws_slot {
 };
 
 void hermes_set_error(const char* fmt, ...);
+// Fill n 32-bit words with v on the stream (a kernel, so it can be captured into graphs).
+hipError_t hermes_fill_u32(hipStream_t st, void* p, size_t n, uint32_t v);
 void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes);
 // Twiddle table e^{dir 2 pi i k / 2^log2L}, k < 2^log2L (device memory owned by ctx).
 const float2* hermes_twiddles(hermes_ctx* ctx, int log2L, int dir);

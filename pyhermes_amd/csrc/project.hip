@@ -435,17 +435,21 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
     int* cursor = off + ntiles + 1;
     int* err = cursor + ntiles;
     long long* bsum = (long long*)(tw + (((size_t)ntiles * 8 + (size_t)ntiles * 4 * 3 + 16 + 15) / 16) * 16);
-    HERMES_CHECK_HIP(hipMemsetAsync(count, 0, (size_t)ntiles * 4, st));
+    HERMES_CHECK_HIP(hermes_fill_u32(st, count, (size_t)ntiles, 0u));
     unsigned long long* wmax = (unsigned long long*)tw;  // first 8 bytes of the u64 slot
-    HERMES_CHECK_HIP(hipMemsetAsync(wmax, 0, sizeof(unsigned long long), st));
+    HERMES_CHECK_HIP(hermes_fill_u32(st, wmax, 2, 0u));
     if (n > 0) {
         k_absmax<<<hermes_blocks(n, 256, 1024), 256, 0, st>>>(w, n, wmax);
         HERMES_LAUNCH_CHECK();
     }
-    HERMES_CHECK_HIP(hipMemsetAsync(err, 0, sizeof(int), st));
+    // Under stream capture invalid positions are reported through the context's sticky error word
+    // (they are projected at the origin, in bounds); eagerly the call checks and fails here.
+    int* errp = ctx->capturing ? ctx->dev_err : err;
+    HERMES_REQUIRE(errp, "hermes_project: no device error word");
+    if (!ctx->capturing) HERMES_CHECK_HIP(hermes_fill_u32(st, err, 1, 0u));
 
     k_proj_count<<<hermes_blocks(n, 256, 4096), 256, 0, st>>>(pos, n, sf, phi_res, g, count, cell_out, frac_out,
-                                                              err);
+                                                              errp);
     HERMES_LAUNCH_CHECK();
     k_scan_reduce<<<nscan, SCAN_THREADS, 0, st>>>(count, ntiles, bsum);
     HERMES_LAUNCH_CHECK();
@@ -453,14 +457,21 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
     HERMES_LAUNCH_CHECK();
     k_scan_apply<<<nscan, SCAN_THREADS, 0, st>>>(count, ntiles, bsum, nscan, off, cursor);
     HERMES_LAUNCH_CHECK();
-    int host[2] = {0, 0};
-    HERMES_CHECK_HIP(hipMemcpyAsync(&host[0], err, sizeof(int), hipMemcpyDeviceToHost, st));
-    HERMES_CHECK_HIP(hipMemcpyAsync(&host[1], off + ntiles, sizeof(int), hipMemcpyDeviceToHost, st));
-    HERMES_CHECK_HIP(hipStreamSynchronize(st));
-    HERMES_REQUIRE(host[0] == 0, "hermes_project: non-finite or out-of-range particle positions");
-    const int total = host[1];
-    ProjRec* rec = (ProjRec*)hermes_workspace(ctx, WS_PROJ_REC, (size_t)(total > 0 ? total : 1) * sizeof(ProjRec));
-    HERMES_REQUIRE(rec, "%s", hermes_last_error());
+    ProjRec* rec = nullptr;
+    if (ctx->capturing) {
+        // a particle's 3-cell footprint meets at most 2 tiles per axis (tile sides >= 4): <= 8 records
+        rec = (ProjRec*)hermes_workspace(ctx, WS_PROJ_REC, (size_t)(n > 0 ? 8 * n : 1) * sizeof(ProjRec));
+        HERMES_REQUIRE(rec, "%s (hermes_project_reserve sizes it)", hermes_last_error());
+    } else {
+        int host[2] = {0, 0};
+        HERMES_CHECK_HIP(hipMemcpyAsync(&host[0], err, sizeof(int), hipMemcpyDeviceToHost, st));
+        HERMES_CHECK_HIP(hipMemcpyAsync(&host[1], off + ntiles, sizeof(int), hipMemcpyDeviceToHost, st));
+        HERMES_CHECK_HIP(hipStreamSynchronize(st));
+        HERMES_REQUIRE(host[0] == 0, "hermes_project: non-finite or out-of-range particle positions");
+        const int total = host[1];
+        rec = (ProjRec*)hermes_workspace(ctx, WS_PROJ_REC, (size_t)(total > 0 ? total : 1) * sizeof(ProjRec));
+        HERMES_REQUIRE(rec, "%s", hermes_last_error());
+    }
     if (n > 0) {
         k_proj_fill<<<hermes_blocks(n, 256, 4096), 256, 0, st>>>(pos, w, n, sf, phi_res, g, off, cursor, rec);
         HERMES_LAUNCH_CHECK();
@@ -486,5 +497,25 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
                                                                                phimax3, ntiles, eps32, eps64, nullptr);
     }
     HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+// Grows the projection workspaces to what a captured projection of up to n_max particles at level
+// J needs (the worst-case record count, 8 per particle), so the capture itself allocates nothing.
+extern "C" int hermes_project_reserve(hermes_ctx* ctx, int J, int64_t n_max) {
+    HERMES_REQUIRE(ctx, "null context");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_project_reserve: the stream is being captured");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_project_reserve: J=%d outside 3..10", J);
+    HERMES_REQUIRE(n_max >= 0 && n_max < (int64_t)1 << 28, "hermes_project_reserve: n_max out of range");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int L = 1 << J;
+    const int lgtx = ilog2(L < MAX_TX ? L : MAX_TX), lgty = ilog2(L < MAX_TY ? L : MAX_TY),
+              lgtz = ilog2(L < MAX_TZ ? L : MAX_TZ);
+    const int ntiles = 1 << (3 * J - lgtx - lgty - lgtz);
+    const int nscan = (ntiles + SCAN_THREADS * SCAN_ITEMS - 1) / (SCAN_THREADS * SCAN_ITEMS);
+    const size_t tile_bytes = (size_t)ntiles * 8 + (size_t)ntiles * 4 * 3 + 64 + (size_t)(nscan + 1) * 8 + 64;
+    HERMES_REQUIRE(hermes_workspace(ctx, WS_PROJ_TILE, tile_bytes), "%s", hermes_last_error());
+    HERMES_REQUIRE(hermes_workspace(ctx, WS_PROJ_REC, (size_t)(n_max > 0 ? 8 * n_max : 1) * sizeof(ProjRec)), "%s",
+                   hermes_last_error());
     return 0;
 }

@@ -317,6 +317,17 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce_radial(int L, const
     }
 }
 
+// out[d] = factor[d] * (sum over blocks of channel ch_of[d]), fixed order.
+__global__ void k_plan_out(const double* __restrict__ partial, int nblocks, int nch, const int* __restrict__ ch_of,
+                           const double* __restrict__ factor, int n_desc, double* __restrict__ out) {
+    for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < n_desc; d += gridDim.x * blockDim.x) {
+        const int c = ch_of[d];
+        double s = 0.0;
+        for (int i = 0; i < nblocks; ++i) s += partial[(int64_t)i * nch + c];
+        out[d] = factor[d] * s;
+    }
+}
+
 __global__ void k_sum_partials(const double* __restrict__ partial, int nblocks, int nd, double* __restrict__ out) {
     // one thread per output; fixed order -> deterministic
     for (int b = blockIdx.x * blockDim.x + threadIdx.x; b < nd; b += gridDim.x * blockDim.x) {
@@ -616,11 +627,18 @@ int hermes_spectrum_mul(hermes_ctx* ctx, int J, int half, int dbl, void* spec, c
     return 0;
 }
 
-// The radial path when every descriptor is a thin shell or an m = 0 shell
-// multipole of order <= 10 (FP32) or <= 4 (FP64).  Returns false (rc untouched)
-// when it does not apply; true with rc = the status otherwise.
-static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
-                               const hermes_window_desc* descs, int n_desc, int nblocks, double* out_host, int& rc) {
+// Radius-grouped channels of a descriptor list: unique (radius, order) pairs, each descriptor a
+// channel times a constant.  False when some descriptor is not a thin shell or an m = 0 multipole
+// of even order <= 10 (FP32) or <= 4 (FP64), or the channels do not fit the kernel's LDS.
+struct RadialChannels {
+    int LM = 0, nu = 0, nch = 0;
+    std::vector<double> radii;   // R / L per radius
+    std::vector<int> chan;       // nu x (LM + 1): channel or -1
+    std::vector<int> ch_of;      // per descriptor
+    std::vector<double> factor;  // per descriptor
+};
+
+static bool radial_channels(int J, int dbl, const hermes_window_desc* descs, int n_desc, RadialChannels& rc) {
     int lmax = 0;
     for (int b = 0; b < n_desc; ++b) {
         const hermes_window_desc& d = descs[b];
@@ -628,34 +646,64 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
         if (d.type != HERMES_WIN_SHELL_MULTIPOLE || d.m != 0 || d.l > (dbl ? 4 : 10)) return false;
         lmax = d.l > lmax ? d.l : lmax;
     }
-    const int LM = lmax == 0 ? 0 : (lmax <= 4 ? 4 : 10);
-    // channels: unique (radius, order); each descriptor = its channel times a constant
-    std::vector<double> radii;
-    std::vector<int> chan, ch_of(n_desc);
-    std::vector<double> factor(n_desc);
-    int nch = 0;
+    rc.LM = lmax == 0 ? 0 : (lmax <= 4 ? 4 : 10);
     const int L = 1 << J;
+    rc.ch_of.resize(n_desc);
+    rc.factor.resize(n_desc);
     for (int b = 0; b < n_desc; ++b) {
         const hermes_window_desc& d = descs[b];
         const double R = d.a0 / L;
         size_t u = 0;
-        while (u < radii.size() && radii[u] != R) ++u;
-        if (u == radii.size()) {
-            radii.push_back(R);
-            chan.insert(chan.end(), LM + 1, -1);
+        while (u < rc.radii.size() && rc.radii[u] != R) ++u;
+        if (u == rc.radii.size()) {
+            rc.radii.push_back(R);
+            rc.chan.insert(rc.chan.end(), rc.LM + 1, -1);
         }
         const int l = d.type == HERMES_WIN_SHELL ? 0 : d.l;
-        int& c = chan[u * (LM + 1) + l];
-        if (c < 0) c = nch++;
-        ch_of[b] = c;
-        factor[b] = d.type == HERMES_WIN_SHELL ? 1.0 : hermes_ylm_cnorm(d.l, 0);
+        int& c = rc.chan[u * (rc.LM + 1) + l];
+        if (c < 0) c = rc.nch++;
+        rc.ch_of[b] = c;
+        rc.factor[b] = d.type == HERMES_WIN_SHELL ? 1.0 : hermes_ylm_cnorm(d.l, 0);
     }
-    const int nu = (int)radii.size();
+    rc.nu = (int)rc.radii.size();
+    return sizeof(double) * (RED_THREADS / 64) * rc.nch <= 64 * 1024;
+}
+
+static hipError_t launch_radial(hipStream_t st, int J, int dbl, int LM, int nblocks, const void* F1, const void* F2,
+                                const double* P, const double* d_radii, int nu, const int* d_chan, int nch,
+                                double* partial) {
+    const int L = 1 << J;
     const size_t shmem = sizeof(double) * (RED_THREADS / 64) * nch;
-    if (shmem > 64 * 1024) return false;
+#define HERMES_RADIAL(LMV, T, C)                                                                      \
+    k_pair_reduce_radial<LMV, T, C><<<nblocks, RED_THREADS, shmem, st>>>(L, (const C*)F1, (const C*)F2, P, d_radii, \
+                                                                        nu, d_chan, nch, partial)
+    if (dbl) {
+        if (LM == 0) HERMES_RADIAL(0, double, double2);
+        else HERMES_RADIAL(4, double, double2);
+    } else {
+        if (LM == 0) HERMES_RADIAL(0, float, float2);
+        else if (LM == 4) HERMES_RADIAL(4, float, float2);
+        else HERMES_RADIAL(10, float, float2);
+    }
+#undef HERMES_RADIAL
+    return hipGetLastError();
+}
+
+static int pair_blocks(int J) {
+    const int Lh = (1 << J) / 2 + 1;
+    return (int)hermes_blocks((int64_t)Lh * Lh * Lh, RED_THREADS, 2048);
+}
+
+// The radial path of hermes_pair_reduce.  Returns false (rc untouched) when it does not apply;
+// true with rc = the status otherwise.
+static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
+                               const hermes_window_desc* descs, int n_desc, int nblocks, double* out_host, int& rc) {
+    RadialChannels ch;
+    if (!radial_channels(J, dbl, descs, n_desc, ch)) return false;
     rc = 0;
+    const int nu = ch.nu, nch = ch.nch;
     const size_t off_chan = sizeof(double) * nu;
-    const size_t off_part = ((off_chan + sizeof(int) * chan.size() + 255) / 256) * 256;
+    const size_t off_part = ((off_chan + sizeof(int) * ch.chan.size() + 255) / 256) * 256;
     const size_t need = off_part + sizeof(double) * ((size_t)nblocks * nch + nch);
     char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);
     if (!ws) {
@@ -671,31 +719,103 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
         rc = 1;
         return true;
     };
-    hipError_t e = hipMemcpyAsync(d_radii, radii.data(), sizeof(double) * nu, hipMemcpyHostToDevice, ctx->stream);
+    hipError_t e = hipMemcpyAsync(d_radii, ch.radii.data(), sizeof(double) * nu, hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return fail(e);
-    e = hipMemcpyAsync(d_chan, chan.data(), sizeof(int) * chan.size(), hipMemcpyHostToDevice, ctx->stream);
+    e = hipMemcpyAsync(d_chan, ch.chan.data(), sizeof(int) * ch.chan.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return fail(e);
-#define HERMES_RADIAL(LMV, T, C)                                                                      \
-    k_pair_reduce_radial<LMV, T, C><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(                    \
-        L, (const C*)F1, (const C*)F2, P, d_radii, nu, d_chan, nch, partial)
-    if (dbl) {
-        if (LM == 0) HERMES_RADIAL(0, double, double2);
-        else HERMES_RADIAL(4, double, double2);
-    } else {
-        if (LM == 0) HERMES_RADIAL(0, float, float2);
-        else if (LM == 4) HERMES_RADIAL(4, float, float2);
-        else HERMES_RADIAL(10, float, float2);
-    }
-#undef HERMES_RADIAL
-    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    if ((e = launch_radial(ctx->stream, J, dbl, ch.LM, nblocks, F1, F2, P, d_radii, nu, d_chan, nch, partial)) !=
+        hipSuccess)
+        return fail(e);
     k_sum_partials<<<hermes_blocks(nch, 64), 64, 0, ctx->stream>>>(partial, nblocks, nch, d_out);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
-    std::vector<double> ch(nch);
-    e = hipMemcpyAsync(ch.data(), d_out, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream);
+    std::vector<double> chs(nch);
+    e = hipMemcpyAsync(chs.data(), d_out, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream);
     if (e != hipSuccess) return fail(e);
     if ((e = hipStreamSynchronize(ctx->stream)) != hipSuccess) return fail(e);
-    for (int b = 0; b < n_desc; ++b) out_host[b] = factor[b] * ch[ch_of[b]];
+    for (int b = 0; b < n_desc; ++b) out_host[b] = ch.factor[b] * chs[ch.ch_of[b]];
     return true;
+}
+
+struct hermes_pair_plan {
+    int J = 0, dbl = 0, LM = 0, nu = 0, nch = 0, n_desc = 0, nblocks = 0;
+    char* mem = nullptr;  // radii | chan | ch_of | factor | partial
+    double* radii = nullptr;
+    int* chan = nullptr;
+    int* ch_of = nullptr;
+    double* factor = nullptr;
+    double* partial = nullptr;
+};
+
+int hermes_pair_plan_destroy(hermes_pair_plan* plan) {
+    if (!plan) return 0;
+    if (plan->mem) (void)hipFree(plan->mem);
+    delete plan;
+    return 0;
+}
+
+int hermes_pair_plan_create(hermes_ctx* ctx, int J, int dbl, const hermes_window_desc* descs, int n_desc,
+                            hermes_pair_plan** out) {
+    HERMES_REQUIRE(ctx && descs && out, "hermes_pair_plan_create: null argument");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_pair_plan_create: the stream is being captured");
+    HERMES_REQUIRE(J >= 1 && J <= 10, "J=%d unsupported", J);
+    HERMES_REQUIRE(n_desc >= 1 && n_desc <= 4096, "hermes_pair_plan_create: n_desc=%d out of range", n_desc);
+    *out = nullptr;
+    for (int b = 0; b < n_desc; ++b) {
+        if (int rc = check_desc(descs[b])) return rc;
+        HERMES_REQUIRE(descs[b].type != HERMES_WIN_SHELL_MULTIPOLE || (descs[b].m == 0 && descs[b].l % 2 == 0),
+                       "pair reduction supports only m = 0, even-l multipole windows");
+    }
+    RadialChannels ch;
+    HERMES_REQUIRE(radial_channels(J, dbl, descs, n_desc, ch),
+                   "hermes_pair_plan_create: plans take thin shells and even-l m = 0 shell multipoles "
+                   "(l <= 10 in FP32, l <= 4 in FP64)");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    hermes_pair_plan* p = new hermes_pair_plan();
+    p->J = J;
+    p->dbl = dbl;
+    p->LM = ch.LM;
+    p->nu = ch.nu;
+    p->nch = ch.nch;
+    p->n_desc = n_desc;
+    p->nblocks = pair_blocks(J);
+    auto al = [](size_t b) { return (b + 255) / 256 * 256; };
+    const size_t o_chan = al(sizeof(double) * ch.nu), o_chof = o_chan + al(sizeof(int) * ch.chan.size()),
+                 o_fac = o_chof + al(sizeof(int) * n_desc), o_part = o_fac + al(sizeof(double) * n_desc),
+                 total = o_part + sizeof(double) * (size_t)p->nblocks * ch.nch;
+    if (hipMalloc(&p->mem, total) != hipSuccess) {
+        delete p;
+        hermes_set_error("hermes_pair_plan_create: hipMalloc of %zu bytes failed", total);
+        return 1;
+    }
+    p->radii = (double*)p->mem;
+    p->chan = (int*)(p->mem + o_chan);
+    p->ch_of = (int*)(p->mem + o_chof);
+    p->factor = (double*)(p->mem + o_fac);
+    p->partial = (double*)(p->mem + o_part);
+    hipError_t e = hipMemcpy(p->radii, ch.radii.data(), sizeof(double) * ch.nu, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->chan, ch.chan.data(), sizeof(int) * ch.chan.size(), hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->ch_of, ch.ch_of.data(), sizeof(int) * n_desc, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(p->factor, ch.factor.data(), sizeof(double) * n_desc, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        hermes_pair_plan_destroy(p);
+        hermes_set_error("hermes_pair_plan_create: upload failed: %s", hipGetErrorString(e));
+        return 1;
+    }
+    *out = p;
+    return 0;
+}
+
+int hermes_pair_reduce_planned(hermes_ctx* ctx, const hermes_pair_plan* plan, const void* F1, const void* F2,
+                               const double* P, double* out_dev) {
+    HERMES_REQUIRE(ctx && plan && F1 && F2 && P && out_dev, "hermes_pair_reduce_planned: null argument");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    HERMES_CHECK_HIP(launch_radial(ctx->stream, plan->J, plan->dbl, plan->LM, plan->nblocks, F1, F2, P, plan->radii,
+                                   plan->nu, plan->chan, plan->nch, plan->partial));
+    k_plan_out<<<hermes_blocks(plan->n_desc, 64), 64, 0, ctx->stream>>>(plan->partial, plan->nblocks, plan->nch,
+                                                                       plan->ch_of, plan->factor, plan->n_desc,
+                                                                       out_dev);
+    HERMES_LAUNCH_CHECK();
+    return 0;
 }
 
 int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,

@@ -257,6 +257,29 @@ def multipole_gram_f32(d1, U, Q, lam, V=None, v_conj=False):
     return out
 
 
+def multipole_gram_f32_dev(d1, U, Q, lam, out_dev, V=None, v_conj=False):
+    """Raw 16 x 16 Gram block into out_dev (256 doubles on the device, no host sync); see
+    hermes_multipole_gram_f32_dev."""
+    L = d1.shape[0]
+    n_u = U.shape[0]
+    n_v = V.shape[0] if V is not None else n_u
+    if out_dev.dtype != torch.float64 or out_dev.numel() < 256 or not out_dev.is_contiguous():
+        raise ValueError("out_dev must be a contiguous float64 tensor of at least 256 elements")
+    q = _np_f64(Q)
+    lm = _np_f64(lam)
+    _lib.call("hermes_multipole_gram_f32_dev", _lib.context(), _J_of(L), ptr(d1), ptr(U), n_u, ptr(V), n_v,
+              1 if v_conj else 0, q.ctypes.data_as(_DP), lm.ctypes.data_as(_DP), ptr(out_dev))
+    return out_dev
+
+
+def gram_block_to_matrix(block, n_u, n_v=None):
+    """(n_u, n_v) Gram sums from a raw block of hermes_multipole_gram_f32_dev (auto when n_v is None)."""
+    h = np.asarray(block, dtype=np.float64).reshape(16, 16)
+    if n_v is None:
+        return h[:n_u, :n_u] + h[:n_u, :n_u].T
+    return h[:n_u, :n_v].copy()
+
+
 def profile_gram(enable):
     """Start (True: reset) or stop timing the Gram kernel; returns (total_ms, launches) so far."""
     ms, n = ctypes.c_double(0.0), ctypes.c_int64(0)
@@ -268,6 +291,91 @@ def mfma_selftest():
     out = np.zeros(1, dtype=np.float64)
     _lib.call("hermes_mfma_selftest", _lib.context(), out.ctypes.data_as(_DP))
     return float(out[0])
+
+
+# ---------------------------------------------------------------- graphs and batch helpers
+def capture_begin():
+    """Start capturing the library stream (torch's current stream; must not be the null stream)."""
+    _lib.call("hermes_capture_begin", _lib.context())
+
+
+def capture_end():
+    """End the capture; returns an instantiated graph handle for graph_launch / graph_destroy."""
+    h = ctypes.c_void_p()
+    _lib.call("hermes_capture_end", _lib.context(), ctypes.byref(h))
+    return h
+
+
+def graph_launch(exec_handle):
+    _lib.call("hermes_graph_launch", _lib.context(), exec_handle)
+
+
+def graph_destroy(exec_handle):
+    if exec_handle:
+        _lib.call("hermes_graph_destroy", exec_handle)
+
+
+def clear_errors():
+    _lib.call("hermes_clear_errors", _lib.context())
+
+
+def read_errors():
+    v = ctypes.c_int(0)
+    _lib.call("hermes_read_errors", _lib.context(), ctypes.byref(v))
+    return int(v.value)
+
+
+def project_reserve(J, n_max):
+    _lib.call("hermes_project_reserve", _lib.context(), int(J), int(n_max))
+
+
+def delta_f32(eps32, a, b, d_out=None, dc_out=None):
+    """d = a * eps - b (FP32) into d_out (real) and/or dc_out (complex64 real part)."""
+    if d_out is None and dc_out is None:
+        raise ValueError("delta_f32 needs an output")
+    for t, dt in ((d_out, torch.float32), (dc_out, torch.complex64)):
+        if t is not None and (t.dtype != dt or t.numel() != eps32.numel() or not t.is_contiguous()):
+            raise ValueError("delta_f32: output shape/dtype mismatch")
+    _lib.call("hermes_delta_f32", _lib.context(), int(eps32.numel()), ptr(eps32), float(a), float(b), ptr(d_out),
+              ptr(dc_out))
+
+
+def half_spectrum_f32(full, half):
+    """half (L, L, L/2+1) <- full (L, L, L) complex64 transform of a real field."""
+    L = full.shape[-1]
+    if full.dtype != torch.complex64 or half.dtype != torch.complex64 or half.shape[-1] != L // 2 + 1:
+        raise ValueError("half_spectrum_f32: expects complex64 (L,L,L) -> (L,L,L/2+1)")
+    _lib.call("hermes_half_spectrum_f32", _lib.context(), _J_of(L), ptr(full), ptr(half))
+    return half
+
+
+class PairPlan:
+    """A fused pair-reduction plan (hermes_pair_plan): window tables on the device, results on the device."""
+
+    def __init__(self, J, descs, dbl=False):
+        self.n = len(descs)
+        arr = (_lib.WindowDesc * self.n)(*descs)
+        h = ctypes.c_void_p()
+        _lib.call("hermes_pair_plan_create", _lib.context(), int(J), 1 if dbl else 0, arr, self.n, ctypes.byref(h))
+        self.handle = h
+        self.J = int(J)
+
+    def reduce(self, F1, F2, P, out_dev):
+        if out_dev.dtype != torch.float64 or out_dev.numel() < self.n:
+            raise ValueError("out_dev must hold n_desc float64 values")
+        _lib.call("hermes_pair_reduce_planned", _lib.context(), self.handle, ptr(F1), ptr(F2), ptr(P), ptr(out_dev))
+        return out_dev
+
+    def close(self):
+        if self.handle:
+            _lib.load().hermes_pair_plan_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 # ---------------------------------------------------------------- Monte Carlo

@@ -131,6 +131,8 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     tm.stop(t)
     n_l = l_max - l_min + 1
     result = np.zeros((r12.size, r13.size, n_l), dtype=np.float64)
+    if auto and r12.size <= 8:
+        return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm)
     blocks12 = _shell_blocks(r12.size)
     blocks13 = _shell_blocks(r13.size)
     U = torch.empty((r12.size, L, L, L), dtype=torch.complex64, device=d1.device)
@@ -164,4 +166,31 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
             tm.stop(t)
             v_re.append(4.0 * np.pi * G / N)
         result[:, :, li] = combine_m_terms(v_re, l)
+    return result
+
+
+def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm):
+    """Auto case with S <= 8 shells: the Gram's MFMA tile is 16 shells wide, so 16 // S (l, m) terms share
+    one contraction and only their diagonal S x S blocks are kept."""
+    L = geo.L
+    N = L ** 3
+    S = r.size
+    pack = 16 // S
+    lm = [(l, m) for l in range(l_min, l_max + 1) for m in range(l + 1)]
+    U = torch.empty((pack * S, L, L, L), dtype=torch.complex64, device=d1.device)
+    v_re = {}
+    for g0 in range(0, len(lm), pack):
+        group = lm[g0:g0 + pack]
+        t = tm.start("window_ifft")
+        for j, (l, m) in enumerate(group):
+            kernels.multipole_window_ifft_f32(H, geo.P, r, l, m, 1.0 / N, out=U[j * S:(j + 1) * S])
+        tm.stop(t)
+        t = tm.start("gram")
+        full = kernels.multipole_gram_f32(d1, U[:len(group) * S], geo.Q, geo.lam)
+        tm.stop(t)
+        for j, (l, m) in enumerate(group):
+            v_re[(l, m)] = 4.0 * np.pi * ((-1.0) ** (l + m)) * full[j * S:(j + 1) * S, j * S:(j + 1) * S] / N
+    result = np.zeros((S, S, l_max - l_min + 1), dtype=np.float64)
+    for li, l in enumerate(range(l_min, l_max + 1)):
+        result[:, :, li] = combine_m_terms([v_re[(l, m)] for m in range(l + 1)], l)
     return result

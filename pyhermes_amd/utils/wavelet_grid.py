@@ -63,13 +63,13 @@ def fourier_power_spectrum(values, k_min, k_max, k_count, phi_resolution):
     """|sum_i phi_i dx exp(-2 pi i k x_i)|^2 at k = k_min + j (k_max - k_min) / k_count, j = 0..k_count.
 
     A dense (k_count + 1) x len(phi) DFT on the host (about 30 ms at k_count = 512), so results are
-    kept per (phi samples, k range, resolution); callers get a read-only array.
+    kept per (phi samples, k range, resolution); callers get their own copy.
     """
     values = np.ascontiguousarray(values, dtype=np.float64)
     key = (hash(values.tobytes()), values.shape[0], float(k_min), float(k_max), int(k_count), int(phi_resolution))
     hit = _POWER_CACHE.get(key)
     if hit is not None:
-        return hit
+        return hit.copy()
     if len(_POWER_CACHE) > 64:
         _POWER_CACHE.clear()
     n = values.shape[0]
@@ -81,9 +81,8 @@ def fourier_power_spectrum(values, k_min, k_max, k_count, phi_resolution):
     re = np.sum(values * dx * np.cos(-2 * np.pi * kg * xg), axis=1)
     im = np.sum(values * dx * np.sin(-2 * np.pi * kg * xg), axis=1)
     out = re ** 2 + im ** 2
-    out.setflags(write=False)
     _POWER_CACHE[key] = out
-    return out
+    return out.copy()
 
 
 def cal_gamma(phi_array, phi_support, phi_resolution):

@@ -90,3 +90,7 @@ def test_ddd_paths_match_oracle(phi, L, lmax):
     d32b = d32.clone()
     cross = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32b, radii, radii, 0, lmax)
     assert np.linalg.norm(cross - batched) <= 1e-4 * np.linalg.norm(batched)
+    # more than 8 shells take the unpacked auto contraction (one (l, m) per Gram): same numbers
+    radii9 = np.concatenate([radii, [1.5, 2.5, 3.0, 4.0, 4.5, 6.0]])
+    wide = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii9, radii9, 0, lmax)
+    assert np.linalg.norm(wide[:3, :3] - batched) <= 1e-5 * np.linalg.norm(batched)
