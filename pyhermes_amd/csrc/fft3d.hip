@@ -97,12 +97,28 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, cons
     const int64_t outer = b / n_mid;
     float2* base = data + outer * outer_stride + (int64_t)mid * mid_stride + (int64_t)zt * TW;
     load_twiddles<LOG2L>(tw, twg, tid, NT);
-#pragma unroll 4
-    for (int i = tid; i < TW * L; i += NT) lds[i] = base[(i / TW) * axis_stride + (i % TW)];
+    // every load of the thread is issued before the first LDS store (TW * L / NT of them), so a
+    // workgroup keeps its whole tile in flight: the pass is bound by HBM latency otherwise
+    constexpr int PER = (TW * L + NT - 1) / NT;
+    constexpr bool EVEN = TW * L % NT == 0;
+    float2 v[PER];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (EVEN || i < TW * L) v[k] = base[(i / TW) * axis_stride + (i % TW)];
+    }
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (EVEN || i < TW * L) lds[i] = v[k];
+    }
     __syncthreads();
     F::run(lds, tw, tid);
-#pragma unroll 4
-    for (int i = tid; i < TW * L; i += NT) base[(i / TW) * axis_stride + (i % TW)] = lds[i];
+#pragma unroll
+    for (int k = 0; k < PER; ++k) {
+        const int i = tid + k * NT;
+        if (EVEN || i < TW * L) base[(i / TW) * axis_stride + (i % TW)] = lds[i];
+    }
 }
 
 template <int LOG2L>

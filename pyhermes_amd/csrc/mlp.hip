@@ -154,11 +154,26 @@ __global__ __launch_bounds__(THREADS) void k_mlp_grad(const __hip_bfloat16* __re
     }
 }
 
-// Sum tile partials in fixed order; mean gradient and loss -> grad_out (NPARAM + 1).
-__global__ void k_mlp_reduce(const float* __restrict__ partial, int ntiles, int n, float* __restrict__ grad_out) {
+// Tile partials are summed in fixed order in two stages: groups of RED_GROUP consecutive tiles
+// (one thread per (group, parameter)), then the group sums; mean gradient and loss -> grad_out.
+constexpr int RED_GROUP = 32;
+
+__global__ void k_mlp_reduce_groups(const float* __restrict__ partial, int ntiles, double* __restrict__ gsum) {
+    const int ngroups = (ntiles + RED_GROUP - 1) / RED_GROUP;
+    const int64_t total = (int64_t)ngroups * (NPARAM + 1);
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int g = (int)(q / (NPARAM + 1)), p = (int)(q % (NPARAM + 1));
+        const int t1 = min(ntiles, (g + 1) * RED_GROUP);
+        double a = 0.0;
+        for (int t = g * RED_GROUP; t < t1; ++t) a += partial[(int64_t)t * (NPARAM + 1) + p];
+        gsum[q] = a;
+    }
+}
+
+__global__ void k_mlp_reduce(const double* __restrict__ gsum, int ngroups, int n, float* __restrict__ grad_out) {
     for (int p = blockIdx.x * blockDim.x + threadIdx.x; p <= NPARAM; p += gridDim.x * blockDim.x) {
         double a = 0.0;
-        for (int t = 0; t < ntiles; ++t) a += partial[(int64_t)t * (NPARAM + 1) + p];
+        for (int g = 0; g < ngroups; ++g) a += gsum[(int64_t)g * (NPARAM + 1) + p];
         grad_out[p] = (float)(a / n);
     }
 }
@@ -212,11 +227,18 @@ int hermes_mlp_grad(hermes_ctx* ctx, const void* wb, const float* x, const float
     HERMES_REQUIRE(ctx && wb && x && y && grad_out && n >= 1 && n <= (1 << 18), "hermes_mlp_grad: invalid argument (batch must be 1..262144)");
     HERMES_CHECK_HIP(hipSetDevice(ctx->device));
     const int ntiles = (n + TILE - 1) / TILE;
-    float* partial = (float*)hermes_workspace(ctx, WS_MLP, sizeof(float) * (size_t)ntiles * (NPARAM + 1));
-    HERMES_REQUIRE(partial, "%s", hermes_last_error());
+    const int ngroups = (ntiles + RED_GROUP - 1) / RED_GROUP;
+    const size_t part_bytes = ((sizeof(float) * (size_t)ntiles * (NPARAM + 1) + 255) / 256) * 256;
+    char* ws = (char*)hermes_workspace(ctx, WS_MLP, part_bytes + sizeof(double) * (size_t)ngroups * (NPARAM + 1));
+    HERMES_REQUIRE(ws, "%s", hermes_last_error());
+    float* partial = (float*)ws;
+    double* gsum = (double*)(ws + part_bytes);
     k_mlp_grad<<<ntiles, THREADS, 0, ctx->stream>>>((const __hip_bfloat16*)wb, x, y, n, clip, partial);
     HERMES_LAUNCH_CHECK();
-    k_mlp_reduce<<<hermes_blocks(NPARAM + 1, 256), 256, 0, ctx->stream>>>(partial, ntiles, n, grad_out);
+    k_mlp_reduce_groups<<<hermes_blocks((int64_t)ngroups * (NPARAM + 1), 256), 256, 0, ctx->stream>>>(partial, ntiles,
+                                                                                                     gsum);
+    HERMES_LAUNCH_CHECK();
+    k_mlp_reduce<<<hermes_blocks(NPARAM + 1, 256), 256, 0, ctx->stream>>>(gsum, ngroups, n, grad_out);
     HERMES_LAUNCH_CHECK();
     return 0;
 }

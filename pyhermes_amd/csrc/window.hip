@@ -317,7 +317,27 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce_radial(int L, const
     }
 }
 
-// out[d] = factor[d] * (sum over blocks of channel ch_of[d]), fixed order.
+// First stage of the fixed-order sum of block partials (nblocks rows of nch): groups of
+// PART_GROUP consecutive rows, one thread per (group, column); the kernels below finish over
+// the (nblocks / PART_GROUP) group rows.
+constexpr int PART_GROUP = 64;
+
+__global__ void k_sum_partial_groups(const double* __restrict__ partial, int nblocks, int nch,
+                                     double* __restrict__ gsum) {
+    const int ngroups = (nblocks + PART_GROUP - 1) / PART_GROUP;
+    const int64_t total = (int64_t)ngroups * nch;
+    for (int64_t q = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; q < total; q += (int64_t)gridDim.x * blockDim.x) {
+        const int g = (int)(q / nch), c = (int)(q % nch);
+        const int r1 = min(nblocks, (g + 1) * PART_GROUP);
+        double s = 0.0;
+        for (int r = g * PART_GROUP; r < r1; ++r) s += partial[(int64_t)r * nch + c];
+        gsum[q] = s;
+    }
+}
+
+static inline int part_groups(int nblocks) { return (nblocks + PART_GROUP - 1) / PART_GROUP; }
+
+// out[d] = factor[d] * (sum over rows of channel ch_of[d]), fixed order.
 __global__ void k_plan_out(const double* __restrict__ partial, int nblocks, int nch, const int* __restrict__ ch_of,
                            const double* __restrict__ factor, int n_desc, double* __restrict__ out) {
     for (int d = blockIdx.x * blockDim.x + threadIdx.x; d < n_desc; d += gridDim.x * blockDim.x) {
@@ -704,7 +724,8 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
     const int nu = ch.nu, nch = ch.nch;
     const size_t off_chan = sizeof(double) * nu;
     const size_t off_part = ((off_chan + sizeof(int) * ch.chan.size() + 255) / 256) * 256;
-    const size_t need = off_part + sizeof(double) * ((size_t)nblocks * nch + nch);
+    const int ng = part_groups(nblocks);
+    const size_t need = off_part + sizeof(double) * ((size_t)nblocks * nch + (size_t)ng * nch + nch);
     char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);
     if (!ws) {
         rc = 1;
@@ -713,7 +734,8 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
     double* d_radii = (double*)ws;
     int* d_chan = (int*)(ws + off_chan);
     double* partial = (double*)(ws + off_part);
-    double* d_out = partial + (size_t)nblocks * nch;
+    double* gsum = partial + (size_t)nblocks * nch;
+    double* d_out = gsum + (size_t)ng * nch;
     auto fail = [&](hipError_t e) {
         hermes_set_error("pair reduction: %s", hipGetErrorString(e));
         rc = 1;
@@ -726,7 +748,9 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
     if ((e = launch_radial(ctx->stream, J, dbl, ch.LM, nblocks, F1, F2, P, d_radii, nu, d_chan, nch, partial)) !=
         hipSuccess)
         return fail(e);
-    k_sum_partials<<<hermes_blocks(nch, 64), 64, 0, ctx->stream>>>(partial, nblocks, nch, d_out);
+    k_sum_partial_groups<<<hermes_blocks((int64_t)ng * nch, 256), 256, 0, ctx->stream>>>(partial, nblocks, nch, gsum);
+    if ((e = hipGetLastError()) != hipSuccess) return fail(e);
+    k_sum_partials<<<hermes_blocks(nch, 64), 64, 0, ctx->stream>>>(gsum, ng, nch, d_out);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
     std::vector<double> chs(nch);
     e = hipMemcpyAsync(chs.data(), d_out, sizeof(double) * nch, hipMemcpyDeviceToHost, ctx->stream);
@@ -738,12 +762,13 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
 
 struct hermes_pair_plan {
     int J = 0, dbl = 0, LM = 0, nu = 0, nch = 0, n_desc = 0, nblocks = 0;
-    char* mem = nullptr;  // radii | chan | ch_of | factor | partial
+    char* mem = nullptr;  // radii | chan | ch_of | factor | partial | group sums
     double* radii = nullptr;
     int* chan = nullptr;
     int* ch_of = nullptr;
     double* factor = nullptr;
     double* partial = nullptr;
+    double* gsum = nullptr;
 };
 
 int hermes_pair_plan_destroy(hermes_pair_plan* plan) {
@@ -781,7 +806,8 @@ int hermes_pair_plan_create(hermes_ctx* ctx, int J, int dbl, const hermes_window
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
     const size_t o_chan = al(sizeof(double) * ch.nu), o_chof = o_chan + al(sizeof(int) * ch.chan.size()),
                  o_fac = o_chof + al(sizeof(int) * n_desc), o_part = o_fac + al(sizeof(double) * n_desc),
-                 total = o_part + sizeof(double) * (size_t)p->nblocks * ch.nch;
+                 o_gs = o_part + al(sizeof(double) * (size_t)p->nblocks * ch.nch),
+                 total = o_gs + sizeof(double) * (size_t)part_groups(p->nblocks) * ch.nch;
     if (hipMalloc(&p->mem, total) != hipSuccess) {
         delete p;
         hermes_set_error("hermes_pair_plan_create: hipMalloc of %zu bytes failed", total);
@@ -792,6 +818,7 @@ int hermes_pair_plan_create(hermes_ctx* ctx, int J, int dbl, const hermes_window
     p->ch_of = (int*)(p->mem + o_chof);
     p->factor = (double*)(p->mem + o_fac);
     p->partial = (double*)(p->mem + o_part);
+    p->gsum = (double*)(p->mem + o_gs);
     hipError_t e = hipMemcpy(p->radii, ch.radii.data(), sizeof(double) * ch.nu, hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->chan, ch.chan.data(), sizeof(int) * ch.chan.size(), hipMemcpyHostToDevice);
     if (e == hipSuccess) e = hipMemcpy(p->ch_of, ch.ch_of.data(), sizeof(int) * n_desc, hipMemcpyHostToDevice);
@@ -811,9 +838,12 @@ int hermes_pair_reduce_planned(hermes_ctx* ctx, const hermes_pair_plan* plan, co
     HERMES_CHECK_HIP(hipSetDevice(ctx->device));
     HERMES_CHECK_HIP(launch_radial(ctx->stream, plan->J, plan->dbl, plan->LM, plan->nblocks, F1, F2, P, plan->radii,
                                    plan->nu, plan->chan, plan->nch, plan->partial));
-    k_plan_out<<<hermes_blocks(plan->n_desc, 64), 64, 0, ctx->stream>>>(plan->partial, plan->nblocks, plan->nch,
-                                                                       plan->ch_of, plan->factor, plan->n_desc,
-                                                                       out_dev);
+    const int ng = part_groups(plan->nblocks);
+    k_sum_partial_groups<<<hermes_blocks((int64_t)ng * plan->nch, 256), 256, 0, ctx->stream>>>(
+        plan->partial, plan->nblocks, plan->nch, plan->gsum);
+    HERMES_LAUNCH_CHECK();
+    k_plan_out<<<hermes_blocks(plan->n_desc, 64), 64, 0, ctx->stream>>>(plan->gsum, ng, plan->nch, plan->ch_of,
+                                                                       plan->factor, plan->n_desc, out_dev);
     HERMES_LAUNCH_CHECK();
     return 0;
 }
@@ -841,14 +871,16 @@ int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const vo
         int rc = -1;
         if (pair_reduce_radial(ctx, J, dbl, F1, F2, P, descs, n_desc, nblocks, out_host, rc)) return rc;
     }
+    const int ng = part_groups(nblocks);
     size_t need = sizeof(hermes_window_desc) * n_desc + sizeof(float) * n_desc + 64 +
-                  sizeof(double) * ((size_t)nblocks * n_desc + n_desc);
+                  sizeof(double) * ((size_t)nblocks * n_desc + (size_t)ng * n_desc + n_desc);
     char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
     hermes_window_desc* d_desc = (hermes_window_desc*)ws;
     float* d_cn = (float*)(ws + sizeof(hermes_window_desc) * n_desc);
     double* partial = (double*)(ws + ((sizeof(hermes_window_desc) * n_desc + sizeof(float) * n_desc + 63) / 64) * 64);
-    double* d_out = partial + (size_t)nblocks * n_desc;
+    double* gsum = partial + (size_t)nblocks * n_desc;
+    double* d_out = gsum + (size_t)ng * n_desc;
     std::vector<float> cn(n_desc);
     for (int b = 0; b < n_desc; ++b)
         cn[b] = descs[b].type == HERMES_WIN_SHELL_MULTIPOLE ? (float)hermes_ylm_cnorm(descs[b].l, 0) : 0.0f;
@@ -863,7 +895,10 @@ int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const vo
         k_pair_reduce<float2, false><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
             L, (const float2*)F1, (const float2*)F2, P, d_desc, d_cn, n_desc, partial);
     HERMES_LAUNCH_CHECK();
-    k_sum_partials<<<hermes_blocks(n_desc, 64), 64, 0, ctx->stream>>>(partial, nblocks, n_desc, d_out);
+    k_sum_partial_groups<<<hermes_blocks((int64_t)ng * n_desc, 256), 256, 0, ctx->stream>>>(partial, nblocks, n_desc,
+                                                                                            gsum);
+    HERMES_LAUNCH_CHECK();
+    k_sum_partials<<<hermes_blocks(n_desc, 64), 64, 0, ctx->stream>>>(gsum, ng, n_desc, d_out);
     HERMES_LAUNCH_CHECK();
     HERMES_CHECK_HIP(hipMemcpyAsync(out_host, d_out, sizeof(double) * n_desc, hipMemcpyDeviceToHost, ctx->stream));
     HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));

@@ -135,6 +135,38 @@ def _multipole_oracle(phi, field, samples, fieldname):
 
 
 @pytest.mark.parametrize("backend,rtol", [("gpu_fp64", 1e-9), ("gpu", 1e-4)])
+def test_corr3pcf_multipole_random_catalogue(phi, backend, rtol):
+    """An explicit random catalogue: delta = D - R, rrr_l from R (the full kernel on the random field,
+    corr3pcf_multipole.py:1363-1395) and zeta_l from the mixing-matrix solve (:1554-1589)."""
+    from pyhermes_amd.mocks import uniform_catalogue
+    from pyhermes_amd.theory import Corr_3PCF_Multipole
+    print(f"Corr_3PCF_Multipole random catalogue backend={backend}", flush=True)
+    box, J = 160.0, 4
+    L = 1 << J
+    pos, w = _catalogue(8000, box, 14)
+    field = _project(pos, w, J, box)
+    rpos = uniform_catalogue(30000, box, 15)
+    rfield = _project(rpos, np.ones(rpos.shape[0]), J, box)
+    res = Corr_3PCF_Multipole({"Corr_3PCF_Multipole": {
+        "sfc_field": field, "random": rfield, "l_min": 0, "l_max": 2,
+        "products": ["zeta_l", "delta_ddd_l", "rrr_l"], "summation_backend": backend, "fout_path": "",
+        "sampling": {"mode": "grid", "r12": {"values": [30.0, 50.0]}, "r13": {"values": [30.0, 50.0]}},
+        "binning_window12": {"type": "shell", "mapping": {"R": "r12"}},
+        "binning_window13": {"type": "shell", "mapping": {"R": "r13"}}}}).run(save_result=False)
+    D, R = field.epsilon, rfield.epsilon
+    scale = (L / box) ** 9
+    samples = list(zip(res.sample_params["r12"], res.sample_params["r13"]))
+    for k, (r12, r13) in enumerate(samples):
+        a, b = r12 * L / box, r13 * L / box
+        ref_delta = omul.calc_ddd_multipole(D - R, D - R, D - R, phi, a, b, 0, 2) * scale
+        ref_rrr = omul.calc_ddd_multipole(R, R, R, phi, a, b, 0, 2) * scale
+        assert np.abs(res.delta_ddd_l[k] - ref_delta).max() <= rtol * np.abs(ref_delta).max(), k
+        assert np.abs(res.rrr_l[k] - ref_rrr).max() <= rtol * np.abs(ref_rrr).max(), k
+        zeta = omul.solve_multipoles_from_ratio(ref_delta, ref_rrr, 2)[0]
+        assert np.abs(res.zeta_l[k] - zeta).max() <= 10 * rtol * np.abs(zeta).max(), k
+
+
+@pytest.mark.parametrize("backend,rtol", [("gpu_fp64", 1e-9), ("gpu", 1e-4)])
 def test_corr3pcf_multipole_task(phi, backend, rtol):
     print(f"Corr_3PCF_Multipole L=16 lmax=2 backend={backend}", flush=True)
     field, res = _multipole_case(phi, backend, ["zeta_l", "ddd_l"])
