@@ -185,6 +185,13 @@ int hermes_multipole_apply_f32(hermes_ctx *ctx, int J, const void *H, const doub
 int hermes_multipole_window_ifft_f32(hermes_ctx *ctx, int J, const void *H, const double *P, const double *radii,
                                      int n_shell, int l, int m, double scale, void *out);
 
+/* The same for n_m orders m (1 or 2, all of the same l) in one pass: the radial factors
+ * j_l(2 pi |k| R_s) are shared among them; out holds n_m * n_shell fields, order i's shell s at
+ * (i * n_shell + s).  No counterpart (the reference builds each (l, m, R) window on its own). */
+int hermes_multipole_window_ifft_multi_f32(hermes_ctx *ctx, int J, const void *H, const double *P,
+                                           const double *radii, int n_shell, int l, const int *ms_host, int n_m,
+                                           double scale, void *out);
+
 /* Deterministic FP64 sum of n real (ncomp = 1) or complex (ncomp = 2) values,
  * stored as float (dbl = 0) or double (dbl = 1); result to host.  Replaces the
  * np.sum in SFCField._derived_field_integral, sfc_field.py:312-324. */

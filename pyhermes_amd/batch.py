@@ -24,7 +24,7 @@ import numpy as np
 import torch
 
 from . import _lib, kernels
-from .multipole import combine_m_terms
+from .multipole import _window_runs, combine_m_terms
 from .utils.wavelet_grid import cal_gamma, fourier_power_spectrum, sample_scaling_function
 
 
@@ -126,9 +126,11 @@ class CatalogueBatch:
         dv["plan"].reduce(H, H, self.P[J], self.pp_out[i])
         S = self.shells.size
         for k, group in enumerate(self.groups):
-            for j, (l, m) in enumerate(group):
-                Uj = self.U[j * S * N:(j + 1) * S * N].view(S, L, L, L)
-                kernels.multipole_window_ifft_f32(H, self.P[J], dv["radii"], l, m, 1.0 / N, out=Uj)
+            pos = 0
+            for l, ms in _window_runs(group):  # orders of one l share the radial factors
+                Uj = self.U[pos * S * N:(pos + len(ms)) * S * N].view(len(ms) * S, L, L, L)
+                kernels.multipole_window_ifft_multi_f32(H, self.P[J], dv["radii"], l, ms, 1.0 / N, Uj)
+                pos += len(ms)
             U = self.U[:len(group) * S * N].view(len(group) * S, L, L, L)
             kernels.multipole_gram_f32_dev(d, U, self.Q, self.lam, self.gram_out[i, k])
 

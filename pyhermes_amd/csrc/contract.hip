@@ -131,6 +131,7 @@ struct GramArgs {
     float lam[27];    // lam_a lam_b lam_c, index (a * 3 + b) * 3 + c
     float lamyz[9];   // lam_b lam_c, index b * 3 + c (auto kernel)
     float gx[9];      // Gamma[l1 * 3 + l2] = sum_a Q[l1][a] lam_a Q[l2][a] (auto kernel)
+    float Qs[9];      // auto kernel: Q[tap][col] sqrt|lam_col|, columns sorted by ascending lam
 };
 
 constexpr int GRAM_WAVES = 4;
@@ -297,6 +298,11 @@ constexpr int AUTO_TILE = AUTO_RY * AUTO_RZ * AUTO_PITCH;     // float2 per buff
 constexpr int AUTO_STAGE = AUTO_RY * AUTO_RZ * 16;            // elements staged per plane
 constexpr int AUTO_SPT = (AUTO_STAGE + 64 * AUTO_WAVES - 1) / (64 * AUTO_WAVES);
 
+// NNEG = number of negative eigenvalues of Gamma; with the columns of Qs sorted by ascending lam and
+// scaled by sqrt|lam|, the filtered slot (b, c) carries sqrt|lam_b lam_c| and its weight's sign is
+// -1 exactly when one of b, c is below NNEG: those slots accumulate into a second pair of MFMA
+// accumulators that is subtracted at each flush.
+template <int NNEG>
 __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __restrict__ d1,
                                                                const float2* __restrict__ U, GramArgs g,
                                                                double* __restrict__ partial) {
@@ -324,12 +330,9 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     const int64_t N = (int64_t)L * L * L;
     const int64_t LL = (int64_t)L * L;
 
-    float Q[9], lamyz[9];
+    float Q[9];
 #pragma unroll
-    for (int i = 0; i < 9; ++i) {
-        Q[i] = g.Q[i];
-        lamyz[i] = g.lamyz[i];
-    }
+    for (int i = 0; i < 9; ++i) Q[i] = g.Qs[i];
     const float g00 = 0.5f * g.gx[0], g11 = 0.5f * g.gx[4], g22 = 0.5f * g.gx[8];
     const float g10 = g.gx[3], g21 = g.gx[7], g20 = g.gx[6];
     const int64_t col = (int64_t)y * L + z;
@@ -394,6 +397,7 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     float dnext = d1[(int64_t)((x0 + 2) & m) * LL + col];
 
     float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
+    float4_t nacr = {0.f, 0.f, 0.f, 0.f}, naci = {0.f, 0.f, 0.f, 0.f};  // negative-weight slots
     // step p: (barrier) filter plane p+2 from LDS, store staged plane p+3, load plane p+4, MFMAs of plane p
 #define AUTO_STEP(YP, YP1, YN, DP, DP1, DN)                                                      \
     if (p < xend) {                                                                            \
@@ -407,12 +411,16 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         const float c1 = g10 * DP1 + g21 * DN;                                                 \
         const float c2 = g20 * DN;                                                             \
         _Pragma("unroll") for (int bc = 0; bc < 9; ++bc) {                                     \
-            const float t0 = c0 * lamyz[bc], t1 = c1 * lamyz[bc], t2 = c2 * lamyz[bc];         \
             const float2 yp = YP[bc];                                                          \
-            const float bx = t0 * yp.x + t1 * YP1[bc].x + t2 * YN[bc].x;                       \
-            const float by = t0 * yp.y + t1 * YP1[bc].y + t2 * YN[bc].y;                       \
-            accr = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.x, bx, accr, 0, 0, 0);              \
-            acci = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.y, by, acci, 0, 0, 0);              \
+            const float bx = c0 * yp.x + c1 * YP1[bc].x + c2 * YN[bc].x;                       \
+            const float by = c0 * yp.y + c1 * YP1[bc].y + c2 * YN[bc].y;                       \
+            if (((bc / 3) < NNEG) != ((bc % 3) < NNEG)) {                                      \
+                nacr = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.x, bx, nacr, 0, 0, 0);          \
+                naci = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.y, by, naci, 0, 0, 0);          \
+            } else {                                                                           \
+                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.x, bx, accr, 0, 0, 0);          \
+                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.y, by, acci, 0, 0, 0);          \
+            }                                                                                  \
         }                                                                                      \
         ++p;                                                                                   \
     }
@@ -422,9 +430,11 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         AUTO_STEP(Y2, Y0, Y1, dC, dA, dB)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            slot[j * 16] += (double)accr[j] + (double)acci[j];
+            slot[j * 16] += ((double)accr[j] + (double)acci[j]) - ((double)nacr[j] + (double)naci[j]);
             accr[j] = 0.f;
             acci[j] = 0.f;
+            nacr[j] = 0.f;
+            naci[j] = 0.f;
         }
     }
 #undef AUTO_STEP
@@ -474,6 +484,20 @@ int hermes_multipole_summand_f64(hermes_ctx* ctx, int J, const double* d1, const
     return 0;
 }
 
+// Waits for the recorded (start, end) event pairs of timed Gram launches and adds their
+// durations to the totals (launches may sit on any stream).
+static int gram_flush(hermes_ctx* ctx) {
+    for (size_t i = 0; i < ctx->gram_pending; ++i) {
+        float ms = 0.0f;
+        HERMES_CHECK_HIP(hipEventSynchronize(ctx->gram_ev[2 * i + 1]));
+        HERMES_CHECK_HIP(hipEventElapsedTime(&ms, ctx->gram_ev[2 * i], ctx->gram_ev[2 * i + 1]));
+        ctx->gram_ms += ms;
+        ctx->gram_calls += 1;
+    }
+    ctx->gram_pending = 0;
+    return 0;
+}
+
 // Launches the Gram kernel and the fixed-order row reduction; the 16 x 16 raw sums land in
 // out_dev (256 doubles; row i, column j at i * 16 + j).  No host synchronisation.
 static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u, const void* V, int n_v,
@@ -503,6 +527,18 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
             for (int a = 0; a < 3; ++a) t += Q3x3_host[l1 * 3 + a] * lam3_host[a] * Q3x3_host[l2 * 3 + a];
             g.gx[l1 * 3 + l2] = (float)t;
         }
+    int order[3] = {0, 1, 2}, nneg = 0;  // eigenvalues ascending (insertion sort of three)
+    for (int i = 1; i < 3; ++i)
+        for (int j = i; j > 0 && lam3_host[order[j]] < lam3_host[order[j - 1]]; --j) {
+            const int t = order[j];
+            order[j] = order[j - 1];
+            order[j - 1] = t;
+        }
+    for (int c = 0; c < 3; ++c) {
+        const double lc = lam3_host[order[c]];
+        if (lc < 0.0) ++nneg;
+        for (int tap = 0; tap < 3; ++tap) g.Qs[tap * 3 + c] = (float)(Q3x3_host[tap * 3 + order[c]] * sqrt(fabs(lc)));
+    }
     // auto kernel: 16 z x 4 y cells per workgroup (needs L >= 16); cross kernel: 4 z x 4 y
     HERMES_REQUIRE(g.cross || J >= 4, "hermes_multipole_gram_f32: J=%d, the auto Gram needs J >= 4", J);
     const int nblocks = g.cross ? (L / 4) * (L / GRAM_WAVES) * (L / g.xseg)
@@ -511,13 +547,31 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
         ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256 + (size_t)RED_MAX_BLOCKS * 256));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
     const bool timed = ctx->time_gram && !ctx->capturing;
-    if (timed) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[0], ctx->stream));
+    size_t slot = 0;
+    if (timed) {
+        if (ctx->gram_pending >= 4096 && gram_flush(ctx)) return 1;
+        slot = 2 * ctx->gram_pending;
+        while (ctx->gram_ev.size() < slot + 2) {
+            hipEvent_t e = nullptr;
+            HERMES_CHECK_HIP(hipEventCreate(&e));
+            ctx->gram_ev.push_back(e);
+        }
+        HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[slot], ctx->stream));
+    }
     if (g.cross)
         k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     else
-        k_gram_auto<<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws);
+        switch (nneg) {
+            case 0: k_gram_auto<0><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 1: k_gram_auto<1><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 2: k_gram_auto<2><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            default: k_gram_auto<3><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+        }
     HERMES_LAUNCH_CHECK();
-    if (timed) HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[1], ctx->stream));
+    if (timed) {
+        HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[slot + 1], ctx->stream));
+        ctx->gram_pending += 1;
+    }
     sum_rows(ctx->stream, ws, nblocks, 256, ws + (size_t)nblocks * 256 + 256, out_dev);
     HERMES_LAUNCH_CHECK();
     if (gout) *gout = g;
@@ -535,12 +589,6 @@ int hermes_multipole_gram_f32(hermes_ctx* ctx, int J, const float* d1, const voi
     double h[256];
     HERMES_CHECK_HIP(hipMemcpyAsync(h, dst, sizeof(double) * 256, hipMemcpyDeviceToHost, ctx->stream));
     HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
-    if (ctx->time_gram) {
-        float ms = 0.0f;
-        HERMES_CHECK_HIP(hipEventElapsedTime(&ms, ctx->gram_ev[0], ctx->gram_ev[1]));
-        ctx->gram_ms += ms;
-        ctx->gram_calls += 1;
-    }
     for (int i = 0; i < g.S1; ++i)
         for (int j = 0; j < g.S2; ++j)
             out_host[i * g.S2 + j] = g.cross ? h[i * 16 + j] : h[i * 16 + j] + h[j * 16 + i];
@@ -555,12 +603,12 @@ int hermes_multipole_gram_f32_dev(hermes_ctx* ctx, int J, const float* d1, const
 
 int hermes_profile_gram(hermes_ctx* ctx, int enable, double* total_ms_out, int64_t* calls_out) {
     HERMES_REQUIRE(ctx, "hermes_profile_gram: null context");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_profile_gram: the stream is being captured");
     HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    if (gram_flush(ctx)) return 1;
     if (total_ms_out) *total_ms_out = ctx->gram_ms;
     if (calls_out) *calls_out = ctx->gram_calls;
     if (enable) {
-        for (int i = 0; i < 2; ++i)
-            if (!ctx->gram_ev[i]) HERMES_CHECK_HIP(hipEventCreate(&ctx->gram_ev[i]));
         ctx->gram_ms = 0.0;
         ctx->gram_calls = 0;
     }

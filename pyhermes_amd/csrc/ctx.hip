@@ -37,7 +37,8 @@ void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes) {
         return nullptr;
     }
     if (ctx->ws[slot]) {
-        (void)hipStreamSynchronize(ctx->stream);
+        // callers may have used the slot from another stream; growth is rare
+        (void)hipDeviceSynchronize();
         (void)hipFree(ctx->ws[slot]);
         ctx->ws[slot] = nullptr;
         ctx->ws_size[slot] = 0;
@@ -84,8 +85,7 @@ int hermes_ctx_destroy(hermes_ctx* ctx) {
     hermes_fft_cache_destroy(ctx->fft);
     if (ctx->twiddles) (void)hipFree(ctx->twiddles);
     if (ctx->dev_err) (void)hipFree(ctx->dev_err);
-    for (int i = 0; i < 2; ++i)
-        if (ctx->gram_ev[i]) (void)hipEventDestroy(ctx->gram_ev[i]);
+    for (hipEvent_t e : ctx->gram_ev) (void)hipEventDestroy(e);
     delete ctx;
     return 0;
 }

@@ -23,7 +23,8 @@ struct hermes_ctx {
     int fft_cols_variant[11][2] = {};  // autotuned strided-pass tile per (log2 L, inverse); 0 = not tuned
     // optional HIP-event timing of the Gram contraction kernel (hermes_profile_gram)
     bool time_gram = false;
-    hipEvent_t gram_ev[2] = {nullptr, nullptr};
+    std::vector<hipEvent_t> gram_ev;  // (start, end) per timed launch, summed when queried
+    size_t gram_pending = 0;          // recorded pairs not yet summed
     double gram_ms = 0.0;
     int64_t gram_calls = 0;
     // stream capture (hermes_capture_begin/end): while set, nothing may allocate, copy to or

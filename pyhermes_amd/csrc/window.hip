@@ -368,6 +368,15 @@ struct ApplyArgs {
     double radii_over_L[MAX_SHELLS];
 };
 
+// Several orders m of one l (the fused window + z-FFT kernel shares the radial factors among them).
+constexpr int MAX_MP = 2;
+struct MultiM {
+    int n;                  // 1..MAX_MP
+    int m[MAX_MP];
+    float cnorm[MAX_MP];    // N_l|m| (-1)^|m| (2|m|-1)!!
+    float sx[MAX_MP], sz[MAX_MP];
+};
+
 __device__ __forceinline__ float2 cmulf(float2 a, float2 b) { return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x); }
 __device__ __forceinline__ float2 conjf2(float2 a) { return make_float2(a.x, -a.y); }
 
@@ -445,87 +454,142 @@ __global__ __launch_bounds__(256) void k_multipole_apply(const float2* __restric
 // K_{R_s,l,m} at its octant representatives (x, y, z <= L/2) exactly as k_multipole_apply does, forms
 // the four mirror lines (x, y), (xm, y), (x, ym), (xm, ym) of H_full * K * scale in LDS with the same
 // mirror rules, transforms them along z and writes each distinct line once.  Loops over shells.
+
+// MP orders m of one l per launch: the radial factors j_l(2 pi |a| R_s) of the 8 aliases are
+// evaluated once per (representative, shell) and shared by the MP windows; the weights
+// P(a) Y_lm(a-hat) are kept per order (premultiplied).  The LDS z-FFT runs over 4 MP lines
+// (line mi * 4 + row) and order mi's shell s lands at out + (mi * n_shell + s) * L^3.
 template <int LOG2L>
-__global__ __launch_bounds__(256) void k_mpwin_zfft(const float2* __restrict__ H, const float* __restrict__ Pf,
-                                                    ApplyArgs A, const float2* __restrict__ twg,
-                                                    float2* __restrict__ out) {
+constexpr int mpwin_threads() {
+    // one z representative 0 .. L/2 - 1 per thread (at least a wave); for L >= 128 the Nyquist
+    // plane z = L/2 is left over and its eight alias terms are computed by eight helper lanes
+    return 64 * (((1 << LOG2L) / 2 + 63) / 64);
+}
+
+template <int LOG2L, int MP>
+__global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
+    const float2* __restrict__ H, const float* __restrict__ Pf, ApplyArgs A, MultiM M,
+    const float2* __restrict__ twg, float2* __restrict__ out) {
     constexpr int L = 1 << LOG2L, Lh = L / 2 + 1;
-    constexpr int NT = 256;
-    constexpr int ZPT = (Lh + NT - 1) / NT;
-    using F = hermes_fft::LdsFft<LOG2L, 4, NT, 1, false>;
+    constexpr int NT = mpwin_threads<LOG2L>();
+    constexpr bool NYQ = Lh > NT;  // z = L/2 by helper lanes 0..7
+    using F = hermes_fft::LdsFft<LOG2L, 4 * MP, NT, 1, false>;
     __shared__ float2 lds[F::ELEMS];
     __shared__ float2 tw[L];
+    __shared__ float2 nyq[8][MP];
     const int tid = threadIdx.x;
     const int x = blockIdx.x / Lh, y = blockIdx.x % Lh;
     const int xm = (L - x) & (L - 1), ym = (L - y) & (L - 1);
     const int64_t N = (int64_t)L * L * L;
     hermes_fft::load_twiddles<LOG2L>(tw, twg, tid, NT);
-    // shell-independent data of this thread's representatives z = tid + i * NT
-    float pw[ZPT][8];
-    float2 yv[ZPT][8];
-    double nrm[ZPT][8];
-    float2 h[ZPT][4];
+    // weights P(a) Y_lm(a-hat) of the 8 aliases per order (premultiplied) for this thread's z and, on the
+    // helper lanes, alias tid of the Nyquist plane
+    auto weights = [&](int z, int a, float2* w, double& nrm, bool& live) {
+        const int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+        const float pw = Pf[ax < 0 ? -ax : ax] * Pf[ay < 0 ? -ay : ay] * Pf[az < 0 ? -az : az];
+        live = pw != 0.0f;
+        nrm = sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
 #pragma unroll
-    for (int i = 0; i < ZPT; ++i) {
-        const int z = tid + i * NT;
-        if (z < Lh) {
-            h[i][0] = H[((int64_t)x * L + y) * Lh + z];
-            h[i][1] = H[((int64_t)xm * L + y) * Lh + z];
-            h[i][2] = H[((int64_t)x * L + ym) * Lh + z];
-            h[i][3] = H[((int64_t)xm * L + ym) * Lh + z];
-#pragma unroll
-            for (int a = 0; a < 8; ++a) {
-                const int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
-                pw[i][a] = Pf[ax < 0 ? -ax : ax] * Pf[ay < 0 ? -ay : ay] * Pf[az < 0 ? -az : az];
-                nrm[i][a] = sqrt((double)ax * ax + (double)ay * ay + (double)az * az);
-                float2 yy = ylm_f32(A.l, A.m < 0 ? -A.m : A.m, (float)ax, (float)ay, (float)az, A.cnorm);
-                if (A.m < 0) {  // Y_{l,-m} = (-1)^m conj(Y_{l,m})
-                    const float sg = ((-A.m) & 1) ? -1.0f : 1.0f;
-                    yy = make_float2(sg * yy.x, -sg * yy.y);
-                }
-                yv[i][a] = yy;
+        for (int mi = 0; mi < MP; ++mi) {
+            const int m = M.m[mi], am = m < 0 ? -m : m;
+            float2 yy = ylm_f32(A.l, am, (float)ax, (float)ay, (float)az, M.cnorm[mi]);
+            if (m < 0) {  // Y_{l,-m} = (-1)^m conj(Y_{l,m})
+                const float sg = (am & 1) ? -1.0f : 1.0f;
+                yy = make_float2(sg * yy.x, -sg * yy.y);
             }
+            w[mi] = make_float2(pw * yy.x, pw * yy.y);
+        }
+    };
+    const int z = tid;
+    const bool zact = z < Lh;
+    float2 wy[8][MP];
+    double nrm[8];
+    float2 h[4];
+    unsigned live = 0;
+    if (zact) {
+        h[0] = H[((int64_t)x * L + y) * Lh + z];
+        h[1] = H[((int64_t)xm * L + y) * Lh + z];
+        h[2] = H[((int64_t)x * L + ym) * Lh + z];
+        h[3] = H[((int64_t)xm * L + ym) * Lh + z];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            bool lv;
+            weights(z, a, wy[a], nrm[a], lv);
+            if (lv) live |= 1u << a;
         }
     }
+    float2 wn[MP], hn[4];
+    double nrm_n = 0.0;
+    bool live_n = false;
+    if (NYQ && tid < 8) weights(L / 2, tid, wn, nrm_n, live_n);
+    if (NYQ && tid < MP) {
+        hn[0] = H[((int64_t)x * L + y) * Lh + L / 2];
+        hn[1] = H[((int64_t)xm * L + y) * Lh + L / 2];
+        hn[2] = H[((int64_t)x * L + ym) * Lh + L / 2];
+        hn[3] = H[((int64_t)xm * L + ym) * Lh + L / 2];
+    }
+    // the four mirror rows of representative z (and, for z != -z, the rows of -z) for order mi
+    auto put = [&](int mi, int zz, const float2 hh[4], float2 Km) {
+        const float2 k = make_float2(Km.x * A.scale, Km.y * A.scale);
+        const float2 kc = conjf2(k);
+        const float sx = M.sx[mi], sz = M.sz[mi];
+        const int r0 = 4 * mi;
+        lds[F::addr(r0 + 0, zz)] = cmulf(hh[0], k);
+        lds[F::addr(r0 + 1, zz)] = cmulf(hh[1], make_float2(sx * kc.x, sx * kc.y));
+        lds[F::addr(r0 + 2, zz)] = cmulf(hh[2], kc);
+        lds[F::addr(r0 + 3, zz)] = cmulf(hh[3], make_float2(sx * k.x, sx * k.y));
+        const int zm = (L - zz) & (L - 1);
+        if (zm != zz) {
+            lds[F::addr(r0 + 0, zm)] = cmulf(conjf2(hh[3]), make_float2(sz * k.x, sz * k.y));
+            lds[F::addr(r0 + 1, zm)] = cmulf(conjf2(hh[2]), make_float2(sz * sx * kc.x, sz * sx * kc.y));
+            lds[F::addr(r0 + 2, zm)] = cmulf(conjf2(hh[1]), make_float2(sz * kc.x, sz * kc.y));
+            lds[F::addr(r0 + 3, zm)] = cmulf(conjf2(hh[0]), make_float2(sz * sx * k.x, sz * sx * k.y));
+        }
+    };
     for (int s = 0; s < A.n_shell; ++s) {
         const double rs = A.radii_over_L[s];
+        if (zact) {
+            float2 K[MP];
 #pragma unroll
-        for (int i = 0; i < ZPT; ++i) {
-            const int z = tid + i * NT;
-            if (z < Lh) {
+            for (int mi = 0; mi < MP; ++mi) K[mi] = make_float2(0.0f, 0.0f);
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                if (!((live >> a) & 1u)) continue;
+                const float j = jl_f32(A.l, nrm[a] * rs);
+#pragma unroll
+                for (int mi = 0; mi < MP; ++mi) {
+                    K[mi].x += j * wy[a][mi].x;
+                    K[mi].y += j * wy[a][mi].y;
+                }
+            }
+#pragma unroll
+            for (int mi = 0; mi < MP; ++mi) put(mi, z, h, K[mi]);
+        }
+        if (NYQ) {
+            if (tid < 8) {
+                const float j = live_n ? jl_f32(A.l, nrm_n * rs) : 0.0f;
+#pragma unroll
+                for (int mi = 0; mi < MP; ++mi) nyq[tid][mi] = make_float2(j * wn[mi].x, j * wn[mi].y);
+            }
+            __syncthreads();
+            if (tid < MP) {
                 float2 K = make_float2(0.0f, 0.0f);
-#pragma unroll
                 for (int a = 0; a < 8; ++a) {
-                    if (pw[i][a] == 0.0f) continue;
-                    const float j = jl_f32(A.l, nrm[i][a] * rs) * pw[i][a];
-                    K.x += j * yv[i][a].x;
-                    K.y += j * yv[i][a].y;
+                    K.x += nyq[a][tid].x;
+                    K.y += nyq[a][tid].y;
                 }
-                const float2 k = make_float2(K.x * A.scale, K.y * A.scale);
-                const float2 kc = conjf2(k);
-                const float sx = A.sx, sz = A.sz;
-                lds[F::addr(0, z)] = cmulf(h[i][0], k);
-                lds[F::addr(1, z)] = cmulf(h[i][1], make_float2(sx * kc.x, sx * kc.y));
-                lds[F::addr(2, z)] = cmulf(h[i][2], kc);
-                lds[F::addr(3, z)] = cmulf(h[i][3], make_float2(sx * k.x, sx * k.y));
-                const int zm = (L - z) & (L - 1);
-                if (zm != z) {
-                    lds[F::addr(0, zm)] = cmulf(conjf2(h[i][3]), make_float2(sz * k.x, sz * k.y));
-                    lds[F::addr(1, zm)] = cmulf(conjf2(h[i][2]), make_float2(sz * sx * kc.x, sz * sx * kc.y));
-                    lds[F::addr(2, zm)] = cmulf(conjf2(h[i][1]), make_float2(sz * kc.x, sz * kc.y));
-                    lds[F::addr(3, zm)] = cmulf(conjf2(h[i][0]), make_float2(sz * sx * k.x, sz * sx * k.y));
-                }
+                put(tid, L / 2, hn, K);
             }
         }
         __syncthreads();
         F::run(lds, tw, tid);
-        float2* o = out + (int64_t)s * N;
-        for (int i = tid; i < 4 * L; i += NT) {
-            const int row = i >> LOG2L, e = i & (L - 1);
+        for (int i = tid; i < 4 * MP * L; i += NT) {
+            const int line = i >> LOG2L, e = i & (L - 1);
+            const int mi = line >> 2, row = line & 3;
             const bool dx = row & 1, dy = row & 2;
             if ((dx && xm == x) || (dy && ym == y)) continue;  // mirror line coincides with line 0 / 2
             const int rx = dx ? xm : x, ry = dy ? ym : y;
-            o[((int64_t)rx * L + ry) * L + e] = lds[F::addr(row, e)];
+            out[(int64_t)(mi * A.n_shell + s) * N + ((int64_t)rx * L + ry) * L + e] = lds[F::addr(line, e)];
         }
         __syncthreads();
     }
@@ -948,29 +1012,66 @@ int hermes_multipole_apply_f32(hermes_ctx* ctx, int J, const void* H, const doub
 
 int hermes_multipole_window_ifft_f32(hermes_ctx* ctx, int J, const void* H, const double* P, const double* radii,
                                      int n_shell, int l, int m, double scale, void* out) {
-    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_window_ifft_f32: null argument");
-    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_window_ifft_f32: J=%d outside 3..10", J);
+    return hermes_multipole_window_ifft_multi_f32(ctx, J, H, P, radii, n_shell, l, &m, 1, scale, out);
+}
+
+int hermes_multipole_window_ifft_multi_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
+                                           const double* radii, int n_shell, int l, const int* ms_host, int n_m,
+                                           double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && radii && ms_host && out, "hermes_multipole_window_ifft_multi_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_window_ifft_multi_f32: J=%d outside 3..10", J);
+    HERMES_REQUIRE(n_m >= 1 && n_m <= MAX_MP, "hermes_multipole_window_ifft_multi_f32: n_m=%d outside 1..%d", n_m,
+                   MAX_MP);
     ApplyArgs A;
     float* Pf = nullptr;
-    int rc = make_apply_args(ctx, J, P, radii, n_shell, l, m, scale, &A, &Pf);
+    int rc = make_apply_args(ctx, J, P, radii, n_shell, l, ms_host[0], scale, &A, &Pf);
     if (rc) return rc;
+    MultiM M{};
+    M.n = n_m;
+    for (int i = 0; i < n_m; ++i) {
+        const int m = ms_host[i], am = m < 0 ? -m : m;
+        HERMES_REQUIRE(m >= -l && m <= l, "invalid multipole (l=%d, m=%d)", l, m);
+        M.m[i] = m;
+        M.cnorm[i] = (float)hermes_ylm_cnorm(l, am);
+        M.sx[i] = (am & 1) ? -1.0f : 1.0f;
+        M.sz[i] = ((l + am) & 1) ? -1.0f : 1.0f;
+    }
     const float2* twg = hermes_twiddles(ctx, J, 1);
     HERMES_REQUIRE(twg, "%s", hermes_last_error());
     const unsigned nblk = (unsigned)(((1 << J) / 2 + 1) * ((1 << J) / 2 + 1));
     const float2* h = (const float2*)H;
     float2* o = (float2*)out;
+#define HERMES_MPW(LG)                                                                                   \
+    if (n_m == 1)                                                                                        \
+        k_mpwin_zfft_multi<LG, 1><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o);  \
+    else                                                                                                 \
+        k_mpwin_zfft_multi<LG, 2><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o);
     switch (J) {
-        case 3: k_mpwin_zfft<3><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
-        case 4: k_mpwin_zfft<4><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
-        case 5: k_mpwin_zfft<5><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
-        case 6: k_mpwin_zfft<6><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
-        case 7: k_mpwin_zfft<7><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
-        case 8: k_mpwin_zfft<8><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
-        case 9: k_mpwin_zfft<9><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
-        default: k_mpwin_zfft<10><<<nblk, 256, 0, ctx->stream>>>(h, Pf, A, twg, o); break;
+        case 3: HERMES_MPW(3) break;
+        case 4: HERMES_MPW(4) break;
+        case 5: HERMES_MPW(5) break;
+        case 6: HERMES_MPW(6) break;
+        case 7: HERMES_MPW(7) break;
+        case 8: HERMES_MPW(8) break;
+        case 9: HERMES_MPW(9) break;
+        default:
+            // L = 1024: two orders per launch would not fit the registers; one launch per order
+            for (int i = 0; i < n_m; ++i) {
+                MultiM Mi{};
+                Mi.n = 1;
+                Mi.m[0] = M.m[i];
+                Mi.cnorm[0] = M.cnorm[i];
+                Mi.sx[0] = M.sx[i];
+                Mi.sz[0] = M.sz[i];
+                k_mpwin_zfft_multi<10, 1><<<nblk, mpwin_threads<10>(), 0, ctx->stream>>>(
+                    h, Pf, A, Mi, twg, o + (size_t)i * n_shell * ((size_t)1 << 30));
+                HERMES_LAUNCH_CHECK();
+            }
+            break;
     }
+#undef HERMES_MPW
     HERMES_LAUNCH_CHECK();
-    return hermes_fft3_inverse_yx(ctx, J, n_shell, o);
+    return hermes_fft3_inverse_yx(ctx, J, n_m * n_shell, o);
 }
 
 int hermes_reduce_sum(hermes_ctx* ctx, const void* x, int64_t n, int ncomp, int dbl, double* out_host) {

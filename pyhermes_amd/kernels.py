@@ -223,6 +223,19 @@ def multipole_window_ifft_f32(H, P, radii_grid, l, m, scale, out=None):
     return out
 
 
+def multipole_window_ifft_multi_f32(H, P, radii_grid, l, ms, scale, out):
+    """(len(ms) * S, L, L, L) complex64: the window + inverse FFT of every shell for each order in ``ms``
+    (1 or 2 orders of one l, sharing the radial factors); order i's shell s at index i * S + s."""
+    r = _np_f64(radii_grid)
+    S = r.size
+    ms = [int(m) for m in ms]
+    L = _check_multipole_inputs(H, P, len(ms) * S, out)
+    arr = (ctypes.c_int * len(ms))(*ms)
+    _lib.call("hermes_multipole_window_ifft_multi_f32", _lib.context(), _J_of(L), ptr(H), ptr(P),
+              r.ctypes.data_as(_DP), S, int(l), arr, len(ms), float(scale), ptr(out))
+    return out
+
+
 def reduce_sum(x):
     """Deterministic FP64 sum of a real or complex tensor."""
     out = np.zeros(2, dtype=np.float64)

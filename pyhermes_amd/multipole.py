@@ -131,8 +131,8 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     tm.stop(t)
     n_l = l_max - l_min + 1
     result = np.zeros((r12.size, r13.size, n_l), dtype=np.float64)
-    if auto and r12.size <= 8:
-        return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm)
+    if auto and r12.size <= 16:
+        return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm)
     blocks12 = _shell_blocks(r12.size)
     blocks13 = _shell_blocks(r13.size)
     U = torch.empty((r12.size, L, L, L), dtype=torch.complex64, device=d1.device)
@@ -169,26 +169,92 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     return result
 
 
-def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm):
-    """Auto case with S <= 8 shells: the Gram's MFMA tile is 16 shells wide, so 16 // S (l, m) terms share
-    one contraction and only their diagonal S x S blocks are kept."""
+_SIDE = {}
+
+
+def _side_stream(device):
+    """A second stream (default priority) per device for the contraction of the pipelined path."""
+    key = torch.device(device).index
+    if key not in _SIDE:
+        _SIDE[key] = torch.cuda.Stream(device=device)
+    return _SIDE[key]
+
+
+def _window_runs(group):
+    """Split consecutive (l, m) terms into window launches: runs of one l, at most two orders each."""
+    runs = []
+    for l, m in group:
+        if runs and runs[-1][0] == l and len(runs[-1][1]) < 2:
+            runs[-1][1].append(m)
+        else:
+            runs.append((l, [m]))
+    return runs
+
+
+def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, overlap=False):
+    """Auto case with S <= 16 shells.  Each iteration takes G = max(16 // S, 2) consecutive (l, m) terms:
+    window + inverse-FFT launches over runs of one l, two orders at a time (they share the radial factors),
+    fill a buffer of G * S shell fields, and Grams over chunks of 16 // S terms follow (the Gram's MFMA tile is
+    16 shells wide; only each term's diagonal S x S block is kept).  The Gram sums stay on the device until
+    the end.  overlap=True runs the windows of iteration k + 1 on the caller's stream while the Grams of
+    iteration k run on a second stream (two buffers).  Measured on the headline workload the two stages share
+    the SIMDs (both are issue-bound), so the overlap gains ~3% and doubles each Gram launch's duration; it is
+    off by default."""
     L = geo.L
     N = L ** 3
     S = r.size
     pack = 16 // S
+    G = max(pack, 2)
     lm = [(l, m) for l in range(l_min, l_max + 1) for m in range(l + 1)]
-    U = torch.empty((pack * S, L, L, L), dtype=torch.complex64, device=d1.device)
-    v_re = {}
-    for g0 in range(0, len(lm), pack):
-        group = lm[g0:g0 + pack]
+    iters = [lm[i:i + G] for i in range(0, len(lm), G)]
+    chunks = [[it[j:j + pack] for j in range(0, len(it), pack)] for it in iters]
+    n_blocks = sum(len(c) for c in chunks)
+    nbuf = min(2, len(iters)) if overlap else 1
+    bufs = [torch.empty((G * S, L, L, L), dtype=torch.complex64, device=d1.device) for _ in range(nbuf)]
+    blocks = torch.zeros((n_blocks, 256), dtype=torch.float64, device=d1.device)
+    main = torch.cuda.current_stream(d1.device)
+    side = _side_stream(d1.device) if overlap else main
+    side.wait_stream(main)  # d1, H and the buffers are ready on the caller's stream
+    gram_done = []
+    owners = []
+    for k, group in enumerate(iters):
+        U = bufs[k % len(bufs)]
+        if overlap and k >= len(bufs):
+            main.wait_event(gram_done[k - len(bufs)])
         t = tm.start("window_ifft")
-        for j, (l, m) in enumerate(group):
-            kernels.multipole_window_ifft_f32(H, geo.P, r, l, m, 1.0 / N, out=U[j * S:(j + 1) * S])
+        pos = 0
+        for l, ms in _window_runs(group):
+            kernels.multipole_window_ifft_multi_f32(H, geo.P, r, l, ms, 1.0 / N,
+                                                     U[pos * S:(pos + len(ms)) * S])
+            pos += len(ms)
         tm.stop(t)
-        t = tm.start("gram")
-        full = kernels.multipole_gram_f32(d1, U[:len(group) * S], geo.Q, geo.lam)
-        tm.stop(t)
-        for j, (l, m) in enumerate(group):
+        grams = []
+        pos = 0
+        for chunk in chunks[k]:
+            grams.append((U[pos * S:(pos + len(chunk)) * S], len(owners)))
+            owners.append(chunk)
+            pos += len(chunk)
+        if not overlap:
+            t = tm.start("gram")
+            for Uc, b in grams:
+                kernels.multipole_gram_f32_dev(d1, Uc, geo.Q, geo.lam, blocks[b])
+            tm.stop(t)
+            continue
+        win_done = torch.cuda.Event()
+        win_done.record(main)
+        with torch.cuda.stream(side):
+            side.wait_event(win_done)
+            for Uc, b in grams:
+                kernels.multipole_gram_f32_dev(d1, Uc, geo.Q, geo.lam, blocks[b])
+            ev = torch.cuda.Event()
+            ev.record(side)
+            gram_done.append(ev)
+    main.wait_stream(side)
+    host = blocks.cpu().numpy()
+    v_re = {}
+    for b, chunk in enumerate(owners):
+        full = kernels.gram_block_to_matrix(host[b], len(chunk) * S)
+        for j, (l, m) in enumerate(chunk):
             v_re[(l, m)] = 4.0 * np.pi * ((-1.0) ** (l + m)) * full[j * S:(j + 1) * S, j * S:(j + 1) * S] / N
     result = np.zeros((S, S, l_max - l_min + 1), dtype=np.float64)
     for li, l in enumerate(range(l_min, l_max + 1)):

@@ -114,3 +114,26 @@ def test_autotuner_records_a_candidate():
     kernels.fft3_c2c_f32(d, inverse=False)
     tw, nt = kernels.fft3_tile_choice(J, inverse=False)
     assert (tw, nt) == (1 << J, 256)
+
+
+@pytest.mark.parametrize("J", [4, 6, 7])
+def test_window_ifft_order_pairs_match_single_orders(phi, J):
+    """Two orders of one l in one launch (shared radial factors) equal two single-order launches to FP32
+    rounding (the two instantiations may contract multiply-adds differently)."""
+    import torch
+    from pyhermes_amd import kernels
+    L = 1 << J
+    g = np.random.default_rng(90 + J)
+    H = kernels.rfftn(kernels.as_device(g.normal(size=(L, L, L)).astype(np.float32), torch.float32))
+    P = kernels.as_device(scaling.fourier_power_spectrum(phi, 0, 1, L, 1024), torch.float64)
+    radii = np.array([1.5, 0.3 * L, 3.25])
+    S = radii.size
+    for l, ms in ((3, [1, 2]), (4, [0, -3]), (10, [9, 10])):
+        print(f"window pair J={J} l={l} m={ms}", flush=True)
+        out = torch.empty((2 * S, L, L, L), dtype=torch.complex64, device=H.device)
+        kernels.multipole_window_ifft_multi_f32(H, P, radii, l, ms, 1.0 / L ** 3, out)
+        for i, m in enumerate(ms):
+            one = kernels.multipole_window_ifft_f32(H, P, radii, l, m, 1.0 / L ** 3)
+            err = float((out[i * S:(i + 1) * S] - one).abs().max() / one.abs().max())
+            print(f"  m={m}: max rel diff {err:.2e}", flush=True)
+            assert err <= 1e-6, (l, m, err)

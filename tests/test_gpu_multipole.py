@@ -59,6 +59,30 @@ def test_gram_f32_matches_oracle(phi, cross, L):
             assert abs(got[i, j] - ref) <= 2e-6 * scale, (i, j, got[i, j], ref)
 
 
+@pytest.mark.parametrize("nneg", [0, 1, 2, 3])
+def test_gram_auto_for_any_eigenvalue_signs(nneg):
+    """The auto contraction for a symmetric Gamma with 0..3 negative eigenvalues, eigenpairs given unsorted."""
+    import torch
+    from pyhermes_amd import kernels
+    L = 16
+    rng = np.random.default_rng(40 + nneg)
+    Qm, _ = np.linalg.qr(rng.normal(size=(3, 3)))
+    lam = np.array([0.9, 0.3, 0.05]) * np.where(np.arange(3) < nneg, -1.0, 1.0)
+    perm = rng.permutation(3)
+    lam, Qm = lam[perm], Qm[:, perm]
+    g = Qm @ np.diag(lam) @ Qm.T
+    d = rng.normal(size=(L, L, L))
+    U = np.stack(_fields(L, 4, 50 + nneg))
+    got = kernels.multipole_gram_f32(kernels.as_device(d, torch.float32), kernels.as_device(U, torch.complex64),
+                                     Qm, lam)
+    for i in range(U.shape[0]):
+        for j in range(U.shape[0]):
+            W = np.conj(U[j])
+            ref = om.summand_c(d, U[i], W, g).real
+            scale = np.sqrt(np.sum(d ** 2) * np.sum(np.abs(U[i]) ** 2) * np.sum(np.abs(W) ** 2) / L ** 3)
+            assert abs(got[i, j] - ref) <= 2e-6 * scale, (nneg, i, j, got[i, j], ref)
+
+
 @pytest.mark.parametrize("L,lmax", [(16, 3)])
 def test_ddd_paths_match_oracle(phi, L, lmax):
     import torch
