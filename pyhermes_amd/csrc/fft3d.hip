@@ -128,12 +128,17 @@ constexpr int col_tile() {
 }
 
 // Column-pass tile shapes tried by the autotuner: (TW, threads).  Grids below 128^3 use variant 0 only.
-constexpr int N_COL_VARIANTS = 4;
+// (TW, threads) per variant: base = col_tile(); 0: (base, 256), 1: (base, 512), 2: (base/2, 256),
+// 3: (base/2, 512), 4: (2 base, 512), 5: (2 base, 1024).  Variants 4 and 5 stage 128 KB per workgroup.
+constexpr int N_COL_VARIANTS = 6;
+
+__host__ __device__ constexpr int col_tw_of(int base, int v) { return v >= 4 ? 2 * base : ((v & 2) ? base / 2 : base); }
+__host__ __device__ constexpr int col_nt_of(int v) { return v >= 4 ? ((v & 1) ? 1024 : 512) : ((v & 1) ? 512 : 256); }
 
 template <int LOG2L, int V>
 struct ColVariant {
-    static constexpr int TW = (V & 2) ? col_tile<LOG2L>() / 2 : col_tile<LOG2L>();
-    static constexpr int NT = (V & 1) ? 512 : 256;
+    static constexpr int TW = col_tw_of(col_tile<LOG2L>(), V);
+    static constexpr int NT = col_nt_of(V);
 };
 
 template <int LOG2L, int V, int DIR>
@@ -161,6 +166,8 @@ int launch_yx(hermes_ctx* ctx, int variant, int batch, float2* data, const float
         case 1: return launch_yx_variant<LOG2L, 1, DIR>(ctx, batch, data, twg);
         case 2: return launch_yx_variant<LOG2L, 2, DIR>(ctx, batch, data, twg);
         case 3: return launch_yx_variant<LOG2L, 3, DIR>(ctx, batch, data, twg);
+        case 4: return launch_yx_variant<LOG2L, 4, DIR>(ctx, batch, data, twg);
+        case 5: return launch_yx_variant<LOG2L, 5, DIR>(ctx, batch, data, twg);
         default: return launch_yx_variant<LOG2L, 0, DIR>(ctx, batch, data, twg);
     }
 }
@@ -307,8 +314,8 @@ int hermes_fft3_tile_choice(hermes_ctx* ctx, int J, int inverse, int* tw_out, in
     }
     const int L = 1 << J;
     const int base = (8192 >> J) < L ? (8192 >> J) : L;
-    *tw_out = (v & 2) ? base / 2 : base;
-    *threads_out = (v & 1) ? 512 : 256;
+    *tw_out = col_tw_of(base, v);
+    *threads_out = col_nt_of(v);
     return 0;
 }
 
@@ -319,7 +326,7 @@ int hermes_fft3_set_tile(hermes_ctx* ctx, int J, int inverse, int tw, int thread
     const int base = (8192 >> J) < L ? (8192 >> J) : L;
     int v = -1;
     for (int c = 0; c < (J < 7 ? 1 : N_COL_VARIANTS); ++c) {
-        const int ctw = (c & 2) ? base / 2 : base, cnt = (c & 1) ? 512 : 256;
+        const int ctw = col_tw_of(base, c), cnt = col_nt_of(c);
         if (ctw == tw && cnt == threads) v = c;
     }
     HERMES_REQUIRE(v >= 0, "hermes_fft3_set_tile: (tw=%d, threads=%d) is not a candidate tile for J=%d", tw, threads,

@@ -94,8 +94,8 @@ def test_every_strided_tile_matches_numpy(J):
     x = _rand_complex((1, L, L, L), 300 + J)
     ref = np.fft.ifftn(x.astype(np.complex128), axes=(1, 2, 3)) * L ** 3
     scale = np.abs(ref).max()
-    for tw in (base, base // 2):
-        for nt in (256, 512):
+    for tw, nt in ((base, 256), (base, 512), (base // 2, 256), (base // 2, 512), (2 * base, 512), (2 * base, 1024)):
+        if True:
             print(f"strided tile J={J} tw={tw} threads={nt}", flush=True)
             kernels.fft3_set_tile(J, tw, nt, inverse=True)
             d = kernels.as_device(x, torch.complex64)
