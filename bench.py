@@ -643,10 +643,12 @@ def main():
     traffic = gram_traffic_from_profile(workload)
     summ = timer.summary()
     n_modes = L * L * (L // 2 + 1)
-    # fused window + inverse FFT of S shells: the half spectrum (complex64) read once and S complex64
-    # fields written by the window/z pass, then read and written by each of the y and x passes
-    win_ms = summ["window_ifft"][0] / summ["window_ifft"][1]
-    win_bytes = n_modes * 8 + S * N * 8 + 2 * (2 * S * N * 8)
+    # fused window + inverse FFT: per call (one or two orders m of one l) the half spectrum (complex64) is
+    # read once and S complex64 fields per order written by the window/z pass, then read and written by
+    # each of the y and x passes
+    win_calls = summ["window_ifft"][1] / args.steps
+    win_ms = summ["window_ifft"][0] / args.steps
+    win_bytes = win_calls * n_modes * 8 + n_mterms * (S * N * 8 + 2 * (2 * S * N * 8))
     if args.copy_probe_mib > 0:
         n_el = args.copy_probe_mib * (1 << 20) // 4
         src = torch.ones(n_el, dtype=torch.float32, device="cuda")
@@ -680,7 +682,7 @@ def main():
         "stages_ms_per_step": stages,
         "projection_ms": proj_ms,
         "projection_hbm_frac": ((args.particles * 32 + N * 4) / (proj_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
-        "window_ifft": {"avg_call_ms": win_ms, "bytes": win_bytes,
+        "window_ifft": {"ms_per_step": win_ms, "calls_per_step": win_calls, "bytes_per_step": win_bytes,
                         "hbm_frac": win_bytes / (win_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     line.update(out)
