@@ -38,6 +38,9 @@ int hermes_ctx_create(int device, hermes_ctx **out);
 int hermes_ctx_destroy(hermes_ctx *ctx);
 int hermes_ctx_set_stream(hermes_ctx *ctx, void *hip_stream);
 int hermes_sync(hermes_ctx *ctx);
+/* Frees every scratch workspace of the context (they grow again on demand); refused
+ * while the context's stream is being captured.  No reference counterpart. */
+int hermes_release_workspaces(hermes_ctx *ctx);
 
 /* ---- graphs of whole batches (no reference counterpart; the reference runs each
  * catalogue eagerly).  hermes_capture_begin starts capturing the context's stream

@@ -25,6 +25,7 @@ _SIGNATURES = {
     "hermes_ctx_destroy": [c_void_p],
     "hermes_ctx_set_stream": [c_void_p, c_void_p],
     "hermes_sync": [c_void_p],
+    "hermes_release_workspaces": [c_void_p],
     "hermes_project": [c_void_p, c_void_p, c_void_p, c_i64, c_int, c_double, c_void_p, c_int, c_int, c_double,
                        c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],
     "hermes_fft_r2c": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p],

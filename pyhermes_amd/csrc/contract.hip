@@ -398,33 +398,49 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
 
     float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
     float4_t nacr = {0.f, 0.f, 0.f, 0.f}, naci = {0.f, 0.f, 0.f, 0.f};  // negative-weight slots
-    // step p: (barrier) filter plane p+2 from LDS, store staged plane p+3, load plane p+4, MFMAs of plane p
-#define AUTO_STEP(YP, YP1, YN, DP, DP1, DN)                                                      \
-    if (p < xend) {                                                                            \
-        __syncthreads();                                                                       \
-        tile_filter((p + 2) & 1, YN);                                                          \
-        stage_store((p + 3) & 1);                                                              \
-        stage_load(p + 4);                                                                     \
-        DN = dnext;                                                                            \
-        dnext = d1[(int64_t)((p + 3) & m) * LL + col];                                         \
-        const float c0 = g00 * DP + g11 * DP1 + g22 * DN;                                      \
-        const float c1 = g10 * DP1 + g21 * DN;                                                 \
-        const float c2 = g20 * DN;                                                             \
+    // Half of the waves (two per SIMD whichever way waves are dealt to SIMDs) run one plane behind:
+    // within a barrier interval they issue the MFMAs of plane p-1 before filtering plane p+2, while
+    // the others filter first and then issue the MFMAs of plane p, so every SIMD has VALU work from
+    // one group to overlap with the matrix-core work of the other.
+    const int wv = __builtin_amdgcn_readfirstlane(wave);
+    const bool lag = ((wv >> 2) ^ wv) & 1;
+    // MFMAs of one plane: YA its filtered values, YB1 / YB2 the next two planes, D* their d1 values
+#define AUTO_MFMA(YA, YB1, YB2, DA, DB1, DB2)                                                    \
+    {                                                                                          \
+        const float c0 = g00 * DA + g11 * DB1 + g22 * DB2;                                     \
+        const float c1 = g10 * DB1 + g21 * DB2;                                                \
+        const float c2 = g20 * DB2;                                                            \
         _Pragma("unroll") for (int bc = 0; bc < 9; ++bc) {                                     \
-            const float2 yp = YP[bc];                                                          \
-            const float bx = c0 * yp.x + c1 * YP1[bc].x + c2 * YN[bc].x;                       \
-            const float by = c0 * yp.y + c1 * YP1[bc].y + c2 * YN[bc].y;                       \
+            const float2 ya = YA[bc];                                                          \
+            const float bx = c0 * ya.x + c1 * YB1[bc].x + c2 * YB2[bc].x;                      \
+            const float by = c0 * ya.y + c1 * YB1[bc].y + c2 * YB2[bc].y;                      \
             if (((bc / 3) < NNEG) != ((bc % 3) < NNEG)) {                                      \
-                nacr = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.x, bx, nacr, 0, 0, 0);          \
-                naci = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.y, by, naci, 0, 0, 0);          \
+                nacr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.x, bx, nacr, 0, 0, 0);          \
+                naci = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.y, by, naci, 0, 0, 0);          \
             } else {                                                                           \
-                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.x, bx, accr, 0, 0, 0);          \
-                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(yp.y, by, acci, 0, 0, 0);          \
+                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.x, bx, accr, 0, 0, 0);          \
+                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.y, by, acci, 0, 0, 0);          \
             }                                                                                  \
+        }                                                                                      \
+    }
+    // step p (p = x0 .. xend): (barrier) [lagging waves: MFMAs of plane p-1, whose values sit in the
+    // YN / DN slots until the filter overwrites them], filter plane p+2 from LDS, store staged plane
+    // p+3, load plane p+4, [leading waves: MFMAs of plane p].  Step xend only finishes plane xend-1.
+#define AUTO_STEP(YP, YP1, YN, DP, DP1, DN)                                                      \
+    if (p <= xend) {                                                                           \
+        __syncthreads();                                                                       \
+        if (lag && p > x0) AUTO_MFMA(YN, YP, YP1, DN, DP, DP1)                                 \
+        if (p < xend) {                                                                        \
+            tile_filter((p + 2) & 1, YN);                                                      \
+            stage_store((p + 3) & 1);                                                          \
+            stage_load(p + 4);                                                                 \
+            DN = dnext;                                                                        \
+            dnext = d1[(int64_t)((p + 3) & m) * LL + col];                                     \
+            if (!lag) AUTO_MFMA(YP, YP1, YN, DP, DP1, DN)                                      \
         }                                                                                      \
         ++p;                                                                                   \
     }
-    for (int p = x0; p < xend;) {
+    for (int p = x0; p <= xend;) {
         AUTO_STEP(Y0, Y1, Y2, dA, dB, dC)
         AUTO_STEP(Y1, Y2, Y0, dB, dC, dA)
         AUTO_STEP(Y2, Y0, Y1, dC, dA, dB)
@@ -438,6 +454,7 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         }
     }
 #undef AUTO_STEP
+#undef AUTO_MFMA
     __syncthreads();
     for (int i = tid; i < 256; i += 64 * AUTO_WAVES) {
         double t = 0.0;

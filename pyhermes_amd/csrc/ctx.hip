@@ -62,6 +62,19 @@ const char* hermes_last_error(void) { return g_last_error; }
 
 int hermes_version(void) { return 100; }
 
+int hermes_release_workspaces(hermes_ctx* ctx) {
+    HERMES_REQUIRE(ctx, "null context");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_release_workspaces: the stream is being captured");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    HERMES_CHECK_HIP(hipDeviceSynchronize());  // the slots may be in use on any stream
+    for (int i = 0; i < 8; ++i) {
+        if (ctx->ws[i]) HERMES_CHECK_HIP(hipFree(ctx->ws[i]));
+        ctx->ws[i] = nullptr;
+        ctx->ws_size[i] = 0;
+    }
+    return 0;
+}
+
 int hermes_ctx_create(int device, hermes_ctx** out) {
     HERMES_REQUIRE(out != nullptr, "hermes_ctx_create: null output pointer");
     int n = 0;

@@ -25,6 +25,7 @@ constexpr int TILE_CELLS = MAX_TX * MAX_TY * MAX_TZ;
 constexpr int ACC_THREADS = 256;
 constexpr int SCAN_THREADS = 1024, SCAN_ITEMS = 4;  // counts per scan block: 4096
 constexpr int ACC_RCHUNK = 512;                      // records staged per LDS chunk
+constexpr int OFFCH = 128;                           // tile ranges staged per LDS block
 
 struct ProjRec {
     uint32_t cell;  // wrapped cell coordinates, 10 bits each (x | y<<10 | z<<20)
@@ -68,25 +69,32 @@ __device__ __forceinline__ int tile_id(const TileGeom& g, int X, int Y, int Z) {
     return (X << (g.lgnty + g.lgntz)) | (Y << g.lgntz) | Z;
 }
 
-// Wave-aggregated counter increment: every active lane adds 1 to ctr[key]; one atomic per distinct
-// key in the wave (catalogues are often stored in spatial order, so a wave's particles share tiles).
-// Returns the counter value before this lane's increment (old value + rank among the lanes with the
-// same key).  Must be reached by all lanes of the wave.
+// Counter increment for every active lane: ctr[key] += 1, returning the value before this lane's
+// increment (old value + rank among the lanes with the same key).  The lanes are first grouped by key
+// with register-only ballots (catalogues stored in spatial order put a wave's particles in few
+// tiles), at most WAVE_GROUPS groups (scattered keys would need one ballot round per lane); lanes left
+// over add 1 each.  Every group leader and left-over lane then issues its atomic in the same
+// instruction, so the wave waits for one atomic round trip.  Must be reached by all lanes of the wave.
+constexpr int WAVE_GROUPS = 8;
 __device__ __forceinline__ int wave_add(int* __restrict__ ctr, int key, bool active) {
     const int lane = threadIdx.x % 64;
     unsigned long long pending = __ballot(active);
-    int mine = 0;
-    while (pending) {
+    int leader_of = lane, rank = 0, group = 1;
+    for (int it = 0; it < WAVE_GROUPS && pending; ++it) {
         const int leader = __ffsll((long long)pending) - 1;
         const int lkey = __shfl(key, leader, 64);
         const unsigned long long same = __ballot(active && key == lkey) & pending;
-        int base = 0;
-        if (lane == leader) base = atomicAdd(&ctr[lkey], __popcll(same));
-        base = __shfl(base, leader, 64);
-        if ((same >> lane) & 1ull) mine = base + __popcll(same & ((1ull << lane) - 1ull));
+        if ((same >> lane) & 1ull) {
+            leader_of = leader;
+            rank = __popcll(same & ((1ull << lane) - 1ull));
+        }
+        if (lane == leader) group = __popcll(same);
         pending &= ~same;
     }
-    return mine;
+    int base = 0;
+    if (active && lane == leader_of) base = atomicAdd(&ctr[key], group);
+    base = __shfl(base, leader_of, 64);
+    return base + rank;
 }
 
 // Replica k (0..7) of a particle: the tile of stencil corner (X[k>>2], Y[(k>>1)&1], Z[k&1]) when that
@@ -194,9 +202,14 @@ __global__ __launch_bounds__(SCAN_THREADS) void k_scan_apply(const int* __restri
     if (blockIdx.x == 0 && tid == 0) off[n] = (int)bsum[nb];
 }
 
-__global__ void k_proj_fill(const double* __restrict__ pos, const double* __restrict__ w, int64_t n, double sf,
-                            int res, TileGeom g, const int* __restrict__ off, int* __restrict__ cursor,
-                            ProjRec* __restrict__ rec) {
+// Also reduces max |w| (as the bit pattern of a non-negative double: integer max of such patterns
+// orders them like the values, so the result is the same in any order) with one atomic per workgroup.
+__global__ __launch_bounds__(256) void k_proj_fill(const double* __restrict__ pos, const double* __restrict__ w,
+                                                   int64_t n, double sf, int res, TileGeom g,
+                                                   const int* __restrict__ off, int* __restrict__ cursor,
+                                                   ProjRec* __restrict__ rec, unsigned long long* __restrict__ wmax) {
+    __shared__ double wred[4];
+    double mw = 0.0;
     const int m = g.L - 1;
     const int64_t stride = (int64_t)gridDim.x * blockDim.x;
     const int64_t nround = (n + stride - 1) / stride * stride;
@@ -212,6 +225,7 @@ __global__ void k_proj_fill(const double* __restrict__ pos, const double* __rest
             r.cell = (uint32_t)(c[0] & m) | ((uint32_t)(c[1] & m) << 10) | ((uint32_t)(c[2] & m) << 20);
             r.frac = (uint32_t)f[0] | ((uint32_t)f[1] << 10) | ((uint32_t)f[2] << 20);
             r.w = w[p];
+            mw = fmax(mw, fabs(r.w));
         }
         int X[2], Y[2], Z[2];
         const int nx = tiles_of(c[0], m, g.lgtx, X), ny = tiles_of(c[1], m, g.lgty, Y),
@@ -223,33 +237,47 @@ __global__ void k_proj_fill(const double* __restrict__ pos, const double* __rest
             if (t >= 0) rec[off[t] + slot] = r;
         }
     }
+    for (int o = 32; o > 0; o >>= 1) mw = fmax(mw, __shfl_xor(mw, o, 64));
+    if (threadIdx.x % 64 == 0) wred[threadIdx.x / 64] = mw;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int i = 1; i < (int)blockDim.x / 64; ++i) mw = fmax(mw, wred[i]);
+        atomicMax(wmax, (unsigned long long)__double_as_longlong(mw));
+    }
 }
 
-// max |w| over all particles as the bit pattern of a non-negative double (integer max of such bit
-// patterns orders them like the values): deterministic in any order.
-__global__ void k_absmax(const double* __restrict__ w, int64_t n, unsigned long long* __restrict__ out) {
-    double mw = 0.0;
-    for (int64_t p = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; p < n; p += (int64_t)gridDim.x * blockDim.x)
-        mw = fmax(mw, fabs(w[p]));
-    for (int o = 32; o > 0; o >>= 1) mw = fmax(mw, __shfl_xor(mw, o, 64));
-    if (threadIdx.x % 64 == 0) atomicMax(out, (unsigned long long)__double_as_longlong(mw));
-}
 
 // Persistent workgroups loop over tiles (t = block, block + grid, ...; consecutive workgroups take
 // consecutive tiles, so concurrent stores stay close in memory).  Zeroing and output phases: thread t
 // takes pairs of consecutive z cells (16-byte LDS accesses at unit lane stride: no bank conflicts),
-// stored as 8-byte FP32 / 16-byte FP64 pairs.  Fixed-point scale per tile: every cell receives at most one contribution per record,
-// each bounded by max|w| * max|phi|^3 (global max |w|), so |sum| < 2^61 after scaling.
-// Fixed-point value q * 2^-S as FP32 without the FP64 route: |q| split into 32-bit halves, one fma;
-// within one ulp of the correctly rounded value.  Valid for S <= 100 (2^-S normal in FP32).
-__device__ __forceinline__ float fixed_to_f32(long long q, float two_32mS, float two_mS) {
+// stored as 8-byte FP32 / 16-byte FP64 pairs.  Fixed-point scale 2^S per tile: every cell receives at
+// most one contribution per record, each bounded by unit = max|w| * max|phi|^3 (global max |w|); S keeps
+// every scaled contribution below 2^51 (so round_to_ll applies) and the tile's sum below 2^61.
+// Fixed-point value q * 2^-S as FP32 without the FP64 route: |q| split into 32-bit halves, one fma,
+// then an exact power-of-two scaling; within one ulp of the correctly rounded value (underflow aside).
+__device__ __forceinline__ float fixed_to_f32(long long q, int S) {
     const unsigned long long a = q < 0 ? (unsigned long long)(-q) : (unsigned long long)q;
-    const float f = fmaf((float)(unsigned)(a >> 32), two_32mS, (float)(unsigned)(a & 0xffffffffull) * two_mS);
-    return q < 0 ? -f : f;
+    const float f = fmaf((float)(unsigned)(a >> 32), 4294967296.0f, (float)(unsigned)(a & 0xffffffffull));
+    return ldexpf(q < 0 ? -f : f, -S);
 }
 
-// FIXED: the full 4 x 8 x 64 tile (grids of 64^3 and larger) with compile-time extents.
-template <bool COUNTS, bool FIXED>
+// Round to the nearest integer (ties to even) for |x| < 2^51: x + 1.5 * 2^52 lies in [2^52, 2^53),
+// where the spacing of doubles is 1, so the addition itself rounds, and the difference of the bit
+// patterns is the integer.  Two instructions instead of the generic double -> int64 conversion.
+__device__ __forceinline__ long long round_to_ll(double x) {
+    constexpr double MAGIC = 6755399441055744.0;  // 1.5 * 2^52
+    return __double_as_longlong(x + MAGIC) - __double_as_longlong(MAGIC);
+}
+
+// OUTS: bit 0 = FP32 field, bit 1 = FP64 field.  FIXED: the full 4 x 8 x 64 tile (grids of 64^3 and
+// larger) with compile-time extents.
+//
+// Memory pipelining: stores count in the same in-order counter as loads, so a wait for a load issued
+// after a tile's output stores also waits for those stores to reach memory.  The only global loads of
+// the tile loop (the next tile's first record chunk) are therefore issued unconditionally (clamped
+// index) before the current tile's stores, and every tile, empty or not, issues the same number of
+// stores; the wait at the top of the next tile then leaves this tile's stores in flight.
+template <bool COUNTS, bool FIXED, int OUTS>
 __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
     const ProjRec* __restrict__ rec, const int* __restrict__ off, const double* __restrict__ phi, int res, TileGeom gg,
     const unsigned long long* __restrict__ wmax_bits, double phimax3, int ntiles, float* __restrict__ out32,
@@ -264,131 +292,176 @@ __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
     __shared__ __align__(16) int cnt[COUNTS ? TILE_CELLS : 4];
     __shared__ double sphi[3 * 1024];            // the scaling function, resident for the workgroup's life
     __shared__ __align__(16) ProjRec srec[ACC_RCHUNK];  // one chunk of the tile's records
+    __shared__ int2 soff[OFFCH];
     const int tid = threadIdx.x;
     const int m = g.L - 1;
     const int tx = 1 << g.lgtx, ty = 1 << g.lgty, tz = 1 << g.lgtz;
     const int ncell = tx * ty * tz;
     const int64_t L = g.L;
+    // the host requires res == 1024 (3 * 1024 samples): all loads of the table in flight at once
+    static_assert((3 * 1024) % ACC_THREADS == 0, "phi table staging");
+    double ph[3 * 1024 / ACC_THREADS];
+#pragma unroll
+    for (int j = 0; j < 3 * 1024 / ACC_THREADS; ++j) ph[j] = phi[tid + j * ACC_THREADS];
     const double unit = __longlong_as_double((long long)*wmax_bits) * phimax3;
-    for (int i = tid; i < 3 * res; i += ACC_THREADS) sphi[i] = phi[i];
+#pragma unroll
+    for (int j = 0; j < 3 * 1024 / ACC_THREADS; ++j) sphi[tid + j * ACC_THREADS] = ph[j];
 
     // Software pipeline over this workgroup's tiles t, t+G, t+2G, ...: while tile t is processed, the
-    // first record chunk of tile t+G is in flight into registers and the offsets of tile t+2G too.
+    // first record chunk of tile t+G is in flight into registers.
     const int G = gridDim.x;
     constexpr int RPT = ACC_RCHUNK / ACC_THREADS;  // records per thread per chunk
-    int t = blockIdx.x;
-    int beg = t < ntiles ? off[t] : 0, end = t < ntiles ? off[t + 1] : 0;
-    int nbeg = t + G < ntiles ? off[t + G] : 0, nend = t + G < ntiles ? off[t + G + 1] : 0;
-    ProjRec pref[RPT];
+    // This workgroup's tile ranges (begin, end) staged in LDS OFFCH tiles at a time: the loop then
+    // issues no global loads for them (a refill, every OFFCH tiles, waits for the stores in flight).
+    auto refill = [&](int k0) {
+        for (int i = tid; i < OFFCH; i += ACC_THREADS) {
+            const int tt = blockIdx.x + (k0 + i) * G;
+            soff[i] = tt < ntiles ? make_int2(off[tt], off[tt + 1]) : make_int2(0, 0);
+        }
+    };
+    // cell pairs per thread: a compile-time count, so the store count of the output phase is known to
+    // the compiler and its waits can leave those stores in flight
+    constexpr int NPAIR = TILE_CELLS / (2 * ACC_THREADS);
 #pragma unroll
-    for (int k = 0; k < RPT; ++k)
-        if (beg + tid + k * ACC_THREADS < end) pref[k] = rec[beg + tid + k * ACC_THREADS];
-    for (; t < ntiles; t += G) {
-        const int tn2 = t + 2 * G;
-        const int nnbeg = tn2 < ntiles ? off[tn2] : 0, nnend = tn2 < ntiles ? off[tn2 + 1] : 0;
+    for (int j = 0; j < NPAIR; ++j) {
+        const int i2 = tid * 2 + j * 2 * ACC_THREADS;
+        *(longlong2*)&acc[i2] = make_longlong2(0, 0);
+        if (COUNTS) *(int2*)&cnt[i2] = make_int2(0, 0);
+    }
+    refill(0);
+    __syncthreads();
+    int beg = soff[0].x, end = soff[0].y;
+    // the next chunk in registers, two records per thread, moved as raw 16-byte vectors (a struct copy
+    // would be placed in private or LDS memory, whose store waits for the load)
+    static_assert(RPT == 2 && sizeof(ProjRec) == 16, "prefetch registers assume two 16-byte records per thread");
+    const uint4* __restrict__ rec4 = (const uint4*)rec;
+    uint4* srec4 = (uint4*)srec;
+    {  // the first tile's first chunk, directly
+        const int i0 = beg + tid, i1 = beg + tid + ACC_THREADS;
+        srec4[tid] = rec4[i0 < end ? i0 : 0];  // index 0 always exists; such entries are never read
+        srec4[tid + ACC_THREADS] = rec4[i1 < end ? i1 : 0];
+    }
+    // offset of each of this thread's cell pairs from its tile's first cell (the same for every tile)
+    int loff[NPAIR];
+#pragma unroll
+    for (int j = 0; j < NPAIR; ++j) {
+        const int i2 = tid * 2 + j * 2 * ACC_THREADS;
+        const int lz = i2 & (tz - 1), ly = (i2 >> g.lgtz) & (ty - 1), lx = i2 >> (g.lgtz + g.lgty);
+        loff[j] = (lx * g.L + ly) * g.L + lz;
+    }
+    int k = 0;  // this workgroup's tile counter: t = blockIdx.x + k G
+    for (int t = blockIdx.x; t < ntiles; t += G, ++k) {
+        if ((k + 1) % OFFCH == 0) {  // the next tile's range starts a new staged block
+            __syncthreads();
+            refill(k + 1);
+        }
         const int tzid = t & ((1 << g.lgntz) - 1), tyid = (t >> g.lgntz) & ((1 << g.lgnty) - 1);
         const int txid = t >> (g.lgntz + g.lgnty);
         const int ox = txid << g.lgtx, oy = tyid << g.lgty, oz = tzid << g.lgtz;
-        auto cell_base = [&](int i2) -> int64_t {
-            const int lz = i2 & (tz - 1), ly = (i2 >> g.lgtz) & (ty - 1), lx = i2 >> (g.lgtz + g.lgty);
-            return ((int64_t)(ox + lx) * L + (oy + ly)) * L + (oz + lz);
-        };
-        __syncthreads();  // the previous tile's taps and output phase are done with srec and acc
-        // this tile's first chunk: prefetched registers -> LDS; then prefetch the next tile's first chunk
-#pragma unroll
-        for (int k = 0; k < RPT; ++k)
-            if (beg + tid + k * ACC_THREADS < end) srec[tid + k * ACC_THREADS] = pref[k];
-#pragma unroll
-        for (int k = 0; k < RPT; ++k)
-            if (nbeg + tid + k * ACC_THREADS < nend) pref[k] = rec[nbeg + tid + k * ACC_THREADS];
-        if (beg == end) {
-            for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
-                const int64_t gi = cell_base(i2);
-                if (out32) *(float2*)(out32 + gi) = make_float2(0.f, 0.f);
-                if (out64) *(double2*)(out64 + gi) = make_double2(0.0, 0.0);
-                if (COUNTS) *(int2*)(counts_out + gi) = make_int2(0, 0);
+        const int64_t tbase = ((int64_t)ox * L + oy) * L + oz;  // the tile's first cell
+        __syncthreads();  // this tile's first chunk is in srec; the previous output phase is done with acc
+        const int2 nr2 = soff[(k + 1) % OFFCH];
+        const int nbeg = nr2.x, nend = nr2.y;
+        // prefetch the next tile's first chunk into registers; it is moved to srec after this tile's
+        // output stores (written at the end of the body, so the wait for it sees only in-loop paths)
+        uint4 pref0, pref1;
+        {
+            const int i0 = nbeg + tid, i1 = nbeg + tid + ACC_THREADS;
+            pref0 = rec4[i0 < nend ? i0 : 0];
+            pref1 = rec4[i1 < nend ? i1 : 0];
+        }
+        const double bound = (double)(end - beg) * unit;
+        int e = 0, eu = 0;
+        frexp(bound, &e);
+        frexp(unit, &eu);
+        int S = 61 - e < 51 - eu ? 61 - e : 51 - eu;
+        if (S > 1000) S = 1000;  // keep 2^S finite for vanishing weights
+        const double mult = ldexp(1.0, S), inv = ldexp(1.0, -S);
+        // (the accumulators are zero here: cleared before the loop and by each tile's output phase)
+        // (record, tap) pairs spread over the workgroup, product order ((w phi_x) phi_y) phi_z; records
+        // beyond the first chunk (rare) are staged synchronously
+        for (int rb = beg; rb < end; rb += ACC_RCHUNK) {
+            const int nr = end - rb < ACC_RCHUNK ? end - rb : ACC_RCHUNK;
+            if (rb != beg) {
+                for (int r = tid; r < nr; r += ACC_THREADS) srec[r] = rec[rb + r];
+                __syncthreads();
             }
-        } else {
-            const double bound = (double)(end - beg) * unit;
-            int e = 0;
-            frexp(bound, &e);
-            int S = 61 - e;
-            if (S > 1000) S = 1000;  // keep 2^S finite for vanishing weights
-            const double mult = ldexp(1.0, S), inv = ldexp(1.0, -S);
-            const bool f32_direct = out32 && !out64 && S <= 100;
-            const float two_32mS = ldexpf(1.0f, 32 - S), two_mS = ldexpf(1.0f, -S);
-            for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
-                *(longlong2*)&acc[i2] = make_longlong2(0, 0);
-                if (COUNTS) *(int2*)&cnt[i2] = make_int2(0, 0);
-            }
-            __syncthreads();
-            // (record, tap) pairs spread over the workgroup, product order ((w phi_x) phi_y) phi_z; records
-            // beyond the first chunk (rare) are staged synchronously
-            for (int rb = beg; rb < end; rb += ACC_RCHUNK) {
-                const int nr = end - rb < ACC_RCHUNK ? end - rb : ACC_RCHUNK;
-                if (rb != beg) {
-                    for (int r = tid; r < nr; r += ACC_THREADS) srec[r] = rec[rb + r];
-                    __syncthreads();
+            // items (record, x tap); y and z taps limited to the tile; 2^S folded into w (exact), so
+            // each contribution is round((((w 2^S) phi_x) phi_y) phi_z)
+            for (int it = tid; it < nr * 3; it += ACC_THREADS) {
+                const int r = it / 3, sx = it - 3 * r;
+                const ProjRec R = srec[r];
+                const int cx = R.cell & 1023;
+                const int lx = ((cx - sx) & m) - ox;
+                if (lx < 0 || lx >= tx) continue;
+                const int cy = (R.cell >> 10) & 1023, cz = (R.cell >> 20) & 1023;
+                const int fx = R.frac & 1023, fy = (R.frac >> 10) & 1023, fz = (R.frac >> 20) & 1023;
+                const double phx = (R.w * mult) * sphi[fx + res * sx];
+                const int rowx = lx << g.lgty;
+#pragma unroll
+                for (int sy = 0; sy < 3; ++sy) {
+                    const int ly = ((cy - sy) & m) - oy;
+                    if (ly < 0 || ly >= ty) continue;
+                    const double phxy = phx * sphi[fy + res * sy];
+                    const int rowxy = (rowx + ly) << g.lgtz;
+#pragma unroll
+                    for (int sz = 0; sz < 3; ++sz) {
+                        const int lz = ((cz - sz) & m) - oz;
+                        if (lz < 0 || lz >= tz) continue;
+                        const long long q = round_to_ll(phxy * sphi[fz + res * sz]);
+                        atomicAdd((unsigned long long*)&acc[rowxy + lz], (unsigned long long)q);
+                    }
                 }
-                // items (record, x tap); y and z taps limited to the tile; 2^S folded into w (exact), so
-                // each contribution is round((((w 2^S) phi_x) phi_y) phi_z)
-                for (int it = tid; it < nr * 3; it += ACC_THREADS) {
-                    const int r = it / 3, sx = it - 3 * r;
+            }
+            if (COUNTS) {
+                for (int r = tid; r < nr; r += ACC_THREADS) {
                     const ProjRec R = srec[r];
-                    const int cx = R.cell & 1023;
-                    const int lx = ((cx - sx) & m) - ox;
-                    if (lx < 0 || lx >= tx) continue;
-                    const int cy = (R.cell >> 10) & 1023, cz = (R.cell >> 20) & 1023;
-                    const int fx = R.frac & 1023, fy = (R.frac >> 10) & 1023, fz = (R.frac >> 20) & 1023;
-                    const double phx = (R.w * mult) * sphi[fx + res * sx];
-                    const int rowx = lx << g.lgty;
-#pragma unroll
-                    for (int sy = 0; sy < 3; ++sy) {
-                        const int ly = ((cy - sy) & m) - oy;
-                        if (ly < 0 || ly >= ty) continue;
-                        const double phxy = phx * sphi[fy + res * sy];
-                        const int rowxy = (rowx + ly) << g.lgtz;
-#pragma unroll
-                        for (int sz = 0; sz < 3; ++sz) {
-                            const int lz = ((cz - sz) & m) - oz;
-                            if (lz < 0 || lz >= tz) continue;
-                            const long long q = __double2ll_rn(phxy * sphi[fz + res * sz]);
-                            atomicAdd((unsigned long long*)&acc[rowxy + lz], (unsigned long long)q);
-                        }
-                    }
+                    const int hx = (int)(R.cell & 1023) - ox, hy = (int)((R.cell >> 10) & 1023) - oy;
+                    const int hz = (int)((R.cell >> 20) & 1023) - oz;
+                    if (hx >= 0 && hx < tx && hy >= 0 && hy < ty && hz >= 0 && hz < tz)
+                        atomicAdd(&cnt[(((hx << g.lgty) + hy) << g.lgtz) + hz], 1);
                 }
-                if (COUNTS) {
-                    for (int r = tid; r < nr; r += ACC_THREADS) {
-                        const ProjRec R = srec[r];
-                        const int hx = (int)(R.cell & 1023) - ox, hy = (int)((R.cell >> 10) & 1023) - oy;
-                        const int hz = (int)((R.cell >> 20) & 1023) - oz;
-                        if (hx >= 0 && hx < tx && hy >= 0 && hy < ty && hz >= 0 && hz < tz)
-                            atomicAdd(&cnt[(((hx << g.lgty) + hy) << g.lgtz) + hz], 1);
-                    }
-                }
-                __syncthreads();  // chunk consumed; after the last chunk the accumulators are complete
             }
-            if (f32_direct && !COUNTS) {
-                for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
-                    const longlong2 a = *(const longlong2*)&acc[i2];
-                    *(float2*)(out32 + cell_base(i2)) =
-                        make_float2(fixed_to_f32(a.x, two_32mS, two_mS), fixed_to_f32(a.y, two_32mS, two_mS));
-                }
-            } else {
-                for (int i2 = tid * 2; i2 < ncell; i2 += ACC_THREADS * 2) {
-                    const int64_t gi = cell_base(i2);
-                    const longlong2 a = *(const longlong2*)&acc[i2];
-                    const double v0 = (double)a.x * inv, v1 = (double)a.y * inv;
-                    if (out32) *(float2*)(out32 + gi) = make_float2((float)v0, (float)v1);
-                    if (out64) *(double2*)(out64 + gi) = make_double2(v0, v1);
-                    if (COUNTS) *(int2*)(counts_out + gi) = *(const int2*)&cnt[i2];
-                }
+            __syncthreads();  // chunk consumed; after the last chunk the accumulators are complete
+        }
+        // output: the same stores for every tile (an empty tile's accumulators are zero); each pair is
+        // cleared for the next tile as it is read
+#pragma unroll
+        for (int j = 0; j < NPAIR; ++j) {
+            const int i2 = tid * 2 + j * 2 * ACC_THREADS;
+            if (!FIXED && i2 >= ncell) break;
+            const int64_t gi = tbase + loff[j];
+            const longlong2 a = *(const longlong2*)&acc[i2];
+            *(longlong2*)&acc[i2] = make_longlong2(0, 0);
+            if (OUTS == 1) {
+                *(float2*)(out32 + gi) = make_float2(fixed_to_f32(a.x, S), fixed_to_f32(a.y, S));
+            } else if (OUTS != 0) {
+                const double v0 = (double)a.x * inv, v1 = (double)a.y * inv;
+                if (OUTS & 1) *(float2*)(out32 + gi) = make_float2((float)v0, (float)v1);
+                *(double2*)(out64 + gi) = make_double2(v0, v1);
+            }
+            if (COUNTS) {
+                *(int2*)(counts_out + gi) = *(const int2*)&cnt[i2];
+                *(int2*)&cnt[i2] = make_int2(0, 0);
             }
         }
+        // srec was last read by the tap loop, before the barrier that closed the last chunk
+        srec4[tid] = pref0;
+        srec4[tid + ACC_THREADS] = pref1;
         beg = nbeg;
         end = nend;
-        nbeg = nnbeg;
-        nend = nnend;
+    }
+}
+
+template <bool COUNTS, bool FIXED>
+void launch_accumulate(int outs, int grid, hipStream_t st, const ProjRec* rec, const int* off, const double* phi,
+                       int res, TileGeom g, const unsigned long long* wmax, double phimax3, int ntiles, float* out32,
+                       double* out64, int32_t* counts) {
+    switch (outs) {
+        case 0: k_proj_accumulate<COUNTS, FIXED, 0><<<grid, ACC_THREADS, 0, st>>>(rec, off, phi, res, g, wmax, phimax3, ntiles, out32, out64, counts); break;
+        case 1: k_proj_accumulate<COUNTS, FIXED, 1><<<grid, ACC_THREADS, 0, st>>>(rec, off, phi, res, g, wmax, phimax3, ntiles, out32, out64, counts); break;
+        case 2: k_proj_accumulate<COUNTS, FIXED, 2><<<grid, ACC_THREADS, 0, st>>>(rec, off, phi, res, g, wmax, phimax3, ntiles, out32, out64, counts); break;
+        default: k_proj_accumulate<COUNTS, FIXED, 3><<<grid, ACC_THREADS, 0, st>>>(rec, off, phi, res, g, wmax, phimax3, ntiles, out32, out64, counts); break;
     }
 }
 
@@ -438,10 +511,6 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
     HERMES_CHECK_HIP(hermes_fill_u32(st, count, (size_t)ntiles, 0u));
     unsigned long long* wmax = (unsigned long long*)tw;  // first 8 bytes of the u64 slot
     HERMES_CHECK_HIP(hermes_fill_u32(st, wmax, 2, 0u));
-    if (n > 0) {
-        k_absmax<<<hermes_blocks(n, 256, 1024), 256, 0, st>>>(w, n, wmax);
-        HERMES_LAUNCH_CHECK();
-    }
     // Under stream capture invalid positions are reported through the context's sticky error word
     // (they are projected at the origin, in bounds); eagerly the call checks and fails here.
     int* errp = ctx->capturing ? ctx->dev_err : err;
@@ -473,7 +542,8 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
         HERMES_REQUIRE(rec, "%s", hermes_last_error());
     }
     if (n > 0) {
-        k_proj_fill<<<hermes_blocks(n, 256, 4096), 256, 0, st>>>(pos, w, n, sf, phi_res, g, off, cursor, rec);
+        k_proj_fill<<<hermes_blocks(n, 256, 4096), 256, 0, st>>>(pos, w, n, sf, phi_res, g, off, cursor, rec,
+                                                                 wmax);
         HERMES_LAUNCH_CHECK();
     }
     const double phimax3 = phi_abs_max * phi_abs_max * phi_abs_max * (1.0 + 1e-12);
@@ -481,20 +551,21 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
     (void)hipDeviceGetAttribute(&n_cu, hipDeviceAttributeMultiprocessorCount, ctx->device);
     const int acc_grid = ntiles < 3 * n_cu ? ntiles : 3 * n_cu;
     const bool full = g.lgtx == 2 && g.lgty == 3 && g.lgtz == 6;
+    const int outs = (eps32 ? 1 : 0) | (eps64 ? 2 : 0);
     if (cell_counts) {
         if (full)
-            k_proj_accumulate<true, true><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax, phimax3,
-                                                                             ntiles, eps32, eps64, cell_counts);
+            launch_accumulate<true, true>(outs, acc_grid, st, rec, off, phi, phi_res, g, wmax, phimax3, ntiles, eps32,
+                                          eps64, cell_counts);
         else
-            k_proj_accumulate<true, false><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax, phimax3,
-                                                                              ntiles, eps32, eps64, cell_counts);
+            launch_accumulate<true, false>(outs, acc_grid, st, rec, off, phi, phi_res, g, wmax, phimax3, ntiles, eps32,
+                                           eps64, cell_counts);
     } else {
         if (full)
-            k_proj_accumulate<false, true><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax, phimax3,
-                                                                              ntiles, eps32, eps64, nullptr);
+            launch_accumulate<false, true>(outs, acc_grid, st, rec, off, phi, phi_res, g, wmax, phimax3, ntiles, eps32,
+                                           eps64, nullptr);
         else
-            k_proj_accumulate<false, false><<<acc_grid, ACC_THREADS, 0, st>>>(rec, off, phi, phi_res, g, wmax,
-                                                                               phimax3, ntiles, eps32, eps64, nullptr);
+            launch_accumulate<false, false>(outs, acc_grid, st, rec, off, phi, phi_res, g, wmax, phimax3, ntiles, eps32,
+                                            eps64, nullptr);
     }
     HERMES_LAUNCH_CHECK();
     return 0;

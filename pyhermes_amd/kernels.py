@@ -342,6 +342,11 @@ def project_reserve(J, n_max):
     _lib.call("hermes_project_reserve", _lib.context(), int(J), int(n_max))
 
 
+def release_workspaces():
+    """Free the library's scratch workspaces on the current device (they grow again on demand)."""
+    _lib.call("hermes_release_workspaces", _lib.context())
+
+
 def delta_f32(eps32, a, b, d_out=None, dc_out=None):
     """d = a * eps - b (FP32) into d_out (real) and/or dc_out (complex64 real part)."""
     if d_out is None and dc_out is None:
