@@ -288,15 +288,26 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
 //
 // Memory: the MFMA lane layout puts 16 shells (arrays N apart) in lanes 0..15, so direct loads touch
 // 16 cache lines per instruction.  Instead a workgroup of 16 waves owns a 16 (z) x 4 (y) cell tile and
-// stages each plane's halo'd tile (6 rows x 18 z x 16 shells) into LDS with loads that run along z,
-// double-buffered one plane ahead; lanes read their 3 x 3 (y, z) neighbourhoods from LDS.
+// stages each plane's halo'd tile (6 rows x 18 z x 16 shells) into LDS with loads that run along z.
+//
+// Filter: Y[b][c] = sum_{ly, lz} Qs[ly][b] Qs[lz][c] F(y - ly, z - lz), the y taps first.  The y-filtered
+// rows T[b](y, z') = sum_ly Qs[ly][b] F(y - ly, z') of a plane are shared by the three cells z' .. z' + 2,
+// so the workgroup computes them once per tile position (4 rows x 18 z x 16 shells, one or two per
+// thread) into LDS, one plane ahead, and each lane applies only the z taps to its three T values per b.
+// Pipeline at step p (after the barrier): z taps of plane p+2 from T buffer (p+2)&1, y taps of plane p+3
+// from raw buffer (p+3)&1 into T buffer (p+3)&1, staged plane p+4 stored to raw buffer (p+4)&1, plane
+// p+5 loaded into registers; each buffer is written and read on opposite sides of a barrier.
 constexpr int AUTO_WAVES = 16;
 constexpr int AUTO_TY = 4, AUTO_TZ = 16;                       // cells per workgroup tile
 constexpr int AUTO_RY = AUTO_TY + 2, AUTO_RZ = AUTO_TZ + 2;   // halo'd tile
 constexpr int AUTO_PITCH = 17;                                // 16 shells + 1 pad per tile position
-constexpr int AUTO_TILE = AUTO_RY * AUTO_RZ * AUTO_PITCH;     // float2 per buffer
+constexpr int AUTO_TILE = AUTO_RY * AUTO_RZ * AUTO_PITCH;     // float2 per raw buffer
+constexpr int AUTO_TBUF = 3 * AUTO_TY * AUTO_RZ * AUTO_PITCH; // float2 per T buffer: [b][row][z'][shell]
 constexpr int AUTO_STAGE = AUTO_RY * AUTO_RZ * 16;            // elements staged per plane
 constexpr int AUTO_SPT = (AUTO_STAGE + 64 * AUTO_WAVES - 1) / (64 * AUTO_WAVES);
+constexpr int AUTO_TPOS = AUTO_TY * AUTO_RZ * 16;             // T positions per plane (1152)
+constexpr int AUTO_TPT = (AUTO_TPOS + 64 * AUTO_WAVES - 1) / (64 * AUTO_WAVES);
+static_assert(2 * (AUTO_TILE + AUTO_TBUF) * 8 >= AUTO_WAVES * 256 * 8, "reduction reuses the tile buffers");
 
 // NNEG = number of negative eigenvalues of Gamma; with the columns of Qs sorted by ascending lam and
 // scaled by sqrt|lam|, the filtered slot (b, c) carries sqrt|lam_b lam_c| and its weight's sign is
@@ -306,8 +317,10 @@ template <int NNEG>
 __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __restrict__ d1,
                                                                const float2* __restrict__ U, GramArgs g,
                                                                double* __restrict__ partial) {
-    __shared__ float2 tile[2][AUTO_TILE];
-    __shared__ double red[AUTO_WAVES][256];
+    // raw buffers, T buffers; the final reduction reuses the same memory
+    __shared__ __align__(16) float2 smem[2 * AUTO_TILE + 2 * AUTO_TBUF];
+    auto tile = [&](int buf) -> float2* { return smem + buf * AUTO_TILE; };
+    auto tb = [&](int buf) -> float2* { return smem + 2 * AUTO_TILE + buf * AUTO_TBUF; };
     const int L = g.L, m = L - 1;
     const int tid = threadIdx.x;
     const int lane = tid % 64, wave = tid / 64;
@@ -325,7 +338,7 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     const int zb = (r / (PZ * nyb)) * PZ + (r % (PZ * nyb)) % PZ;
     const int yb = (r % (PZ * nyb)) / PZ;
     const int z0 = zb * AUTO_TZ, y0 = yb * AUTO_TY;
-    const int y = y0 + yw, z = z0 + 4 * zw + k;
+    const int y = y0 + yw;
     const int x0 = xs * g.xseg, xend = x0 + g.xseg;
     const int64_t N = (int64_t)L * L * L;
     const int64_t LL = (int64_t)L * L;
@@ -335,7 +348,7 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     for (int i = 0; i < 9; ++i) Q[i] = g.Qs[i];
     const float g00 = 0.5f * g.gx[0], g11 = 0.5f * g.gx[4], g22 = 0.5f * g.gx[8];
     const float g10 = g.gx[3], g21 = g.gx[7], g20 = g.gx[6];
-    const int64_t col = (int64_t)y * L + z;
+    const int64_t col = (int64_t)y * L + (z0 + 4 * zw + k);
 
     // staging: element e = (sh * AUTO_RY + ry) * AUTO_RZ + rz (z fastest: loads run along z);
     // padding shells (sh >= S1) read shell 0 (their result rows are never read)
@@ -360,37 +373,73 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     auto stage_store = [&](int buf) {
 #pragma unroll
         for (int i = 0; i < AUTO_SPT; ++i)
-            if (act[i]) tile[buf][dst[i]] = stg[i];
+            if (act[i]) tile(buf)[dst[i]] = stg[i];
     };
-    // this lane's neighbourhood: raw[ly * 3 + lz] = F(y - ly, z - lz) at tile row yw+2-ly, column 4zw+k+2-lz
-    const int me = ((yw + 2) * AUTO_RZ + (4 * zw + k + 2)) * AUTO_PITCH + s;
-    auto tile_filter = [&](int buf, float2 Y[9]) {
-        float2 raw[9];
+    // y taps: position u = (row * AUTO_RZ + z') * 16 + sh reads raw rows row + 2 - ly (tile row r holds
+    // y0 - 2 + r) and writes T[b][row][z'][sh]
+    int traw[AUTO_TPT], tout[AUTO_TPT];
+    bool tact[AUTO_TPT];
 #pragma unroll
-        for (int ly = 0; ly < 3; ++ly)
+    for (int i = 0; i < AUTO_TPT; ++i) {
+        const int u = tid + i * 64 * AUTO_WAVES;
+        tact[i] = u < AUTO_TPOS;
+        const int uu = tact[i] ? u : 0;
+        const int sh = uu % 16, zp = (uu / 16) % AUTO_RZ, row = uu / (16 * AUTO_RZ);
+        traw[i] = ((row + 2) * AUTO_RZ + zp) * AUTO_PITCH + sh;
+        tout[i] = (row * AUTO_RZ + zp) * AUTO_PITCH + sh;
+    }
+    auto tphase = [&](int buf) {
 #pragma unroll
-            for (int lz = 0; lz < 3; ++lz) raw[ly * 3 + lz] = tile[buf][me - (ly * AUTO_RZ + lz) * AUTO_PITCH];
-        filter_plane(raw, Q, Y);
+        for (int i = 0; i < AUTO_TPT; ++i) {
+            if (!tact[i]) continue;
+            const float2 r0 = tile(buf)[traw[i]];
+            const float2 r1 = tile(buf)[traw[i] - AUTO_RZ * AUTO_PITCH];
+            const float2 r2 = tile(buf)[traw[i] - 2 * AUTO_RZ * AUTO_PITCH];
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const float q0 = Q[0 * 3 + b], q1 = Q[1 * 3 + b], q2 = Q[2 * 3 + b];
+                tb(buf)[b * (AUTO_TY * AUTO_RZ * AUTO_PITCH) + tout[i]] =
+                    make_float2(q0 * r0.x + q1 * r1.x + q2 * r2.x, q0 * r0.y + q1 * r1.y + q2 * r2.y);
+            }
+        }
+    };
+    // z taps of this lane's cell: T[b] at z' = (4 zw + k + 2) - lz of row yw
+    const int tme = (yw * AUTO_RZ + (4 * zw + k + 2)) * AUTO_PITCH + s;
+    auto zfilter = [&](int buf, float2 Y[9]) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+            const float2* t = tb(buf) + b * (AUTO_TY * AUTO_RZ * AUTO_PITCH) + tme;
+            const float2 u0 = t[0], u1 = t[-AUTO_PITCH], u2 = t[-2 * AUTO_PITCH];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const float q0 = Q[0 * 3 + c], q1 = Q[1 * 3 + c], q2 = Q[2 * 3 + c];
+                Y[b * 3 + c] = make_float2(q0 * u0.x + q1 * u1.x + q2 * u2.x, q0 * u0.y + q1 * u1.y + q2 * u2.y);
+            }
+        }
     };
 
-    double* slot = &red[wave][(4 * (lane / 16)) * 16 + (lane % 16)];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) slot[j * 16] = 0.0;
+    double acc_slot[4] = {0.0, 0.0, 0.0, 0.0};  // this lane's G entries 4 (lane / 16) + j, column lane % 16
 
     float2 Y0[9], Y1[9], Y2[9];
-    // warm-up: Y0 = plane x0, Y1 = plane x0+1, tile[(x0+2)&1] = plane x0+2, stg = plane x0+3
+    // warm-up (the steps x0-3 .. x0-1 without MFMAs): Y0 = plane x0, Y1 = plane x0+1, T buffer
+    // (x0+2)&1 = plane x0+2, raw buffer (x0+3)&1 = plane x0+3, stg = plane x0+4
     stage_load(x0);
     stage_store(x0 & 1);
     stage_load(x0 + 1);
     __syncthreads();
-    tile_filter(x0 & 1, Y0);
+    tphase(x0 & 1);
     stage_store((x0 + 1) & 1);
     stage_load(x0 + 2);
     __syncthreads();
-    tile_filter((x0 + 1) & 1, Y1);
-    __syncthreads();
+    zfilter(x0 & 1, Y0);
+    tphase((x0 + 1) & 1);
     stage_store(x0 & 1);
     stage_load(x0 + 3);
+    __syncthreads();
+    zfilter((x0 + 1) & 1, Y1);
+    tphase(x0 & 1);
+    stage_store((x0 + 1) & 1);
+    stage_load(x0 + 4);
     float dA = d1[(int64_t)(x0 & m) * LL + col];
     float dB = d1[(int64_t)((x0 + 1) & m) * LL + col];
     float dC = 0.0f;
@@ -399,9 +448,9 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
     float4_t nacr = {0.f, 0.f, 0.f, 0.f}, naci = {0.f, 0.f, 0.f, 0.f};  // negative-weight slots
     // Half of the waves (two per SIMD whichever way waves are dealt to SIMDs) run one plane behind:
-    // within a barrier interval they issue the MFMAs of plane p-1 before filtering plane p+2, while
-    // the others filter first and then issue the MFMAs of plane p, so every SIMD has VALU work from
-    // one group to overlap with the matrix-core work of the other.
+    // within a barrier interval they issue the MFMAs of plane p-1 before the filter work of the step,
+    // while the others filter first and then issue the MFMAs of plane p, so every SIMD has VALU work
+    // from one group beside the matrix work of the other.
     const int wv = __builtin_amdgcn_readfirstlane(wave);
     const bool lag = ((wv >> 2) ^ wv) & 1;
     // MFMAs of one plane: YA its filtered values, YB1 / YB2 the next two planes, D* their d1 values
@@ -424,16 +473,18 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         }                                                                                      \
     }
     // step p (p = x0 .. xend): (barrier) [lagging waves: MFMAs of plane p-1, whose values sit in the
-    // YN / DN slots until the filter overwrites them], filter plane p+2 from LDS, store staged plane
-    // p+3, load plane p+4, [leading waves: MFMAs of plane p].  Step xend only finishes plane xend-1.
+    // YN / DN slots until the filter overwrites them], z taps of plane p+2, y taps of plane p+3, store
+    // of staged plane p+4, load of plane p+5, [leading waves: MFMAs of plane p].  Step xend only
+    // finishes plane xend-1.
 #define AUTO_STEP(YP, YP1, YN, DP, DP1, DN)                                                      \
     if (p <= xend) {                                                                           \
         __syncthreads();                                                                       \
         if (lag && p > x0) AUTO_MFMA(YN, YP, YP1, DN, DP, DP1)                                 \
         if (p < xend) {                                                                        \
-            tile_filter((p + 2) & 1, YN);                                                      \
-            stage_store((p + 3) & 1);                                                          \
-            stage_load(p + 4);                                                                 \
+            zfilter((p + 2) & 1, YN);                                                          \
+            tphase((p + 3) & 1);                                                               \
+            stage_store((p + 4) & 1);                                                          \
+            stage_load(p + 5);                                                                 \
             DN = dnext;                                                                        \
             dnext = d1[(int64_t)((p + 3) & m) * LL + col];                                     \
             if (!lag) AUTO_MFMA(YP, YP1, YN, DP, DP1, DN)                                      \
@@ -446,7 +497,7 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         AUTO_STEP(Y2, Y0, Y1, dC, dA, dB)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            slot[j * 16] += ((double)accr[j] + (double)acci[j]) - ((double)nacr[j] + (double)naci[j]);
+            acc_slot[j] += ((double)accr[j] + (double)acci[j]) - ((double)nacr[j] + (double)naci[j]);
             accr[j] = 0.f;
             acci[j] = 0.f;
             nacr[j] = 0.f;
@@ -455,11 +506,16 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     }
 #undef AUTO_STEP
 #undef AUTO_MFMA
+    // per-wave G blocks through LDS (the tile buffers are free after this barrier), summed in a fixed order
+    __syncthreads();
+    double* red = (double*)smem;  // [AUTO_WAVES][256]
+#pragma unroll
+    for (int j = 0; j < 4; ++j) red[wave * 256 + (4 * (lane / 16) + j) * 16 + (lane % 16)] = acc_slot[j];
     __syncthreads();
     for (int i = tid; i < 256; i += 64 * AUTO_WAVES) {
         double t = 0.0;
 #pragma unroll
-        for (int w = 0; w < AUTO_WAVES; ++w) t += red[w][i];
+        for (int w = 0; w < AUTO_WAVES; ++w) t += red[w * 256 + i];
         partial[(int64_t)blockIdx.x * 256 + i] = t;
     }
 }

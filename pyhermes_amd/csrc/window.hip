@@ -583,13 +583,23 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
         }
         __syncthreads();
         F::run(lds, tw, tid);
-        for (int i = tid; i < 4 * MP * L; i += NT) {
-            const int line = i >> LOG2L, e = i & (L - 1);
+        // lines known at compile time: the skip of a mirror line that coincides with line 0 / 2 and the
+        // line's base address are uniform, and a line's LDS reads are all issued before its stores
+        constexpr int EPT = (L + NT - 1) / NT;  // elements per thread and line
+#pragma unroll
+        for (int line = 0; line < 4 * MP; ++line) {
             const int mi = line >> 2, row = line & 3;
             const bool dx = row & 1, dy = row & 2;
-            if ((dx && xm == x) || (dy && ym == y)) continue;  // mirror line coincides with line 0 / 2
+            if ((dx && xm == x) || (dy && ym == y)) continue;
             const int rx = dx ? xm : x, ry = dy ? ym : y;
-            out[(int64_t)(mi * A.n_shell + s) * N + ((int64_t)rx * L + ry) * L + e] = lds[F::addr(line, e)];
+            float2* __restrict__ o = out + (int64_t)(mi * A.n_shell + s) * N + ((int64_t)rx * L + ry) * L;
+            float2 v[EPT];
+#pragma unroll
+            for (int q = 0; q < EPT; ++q)
+                if (L % NT == 0 || tid + q * NT < L) v[q] = lds[F::addr(line, tid + q * NT)];
+#pragma unroll
+            for (int q = 0; q < EPT; ++q)
+                if (L % NT == 0 || tid + q * NT < L) o[tid + q * NT] = v[q];
         }
         __syncthreads();
     }
