@@ -54,6 +54,17 @@ void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes) {
     return p;
 }
 
+int hermes_read_word(hermes_ctx* ctx, const int* dev, int* out) {
+    if (!ctx->host_word) HERMES_CHECK_HIP(hipHostMalloc((void**)&ctx->host_word, 64, hipHostMallocDefault));
+    HERMES_CHECK_HIP(hipMemcpyAsync(ctx->host_word, dev, sizeof(int), hipMemcpyDeviceToHost, ctx->stream));
+    hipError_t e;
+    while ((e = hipStreamQuery(ctx->stream)) == hipErrorNotReady) {
+    }
+    HERMES_CHECK_HIP(e);
+    *out = *(volatile int*)ctx->host_word;
+    return 0;
+}
+
 void hermes_fft_cache_destroy(hermes_fft_cache* c);
 
 extern "C" {
@@ -98,6 +109,7 @@ int hermes_ctx_destroy(hermes_ctx* ctx) {
     hermes_fft_cache_destroy(ctx->fft);
     if (ctx->twiddles) (void)hipFree(ctx->twiddles);
     if (ctx->dev_err) (void)hipFree(ctx->dev_err);
+    if (ctx->host_word) (void)hipHostFree(ctx->host_word);
     for (hipEvent_t e : ctx->gram_ev) (void)hipEventDestroy(e);
     delete ctx;
     return 0;

@@ -31,6 +31,7 @@ struct hermes_ctx {
     // from the host, memset or synchronise; kernels report input errors into dev_err
     bool capturing = false;
     int* dev_err = nullptr;  // sticky device error word (see hermes_read_errors)
+    int* host_word = nullptr;  // pinned host word for hermes_read_word (allocated on first use)
 };
 
 // Workspace slots (each slot is an independent grow-only buffer).
@@ -49,6 +50,9 @@ void hermes_set_error(const char* fmt, ...);
 // Fill n 32-bit words with v on the stream (a kernel, so it can be captured into graphs).
 hipError_t hermes_fill_u32(hipStream_t st, void* p, size_t n, uint32_t v);
 void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes);
+// Reads one device int after the stream's preceding work, waiting by polling the stream (a blocking
+// wait would add the host thread's wake-up latency to every eager call).  Not while capturing.
+int hermes_read_word(hermes_ctx* ctx, const int* dev, int* out);
 // Twiddle table e^{dir 2 pi i k / 2^log2L}, k < 2^log2L (device memory owned by ctx).
 const float2* hermes_twiddles(hermes_ctx* ctx, int log2L, int dir);
 // Inverse FFT along y and x of a batch of (L, L, L) complex64 grids already transformed along z.

@@ -253,12 +253,12 @@ __global__ __launch_bounds__(256) void k_proj_fill(const double* __restrict__ po
 // stored as 8-byte FP32 / 16-byte FP64 pairs.  Fixed-point scale 2^S per tile: every cell receives at
 // most one contribution per record, each bounded by unit = max|w| * max|phi|^3 (global max |w|); S keeps
 // every scaled contribution below 2^51 (so round_to_ll applies) and the tile's sum below 2^61.
-// Fixed-point value q * 2^-S as FP32 without the FP64 route: |q| split into 32-bit halves, one fma,
-// then an exact power-of-two scaling; within one ulp of the correctly rounded value (underflow aside).
+// Fixed-point value q * 2^-S as FP32: the signed high word and the unsigned low word converted
+// exactly, joined by one FP64 fma (q itself, rounded once: |q| < 2^61), scaled exactly by 2^-S and
+// rounded to FP32; within one ulp of the correctly rounded value.
 __device__ __forceinline__ float fixed_to_f32(long long q, int S) {
-    const unsigned long long a = q < 0 ? (unsigned long long)(-q) : (unsigned long long)q;
-    const float f = fmaf((float)(unsigned)(a >> 32), 4294967296.0f, (float)(unsigned)(a & 0xffffffffull));
-    return ldexpf(q < 0 ? -f : f, -S);
+    const double hi = (double)(int)(q >> 32), lo = (double)(unsigned)(q & 0xffffffffll);
+    return (float)ldexp(fma(hi, 4294967296.0, lo), -S);
 }
 
 // Round to the nearest integer (ties to even) for |x| < 2^51: x + 1.5 * 2^52 lies in [2^52, 2^53),
@@ -511,8 +511,8 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
     HERMES_CHECK_HIP(hermes_fill_u32(st, count, (size_t)ntiles, 0u));
     unsigned long long* wmax = (unsigned long long*)tw;  // first 8 bytes of the u64 slot
     HERMES_CHECK_HIP(hermes_fill_u32(st, wmax, 2, 0u));
-    // Under stream capture invalid positions are reported through the context's sticky error word
-    // (they are projected at the origin, in bounds); eagerly the call checks and fails here.
+    // Invalid positions are projected at the origin (in bounds) and flagged: under stream capture in the
+    // context's sticky error word, eagerly in a word the call reads back before it returns (and fails).
     int* errp = ctx->capturing ? ctx->dev_err : err;
     HERMES_REQUIRE(errp, "hermes_project: no device error word");
     if (!ctx->capturing) HERMES_CHECK_HIP(hermes_fill_u32(st, err, 1, 0u));
@@ -526,17 +526,24 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
     HERMES_LAUNCH_CHECK();
     k_scan_apply<<<nscan, SCAN_THREADS, 0, st>>>(count, ntiles, bsum, nscan, off, cursor);
     HERMES_LAUNCH_CHECK();
+    // Records: a particle's 3-cell footprint meets at most 2 tiles per axis (tile sides >= 4), so 8n
+    // records always suffice.  Up to RESERVE_BYTES that worst case is reserved and the kernels run back
+    // to back, the positions checked once at the end; larger catalogues are sized exactly after the
+    // count (a host synchronisation).  Capturing always uses the worst case (hermes_project_reserve).
+    constexpr size_t RESERVE_BYTES = (size_t)2 << 30;
+    const size_t worst = (size_t)(n > 0 ? 8 * n : 1) * sizeof(ProjRec);
     ProjRec* rec = nullptr;
-    if (ctx->capturing) {
-        // a particle's 3-cell footprint meets at most 2 tiles per axis (tile sides >= 4): <= 8 records
-        rec = (ProjRec*)hermes_workspace(ctx, WS_PROJ_REC, (size_t)(n > 0 ? 8 * n : 1) * sizeof(ProjRec));
-        HERMES_REQUIRE(rec, "%s (hermes_project_reserve sizes it)", hermes_last_error());
+    bool checked = false;
+    if (ctx->capturing || worst <= RESERVE_BYTES) {
+        rec = (ProjRec*)hermes_workspace(ctx, WS_PROJ_REC, worst);
+        HERMES_REQUIRE(rec, "%s%s", hermes_last_error(), ctx->capturing ? " (hermes_project_reserve sizes it)" : "");
     } else {
         int host[2] = {0, 0};
         HERMES_CHECK_HIP(hipMemcpyAsync(&host[0], err, sizeof(int), hipMemcpyDeviceToHost, st));
         HERMES_CHECK_HIP(hipMemcpyAsync(&host[1], off + ntiles, sizeof(int), hipMemcpyDeviceToHost, st));
         HERMES_CHECK_HIP(hipStreamSynchronize(st));
         HERMES_REQUIRE(host[0] == 0, "hermes_project: non-finite or out-of-range particle positions");
+        checked = true;
         const int total = host[1];
         rec = (ProjRec*)hermes_workspace(ctx, WS_PROJ_REC, (size_t)(total > 0 ? total : 1) * sizeof(ProjRec));
         HERMES_REQUIRE(rec, "%s", hermes_last_error());
@@ -568,6 +575,11 @@ extern "C" int hermes_project(hermes_ctx* ctx, const double* pos, const double* 
                                             eps64, nullptr);
     }
     HERMES_LAUNCH_CHECK();
+    if (!ctx->capturing && !checked) {  // invalid positions were projected at the origin (in bounds)
+        int host_err = 0;
+        if (hermes_read_word(ctx, err, &host_err)) return 1;
+        HERMES_REQUIRE(host_err == 0, "hermes_project: non-finite or out-of-range particle positions");
+    }
     return 0;
 }
 

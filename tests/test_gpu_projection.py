@@ -89,3 +89,38 @@ def test_fft_roundtrip_against_numpy(dtype):
     assert np.abs(f.cpu().numpy() - np.fft.fftn(c)).max() <= tol * np.abs(np.fft.fftn(c)).max()
     kernels.fftn_c2c(f, inverse=True, out=f)
     assert np.abs(f.cpu().numpy() / L ** 3 - c).max() <= tol * 10
+
+
+@pytest.mark.parametrize("case", ["f32_only", "f64_only", "counts_only", "dense_tiles", "tiny_weights"])
+def test_projection_output_modes_match_oracle(phi, case):
+    """Each output combination on its own (FP32 only is the batched pipeline's mode), tiles holding more
+    than one 512-record chunk, and weights small enough that the fixed-point scale exceeds 2^100."""
+    from pyhermes_amd import kernels
+    box, J = 1000.0, 6  # 64^3: the full 4 x 8 x 64 tiles
+    rng = np.random.default_rng(55)
+    if case == "dense_tiles":  # 6000 particles in a 4 x 8 x 31-cell box: tiles with > 512 records
+        pos = rng.uniform(0.0, 1.0, size=(6000, 3)) * np.array([4.0, 8.0, 31.0]) * box / (1 << J) + 100.0
+    else:
+        pos = rng.uniform(0.0, box, size=(30000, 3))
+    w = rng.lognormal(0.0, 0.3, pos.shape[0]) * (1e-35 if case == "tiny_weights" else 1.0)
+    print(f"projection output mode {case}", flush=True)
+    eps_o, cell_o, _ = oproj.project_c(pos, w, phi, J, box)
+    want32, want64, counts = case != "f64_only" and case != "counts_only", case == "f64_only", case == "counts_only"
+    out = kernels.project(pos, w, J, box, phi, want32=want32, want64=want64, want_counts=counts)
+    if counts:
+        L = 1 << J
+        wc = np.mod(cell_o, L)
+        counts_o = np.zeros((L, L, L), dtype=np.int64)
+        np.add.at(counts_o, (wc[:, 0], wc[:, 1], wc[:, 2]), 1)
+        np.testing.assert_array_equal(out["counts"].cpu().numpy(), counts_o)
+        return
+    if want64:
+        e64 = out["eps64"].cpu().numpy()
+        assert np.abs(e64 - eps_o).max() <= 1e-12 * np.abs(eps_o).max()
+        return
+    e32 = out["eps32"].cpu().numpy().astype(np.float64)
+    rel32 = np.linalg.norm(e32 - eps_o) / np.linalg.norm(eps_o)
+    assert rel32 < 1e-6, rel32
+    both = kernels.project(pos, w, J, box, phi, want32=True, want64=True)["eps32"].cpu().numpy().astype(np.float64)
+    # the FP32-only conversion and the FP64-then-FP32 route differ by at most one rounding step
+    assert np.all(np.abs(e32 - both) <= 2.0 ** -22 * np.abs(both)), np.abs(e32 - both).max()
