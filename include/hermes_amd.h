@@ -195,6 +195,15 @@ int hermes_multipole_window_ifft_multi_f32(hermes_ctx *ctx, int J, const void *H
                                            const double *radii, int n_shell, int l, const int *ms_host, int n_m,
                                            double scale, void *out);
 
+/* The two halves of hermes_multipole_window_ifft_multi_f32, so that they can run on different
+ * streams: the windows and the z pass (out then holds the fields transformed along z only), and the
+ * y and x inverse passes of a batch of (L, L, L) complex64 grids already transformed along z.  The
+ * second also completes any other such batch.  No counterpart. */
+int hermes_multipole_window_zpass_multi_f32(hermes_ctx *ctx, int J, const void *H, const double *P,
+                                            const double *radii, int n_shell, int l, const int *ms_host, int n_m,
+                                            double scale, void *out);
+int hermes_fft3_inverse_yx_f32(hermes_ctx *ctx, int J, int batch, void *data);
+
 /* Deterministic FP64 sum of n real (ncomp = 1) or complex (ncomp = 2) values,
  * stored as float (dbl = 0) or double (dbl = 1); result to host.  Replaces the
  * np.sum in SFCField._derived_field_integral, sfc_field.py:312-324. */

@@ -66,6 +66,9 @@ _SIGNATURES.update({
                                    c_void_p],
     "hermes_multipole_window_ifft_multi_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int,
                                                ctypes.POINTER(c_int), c_int, c_double, c_void_p],
+    "hermes_multipole_window_zpass_multi_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int,
+                                               ctypes.POINTER(c_int), c_int, c_double, c_void_p],
+    "hermes_fft3_inverse_yx_f32": [c_void_p, c_int, c_int, c_void_p],
     "hermes_reduce_sum": [c_void_p, c_void_p, c_i64, c_int, c_int, _dpp],
     "hermes_multipole_summand_f64": [c_void_p, c_int, c_void_p, c_void_p, c_void_p, _dpp, _dpp],
     "hermes_multipole_gram_f32": [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, _dpp, _dpp,

@@ -303,6 +303,13 @@ int hermes_fft3_inverse_yx(hermes_ctx* ctx, int J, int batch, float2* data) {
 
 extern "C" {
 
+int hermes_fft3_inverse_yx_f32(hermes_ctx* ctx, int J, int batch, void* data) {
+    HERMES_REQUIRE(ctx && data, "hermes_fft3_inverse_yx_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_fft3_inverse_yx_f32: J=%d outside 3..10", J);
+    HERMES_REQUIRE(batch >= 1, "hermes_fft3_inverse_yx_f32: batch=%d", batch);
+    return hermes_fft3_inverse_yx(ctx, J, batch, (float2*)data);
+}
+
 int hermes_fft3_tile_choice(hermes_ctx* ctx, int J, int inverse, int* tw_out, int* threads_out) {
     HERMES_REQUIRE(ctx && tw_out && threads_out, "hermes_fft3_tile_choice: null argument");
     HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_fft3_tile_choice: J=%d outside 3..10", J);

@@ -1025,12 +1025,12 @@ int hermes_multipole_window_ifft_f32(hermes_ctx* ctx, int J, const void* H, cons
     return hermes_multipole_window_ifft_multi_f32(ctx, J, H, P, radii, n_shell, l, &m, 1, scale, out);
 }
 
-int hermes_multipole_window_ifft_multi_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
-                                           const double* radii, int n_shell, int l, const int* ms_host, int n_m,
-                                           double scale, void* out) {
-    HERMES_REQUIRE(ctx && H && P && radii && ms_host && out, "hermes_multipole_window_ifft_multi_f32: null argument");
-    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_window_ifft_multi_f32: J=%d outside 3..10", J);
-    HERMES_REQUIRE(n_m >= 1 && n_m <= MAX_MP, "hermes_multipole_window_ifft_multi_f32: n_m=%d outside 1..%d", n_m,
+int hermes_multipole_window_zpass_multi_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
+                                            const double* radii, int n_shell, int l, const int* ms_host, int n_m,
+                                            double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && radii && ms_host && out, "hermes_multipole_window_zpass_multi_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_window_zpass_multi_f32: J=%d outside 3..10", J);
+    HERMES_REQUIRE(n_m >= 1 && n_m <= MAX_MP, "hermes_multipole_window_zpass_multi_f32: n_m=%d outside 1..%d", n_m,
                    MAX_MP);
     ApplyArgs A;
     float* Pf = nullptr;
@@ -1081,7 +1081,15 @@ int hermes_multipole_window_ifft_multi_f32(hermes_ctx* ctx, int J, const void* H
     }
 #undef HERMES_MPW
     HERMES_LAUNCH_CHECK();
-    return hermes_fft3_inverse_yx(ctx, J, n_m * n_shell, o);
+    return 0;
+}
+
+int hermes_multipole_window_ifft_multi_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
+                                           const double* radii, int n_shell, int l, const int* ms_host, int n_m,
+                                           double scale, void* out) {
+    const int rc = hermes_multipole_window_zpass_multi_f32(ctx, J, H, P, radii, n_shell, l, ms_host, n_m, scale, out);
+    if (rc) return rc;
+    return hermes_fft3_inverse_yx(ctx, J, n_m * n_shell, (float2*)out);
 }
 
 int hermes_reduce_sum(hermes_ctx* ctx, const void* x, int64_t n, int ncomp, int dbl, double* out_host) {

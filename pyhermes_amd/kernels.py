@@ -236,6 +236,30 @@ def multipole_window_ifft_multi_f32(H, P, radii_grid, l, ms, scale, out):
     return out
 
 
+def multipole_window_zpass_multi_f32(H, P, radii_grid, l, ms, scale, out):
+    """First half of multipole_window_ifft_multi_f32: the windows and the z pass only (``out`` holds the
+    fields transformed along z); fft3_inverse_yx_f32 completes them, possibly on another stream."""
+    r = _np_f64(radii_grid)
+    S = r.size
+    ms = [int(m) for m in ms]
+    L = _check_multipole_inputs(H, P, len(ms) * S, out)
+    arr = (ctypes.c_int * len(ms))(*ms)
+    _lib.call("hermes_multipole_window_zpass_multi_f32", _lib.context(), _J_of(L), ptr(H), ptr(P),
+              r.ctypes.data_as(_DP), S, int(l), arr, len(ms), float(scale), ptr(out))
+    return out
+
+
+def fft3_inverse_yx_f32(x):
+    """The y and x inverse passes (unnormalised) of a contiguous complex64 (.., L, L, L) batch whose z
+    lines are already transformed."""
+    if x.dtype != torch.complex64 or not x.is_contiguous() or x.ndim < 3:
+        raise ValueError("fft3_inverse_yx_f32 needs a contiguous complex64 (.., L, L, L) tensor")
+    L = x.shape[-1]
+    batch = int(np.prod(x.shape[:-3])) if x.ndim > 3 else 1
+    _lib.call("hermes_fft3_inverse_yx_f32", _lib.context(), _J_of(L), batch, ptr(x))
+    return x
+
+
 def reduce_sum(x):
     """Deterministic FP64 sum of a real or complex tensor."""
     out = np.zeros(2, dtype=np.float64)

@@ -117,8 +117,11 @@ def _shell_blocks(n, size=16):
     return [(i, min(n, i + size)) for i in range(0, n, size)]
 
 
-def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min, l_max, timer=None):
-    """All (r12, r13) shell pairs: returns (n12, n13, n_l) float64.  d*: (L,L,L) float32 device tensors."""
+def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min, l_max, timer=None,
+                              schedule="serial"):
+    """All (r12, r13) shell pairs: returns (n12, n13, n_l) float64.  d*: (L,L,L) float32 device tensors.
+    ``schedule`` picks the stream schedule of the auto case with at most 16 shells (see
+    _ddd_auto_pipelined): "serial", "split" or "overlap"."""
     L, J = geo.L, geo.J
     tm = timer if timer is not None else _NoTimer()
     N = L ** 3
@@ -132,7 +135,7 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     n_l = l_max - l_min + 1
     result = np.zeros((r12.size, r13.size, n_l), dtype=np.float64)
     if auto and r12.size <= 16:
-        return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm)
+        return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm, schedule=schedule)
     blocks12 = _shell_blocks(r12.size)
     blocks13 = _shell_blocks(r13.size)
     U = torch.empty((r12.size, L, L, L), dtype=torch.complex64, device=d1.device)
@@ -191,15 +194,24 @@ def _window_runs(group):
     return runs
 
 
-def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, overlap=False):
+def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, schedule="serial"):
     """Auto case with S <= 16 shells.  Each iteration takes G = max(16 // S, 2) consecutive (l, m) terms:
     window + inverse-FFT launches over runs of one l, two orders at a time (they share the radial factors),
     fill a buffer of G * S shell fields, and Grams over chunks of 16 // S terms follow (the Gram's MFMA tile is
     16 shells wide; only each term's diagonal S x S block is kept).  The Gram sums stay on the device until
-    the end.  overlap=True runs the windows of iteration k + 1 on the caller's stream while the Grams of
-    iteration k run on a second stream (two buffers).  Measured on the headline workload the two stages share
-    the SIMDs (both are issue-bound), so the overlap gains ~3% and doubles each Gram launch's duration; it is
-    off by default."""
+    the end.  Schedules (two buffers, a second stream, events between them, except "serial"):
+      "serial":  everything in order on the caller's stream (the default: neither other schedule gains on
+                 the headline workload).
+      "split":   the windows and z pass of iteration k + 1 run on the caller's stream while the y and x passes
+                 and then the Grams of iteration k run on the second stream.  The VALU-bound window kernel
+                 and the memory-bound passes do not share CUs well: the passes slow down by about the
+                 window kernel's time (46.95 against 47.25 sets/s).
+      "overlap": whole windows + transforms of iteration k + 1 beside the Grams of iteration k (both stages
+                 are then ALU-bound; measured at ~3% over the serial schedule before later kernel changes)."""
+    if schedule not in ("split", "serial", "overlap"):
+        raise ValueError(f"unknown schedule {schedule!r}")
+    overlap = schedule == "overlap"
+    split = schedule == "split"
     L = geo.L
     N = L ** 3
     S = r.size
@@ -209,31 +221,48 @@ def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, overlap=False):
     iters = [lm[i:i + G] for i in range(0, len(lm), G)]
     chunks = [[it[j:j + pack] for j in range(0, len(it), pack)] for it in iters]
     n_blocks = sum(len(c) for c in chunks)
-    nbuf = min(2, len(iters)) if overlap else 1
+    nbuf = min(2, len(iters)) if (overlap or split) else 1
     bufs = [torch.empty((G * S, L, L, L), dtype=torch.complex64, device=d1.device) for _ in range(nbuf)]
     blocks = torch.zeros((n_blocks, 256), dtype=torch.float64, device=d1.device)
     main = torch.cuda.current_stream(d1.device)
-    side = _side_stream(d1.device) if overlap else main
+    side = _side_stream(d1.device) if (overlap or split) else main
     side.wait_stream(main)  # d1, H and the buffers are ready on the caller's stream
     gram_done = []
     owners = []
     for k, group in enumerate(iters):
         U = bufs[k % len(bufs)]
-        if overlap and k >= len(bufs):
-            main.wait_event(gram_done[k - len(bufs)])
-        t = tm.start("window_ifft")
+        if (overlap or split) and k >= len(bufs):
+            main.wait_event(gram_done[k - len(bufs)])  # the buffer's previous Grams are done
+        t = tm.start("window_z" if split else "window_ifft")
         pos = 0
         for l, ms in _window_runs(group):
-            kernels.multipole_window_ifft_multi_f32(H, geo.P, r, l, ms, 1.0 / N,
-                                                     U[pos * S:(pos + len(ms)) * S])
+            win = kernels.multipole_window_zpass_multi_f32 if split else kernels.multipole_window_ifft_multi_f32
+            win(H, geo.P, r, l, ms, 1.0 / N, U[pos * S:(pos + len(ms)) * S])
             pos += len(ms)
         tm.stop(t)
+        n_fields = pos * S
         grams = []
         pos = 0
         for chunk in chunks[k]:
             grams.append((U[pos * S:(pos + len(chunk)) * S], len(owners)))
             owners.append(chunk)
             pos += len(chunk)
+        if split:
+            z_done = torch.cuda.Event()
+            z_done.record(main)
+            with torch.cuda.stream(side):
+                side.wait_event(z_done)
+                t = tm.start("yx_passes")
+                kernels.fft3_inverse_yx_f32(U[:n_fields])
+                tm.stop(t)
+                t = tm.start("gram")
+                for Uc, b in grams:
+                    kernels.multipole_gram_f32_dev(d1, Uc, geo.Q, geo.lam, blocks[b])
+                tm.stop(t)
+                ev = torch.cuda.Event()
+                ev.record(side)
+                gram_done.append(ev)
+            continue
         if not overlap:
             t = tm.start("gram")
             for Uc, b in grams:
