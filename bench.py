@@ -654,7 +654,8 @@ def main():
     win_key = "window_z" if "window_z" in summ else "window_ifft"
     win_calls = summ[win_key][1] / args.steps
     win_ms = (summ[win_key][0] + (summ["yx_passes"][0] if "yx_passes" in summ else 0.0)) / args.steps
-    win_bytes = win_calls * n_modes * 8 + n_mterms * (S * N * 8 + 2 * (2 * S * N * 8))
+    n_fields = multipole.fields_per_set(0, lmax, L, S, args.schedule)  # m = 0 terms packed in pairs
+    win_bytes = win_calls * n_modes * 8 + n_fields * (S * N * 8 + 2 * (2 * S * N * 8))
     if args.copy_probe_mib > 0:
         n_el = args.copy_probe_mib * (1 << 20) // 4
         src = torch.ones(n_el, dtype=torch.float32, device="cuda")
@@ -688,7 +689,8 @@ def main():
         "stages_ms_per_step": stages,
         "projection_ms": proj_ms,
         "projection_hbm_frac": ((args.particles * 32 + N * 4) / (proj_ms * 1e-3) / 1e9) / HBM_PEAK_GBS,
-        "window_ifft": {"ms_per_step": win_ms, "calls_per_step": win_calls, "bytes_per_step": win_bytes,
+        "window_ifft": {"ms_per_step": win_ms, "calls_per_step": win_calls, "field_groups_per_step": n_fields,
+                        "bytes_per_step": win_bytes,
                         "hbm_frac": win_bytes / (win_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
     }
     line.update(out)

@@ -195,6 +195,14 @@ int hermes_multipole_window_ifft_multi_f32(hermes_ctx *ctx, int J, const void *H
                                            const double *radii, int n_shell, int l, const int *ms_host, int n_m,
                                            double scale, void *out);
 
+/* The (l, 0) and (l + 1, 0) windows of every shell packed into one complex field per shell (J <= 9):
+ * the window of the even degree is real and even in k and that of the odd degree real and odd, so
+ * their fields are the real and the imaginary part of the result (a Gram with separate real and
+ * imaginary sums, hermes_multipole_gram_f32_dev_ri, takes them apart).  out: n_shell fields.
+ * No counterpart. */
+int hermes_multipole_window_ifft_pack0_f32(hermes_ctx *ctx, int J, const void *H, const double *P,
+                                           const double *radii, int n_shell, int l, double scale, void *out);
+
 /* The two halves of hermes_multipole_window_ifft_multi_f32, so that they can run on different
  * streams: the windows and the z pass (out then holds the fields transformed along z only), and the
  * y and x inverse passes of a batch of (L, L, L) complex64 grids already transformed along z.  The
@@ -235,6 +243,13 @@ int hermes_multipole_gram_f32(hermes_ctx *ctx, int J, const float *d1, const voi
 int hermes_multipole_gram_f32_dev(hermes_ctx *ctx, int J, const float *d1, const void *U, int n_u, const void *V,
                                   int n_v, int v_conj, const double *Q3x3_host, const double *lam3_host,
                                   double *out_dev);
+
+/* The auto Gram (V = U) with the products of the real parts and of the imaginary parts summed apart:
+ * out_dev (device, 512 doubles) holds the real block then the imaginary block, each in the 16 x 16
+ * layout of hermes_multipole_gram_f32_dev.  For fields that pack two real fields (see
+ * hermes_multipole_window_ifft_pack0_f32).  No counterpart. */
+int hermes_multipole_gram_f32_dev_ri(hermes_ctx *ctx, int J, const float *d1, const void *U, int n_u,
+                                     const double *Q3x3_host, const double *lam3_host, double *out_dev);
 
 /* Checks the lane layout of the 16x16x4 FP32 matrix instruction used above;
  * max_abs_err = 0 on a conforming device.  (No counterpart in the reference.) */

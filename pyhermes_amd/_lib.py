@@ -75,6 +75,9 @@ _SIGNATURES.update({
                                   _dpp],
     "hermes_multipole_gram_f32_dev": [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_int, c_int, _dpp,
                                       _dpp, c_void_p],
+    "hermes_multipole_gram_f32_dev_ri": [c_void_p, c_int, c_void_p, c_void_p, c_int, _dpp, _dpp, c_void_p],
+    "hermes_multipole_window_ifft_pack0_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_double,
+                                               c_void_p],
     "hermes_pair_plan_create": [c_void_p, c_int, c_int, ctypes.POINTER(WindowDesc), c_int, ctypes.POINTER(c_void_p)],
     "hermes_pair_plan_destroy": [c_void_p],
     "hermes_pair_reduce_planned": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -94,7 +94,8 @@ __global__ __launch_bounds__(SUM_THREADS) void k_summand_f64(int L, const double
 constexpr int RED_THREADS = 256;
 
 __global__ __launch_bounds__(RED_THREADS) void k_sum_rows(const double* __restrict__ in, int64_t nb, int width,
-                                                          int64_t rows_per_block, double* __restrict__ out) {
+                                                          int64_t stride, int64_t rows_per_block,
+                                                          double* __restrict__ out) {
     __shared__ double sh[RED_THREADS];
     const int phases = RED_THREADS / width;
     const int j = threadIdx.x % width, ph = threadIdx.x / width;
@@ -102,7 +103,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_sum_rows(const double* __restri
     const int64_t r1 = r0 + rows_per_block < nb ? r0 + rows_per_block : nb;
     double t = 0.0;
     if (ph < phases)
-        for (int64_t r = r0 + ph; r < r1; r += phases) t += in[r * width + j];
+        for (int64_t r = r0 + ph; r < r1; r += phases) t += in[r * stride + j];
     sh[threadIdx.x] = t;
     __syncthreads();
     if (threadIdx.x < width) {
@@ -115,12 +116,15 @@ __global__ __launch_bounds__(RED_THREADS) void k_sum_rows(const double* __restri
 constexpr int RED_MAX_BLOCKS = 1024;
 
 // Two launches: <= RED_MAX_BLOCKS partial rows, then one row.  tmp holds RED_MAX_BLOCKS * width doubles.
-static void sum_rows(hipStream_t st, const double* in, int64_t nb, int width, double* tmp, double* out) {
+// Column sums of nb rows (width <= RED_THREADS columns, rows stride apart) in a fixed order.
+static void sum_rows(hipStream_t st, const double* in, int64_t nb, int width, double* tmp, double* out,
+                     int64_t stride = 0) {
+    if (stride == 0) stride = width;
     int64_t rpb = (nb + RED_MAX_BLOCKS - 1) / RED_MAX_BLOCKS;
     if (rpb < 16) rpb = 16;
     const int64_t nblk = (nb + rpb - 1) / rpb;
-    k_sum_rows<<<(unsigned)nblk, RED_THREADS, 0, st>>>(in, nb, width, rpb, tmp);
-    k_sum_rows<<<1, RED_THREADS, 0, st>>>(tmp, nblk, width, nblk, out);
+    k_sum_rows<<<(unsigned)nblk, RED_THREADS, 0, st>>>(in, nb, width, stride, rpb, tmp);
+    k_sum_rows<<<1, RED_THREADS, 0, st>>>(tmp, nblk, width, width, nblk, out);
 }
 
 // ---------------------------------------------------------------- batched FP32 MFMA Gram
@@ -313,12 +317,18 @@ static_assert(2 * (AUTO_TILE + AUTO_TBUF) * 8 >= AUTO_WAVES * 256 * 8, "reductio
 // scaled by sqrt|lam|, the filtered slot (b, c) carries sqrt|lam_b lam_c| and its weight's sign is
 // -1 exactly when one of b, c is below NNEG: those slots accumulate into a second pair of MFMA
 // accumulators that is subtracted at each flush.
-template <int NNEG>
+//
+// SPLIT: the products of the real parts and those of the imaginary parts are summed apart (rows of 512
+// partials: real block, then imaginary block; the imaginary sums accumulate in LDS, keeping the
+// register count within four waves per SIMD), for fields that pack two real fields as real and
+// imaginary part.
+template <int NNEG, bool SPLIT = false>
 __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __restrict__ d1,
                                                                const float2* __restrict__ U, GramArgs g,
                                                                double* __restrict__ partial) {
     // raw buffers, T buffers; the final reduction reuses the same memory
     __shared__ __align__(16) float2 smem[2 * AUTO_TILE + 2 * AUTO_TBUF];
+    __shared__ double redi[SPLIT ? AUTO_WAVES * 256 : 1];  // imaginary sums per wave (SPLIT)
     auto tile = [&](int buf) -> float2* { return smem + buf * AUTO_TILE; };
     auto tb = [&](int buf) -> float2* { return smem + 2 * AUTO_TILE + buf * AUTO_TBUF; };
     const int L = g.L, m = L - 1;
@@ -419,6 +429,11 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     };
 
     double acc_slot[4] = {0.0, 0.0, 0.0, 0.0};  // this lane's G entries 4 (lane / 16) + j, column lane % 16
+    double* islot = redi + (SPLIT ? wave * 256 + (4 * (lane / 16)) * 16 + (lane % 16) : 0);
+    if (SPLIT) {
+#pragma unroll
+        for (int j = 0; j < 4; ++j) islot[j * 16] = 0.0;
+    }
 
     float2 Y0[9], Y1[9], Y2[9];
     // warm-up (the steps x0-3 .. x0-1 without MFMAs): Y0 = plane x0, Y1 = plane x0+1, T buffer
@@ -497,7 +512,12 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         AUTO_STEP(Y2, Y0, Y1, dC, dA, dB)
 #pragma unroll
         for (int j = 0; j < 4; ++j) {
-            acc_slot[j] += ((double)accr[j] + (double)acci[j]) - ((double)nacr[j] + (double)naci[j]);
+            if (SPLIT) {
+                acc_slot[j] += (double)accr[j] - (double)nacr[j];
+                islot[j * 16] += (double)acci[j] - (double)naci[j];
+            } else {
+                acc_slot[j] += ((double)accr[j] + (double)acci[j]) - ((double)nacr[j] + (double)naci[j]);
+            }
             accr[j] = 0.f;
             acci[j] = 0.f;
             nacr[j] = 0.f;
@@ -512,11 +532,18 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
 #pragma unroll
     for (int j = 0; j < 4; ++j) red[wave * 256 + (4 * (lane / 16) + j) * 16 + (lane % 16)] = acc_slot[j];
     __syncthreads();
+    constexpr int ROW = SPLIT ? 512 : 256;
     for (int i = tid; i < 256; i += 64 * AUTO_WAVES) {
         double t = 0.0;
 #pragma unroll
         for (int w = 0; w < AUTO_WAVES; ++w) t += red[w * 256 + i];
-        partial[(int64_t)blockIdx.x * 256 + i] = t;
+        partial[(int64_t)blockIdx.x * ROW + i] = t;
+        if (SPLIT) {
+            double u = 0.0;
+#pragma unroll
+            for (int w = 0; w < AUTO_WAVES; ++w) u += redi[w * 256 + i];
+            partial[(int64_t)blockIdx.x * ROW + 256 + i] = u;
+        }
     }
 }
 
@@ -574,7 +601,8 @@ static int gram_flush(hermes_ctx* ctx) {
 // Launches the Gram kernel and the fixed-order row reduction; the 16 x 16 raw sums land in
 // out_dev (256 doubles; row i, column j at i * 16 + j).  No host synchronisation.
 static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u, const void* V, int n_v,
-                       int v_conj, const double* Q3x3_host, const double* lam3_host, double* out_dev, GramArgs* gout) {
+                       int v_conj, const double* Q3x3_host, const double* lam3_host, double* out_dev, GramArgs* gout,
+                       bool split = false) {
     HERMES_REQUIRE(ctx && d1 && U && Q3x3_host && lam3_host && out_dev, "hermes_multipole_gram_f32: null argument");
     HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_gram_f32: J=%d unsupported (needs L >= 8)", J);
     HERMES_REQUIRE(n_u >= 1 && n_u <= 16, "n_u=%d outside 1..16", n_u);
@@ -616,8 +644,10 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
     HERMES_REQUIRE(g.cross || J >= 4, "hermes_multipole_gram_f32: J=%d, the auto Gram needs J >= 4", J);
     const int nblocks = g.cross ? (L / 4) * (L / GRAM_WAVES) * (L / g.xseg)
                                 : (L / AUTO_TZ) * (L / AUTO_TY) * (L / g.xseg);
+    HERMES_REQUIRE(!split || !g.cross, "hermes_multipole_gram_f32: split sums are for the auto Gram");
+    const int row = split ? 512 : 256;
     double* ws = (double*)hermes_workspace(
-        ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * 256 + 256 + (size_t)RED_MAX_BLOCKS * 256));
+        ctx, WS_CONTRACT, sizeof(double) * ((size_t)nblocks * row + row + (size_t)RED_MAX_BLOCKS * row));
     HERMES_REQUIRE(ws, "%s", hermes_last_error());
     const bool timed = ctx->time_gram && !ctx->capturing;
     size_t slot = 0;
@@ -634,18 +664,24 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
     if (g.cross)
         k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     else
-        switch (nneg) {
+        switch (nneg + (split ? 4 : 0)) {
             case 0: k_gram_auto<0><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
             case 1: k_gram_auto<1><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
             case 2: k_gram_auto<2><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            default: k_gram_auto<3><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 3: k_gram_auto<3><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 4: k_gram_auto<0, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 5: k_gram_auto<1, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 6: k_gram_auto<2, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            default: k_gram_auto<3, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
         }
     HERMES_LAUNCH_CHECK();
     if (timed) {
         HERMES_CHECK_HIP(hipEventRecord(ctx->gram_ev[slot + 1], ctx->stream));
         ctx->gram_pending += 1;
     }
-    sum_rows(ctx->stream, ws, nblocks, 256, ws + (size_t)nblocks * 256 + 256, out_dev);
+    double* tmp = ws + (size_t)nblocks * row + row;
+    sum_rows(ctx->stream, ws, nblocks, 256, tmp, out_dev, row);
+    if (split) sum_rows(ctx->stream, ws + 256, nblocks, 256, tmp, out_dev + 256, row);
     HERMES_LAUNCH_CHECK();
     if (gout) *gout = g;
     return 0;
@@ -672,6 +708,11 @@ int hermes_multipole_gram_f32_dev(hermes_ctx* ctx, int J, const float* d1, const
                                   int n_v, int v_conj, const double* Q3x3_host, const double* lam3_host,
                                   double* out_dev) {
     return gram_launch(ctx, J, d1, U, n_u, V, n_v, v_conj, Q3x3_host, lam3_host, out_dev, nullptr);
+}
+
+int hermes_multipole_gram_f32_dev_ri(hermes_ctx* ctx, int J, const float* d1, const void* U, int n_u,
+                                     const double* Q3x3_host, const double* lam3_host, double* out_dev) {
+    return gram_launch(ctx, J, d1, U, n_u, nullptr, 0, 0, Q3x3_host, lam3_host, out_dev, nullptr, true);
 }
 
 int hermes_profile_gram(hermes_ctx* ctx, int enable, double* total_ms_out, int64_t* calls_out) {

@@ -220,6 +220,34 @@ __device__ inline float jl_f32(int l, double t) {
     return ql / dfact * sum;
 }
 
+// j_l and j_{l+1} of 2 pi t together: the upward recurrence of jl_f32 run one step further where
+// it serves both (q >= l + 2), otherwise each from jl_f32; the same values as two jl_f32 calls.
+__device__ inline void jl_pair_f32(int l, double t, float& jl, float& jl1) {
+    const float q = (float)(2.0 * HERMES_PI * t);
+    if (t != 0.0 && q >= (float)l + 2.0f) {
+        const float fr = (float)(t - floor(t));
+        const float s = __builtin_amdgcn_sinf(fr), c = __builtin_amdgcn_cosf(fr);
+        const float iq = __builtin_amdgcn_rcpf(q);
+        float jm = s * iq;
+        float jc = (jm - c) * iq;
+        if (l == 0) {
+            jl = jm;
+            jl1 = jc;
+            return;
+        }
+        for (int n = 1; n < l; ++n) {
+            float jn = (float)(2 * n + 1) * iq * jc - jm;
+            jm = jc;
+            jc = jn;
+        }
+        jl = jc;
+        jl1 = (float)(2 * l + 1) * iq * jc - jm;
+        return;
+    }
+    jl = jl_f32(l, t);
+    jl1 = jl_f32(l + 1, t);
+}
+
 // Y_l^m(a-hat) for integer lattice vector a (as floats), m >= 0; cnorm = N_lm (-1)^m (2m-1)!!.
 __device__ inline float2 ylm_f32(int l, int m, float ax, float ay, float az, float cnorm) {
     float r2 = ax * ax + ay * ay + az * az;

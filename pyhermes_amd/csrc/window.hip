@@ -372,6 +372,7 @@ struct ApplyArgs {
 constexpr int MAX_MP = 2;
 struct MultiM {
     int n;                  // 1..MAX_MP
+    int l[MAX_MP];          // degree of each order (one l, or l and l + 1 for the packed m = 0 pair)
     int m[MAX_MP];
     float cnorm[MAX_MP];    // N_l|m| (-1)^|m| (2|m|-1)!!
     float sx[MAX_MP], sz[MAX_MP];
@@ -466,14 +467,20 @@ constexpr int mpwin_threads() {
     return 64 * (((1 << LOG2L) / 2 + 63) / 64);
 }
 
-template <int LOG2L, int MP>
+// PACK (MP = 2): the orders are (l, 0) and (l + 1, 0), whose windows are real with opposite parity
+// under k -> -k, so the field of the even degree is real and that of the odd one imaginary: their
+// windows are summed into one set of four lines (one field per shell, half the transforms), and the
+// real and imaginary products of the Gram separate them again.
+template <int LOG2L, int MP, bool PACK = false>
 __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
     const float2* __restrict__ H, const float* __restrict__ Pf, ApplyArgs A, MultiM M,
     const float2* __restrict__ twg, float2* __restrict__ out) {
+    static_assert(!PACK || MP == 2, "packing needs two orders");
     constexpr int L = 1 << LOG2L, Lh = L / 2 + 1;
     constexpr int NT = mpwin_threads<LOG2L>();
     constexpr bool NYQ = Lh > NT;  // z = L/2 by helper lanes 0..7
-    using F = hermes_fft::LdsFft<LOG2L, 4 * MP, NT, 1, false>;
+    constexpr int NLINE = PACK ? 4 : 4 * MP;  // lines transformed per shell
+    using F = hermes_fft::LdsFft<LOG2L, NLINE, NT, 1, false>;
     __shared__ float2 lds[F::ELEMS];
     __shared__ float2 tw[L];
     __shared__ float2 nyq[8][MP];
@@ -492,7 +499,7 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
 #pragma unroll
         for (int mi = 0; mi < MP; ++mi) {
             const int m = M.m[mi], am = m < 0 ? -m : m;
-            float2 yy = ylm_f32(A.l, am, (float)ax, (float)ay, (float)az, M.cnorm[mi]);
+            float2 yy = ylm_f32(M.l[mi], am, (float)ax, (float)ay, (float)az, M.cnorm[mi]);
             if (m < 0) {  // Y_{l,-m} = (-1)^m conj(Y_{l,m})
                 const float sg = (am & 1) ? -1.0f : 1.0f;
                 yy = make_float2(sg * yy.x, -sg * yy.y);
@@ -546,6 +553,37 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
             lds[F::addr(r0 + 3, zm)] = cmulf(conjf2(hh[0]), make_float2(sz * sx * k.x, sz * sx * k.y));
         }
     };
+    // packed pair: the sum of the two orders' mirrored window values on each of the four lines (the
+    // products with H distribute over the sum)
+    auto put_pack = [&](int zz, const float2 hh[4], float2 K0, float2 K1) {
+        const float2 k0 = make_float2(K0.x * A.scale, K0.y * A.scale), kc0 = conjf2(k0);
+        const float2 k1 = make_float2(K1.x * A.scale, K1.y * A.scale), kc1 = conjf2(k1);
+        const float sx0 = M.sx[0], sz0 = M.sz[0], sx1 = M.sx[1], sz1 = M.sz[1];
+        auto mix = [](float a0, float2 u0, float a1, float2 u1) {
+            return make_float2(a0 * u0.x + a1 * u1.x, a0 * u0.y + a1 * u1.y);
+        };
+        lds[F::addr(0, zz)] = cmulf(hh[0], mix(1.0f, k0, 1.0f, k1));
+        lds[F::addr(1, zz)] = cmulf(hh[1], mix(sx0, kc0, sx1, kc1));
+        lds[F::addr(2, zz)] = cmulf(hh[2], mix(1.0f, kc0, 1.0f, kc1));
+        lds[F::addr(3, zz)] = cmulf(hh[3], mix(sx0, k0, sx1, k1));
+        const int zm = (L - zz) & (L - 1);
+        if (zm != zz) {
+            lds[F::addr(0, zm)] = cmulf(conjf2(hh[3]), mix(sz0, k0, sz1, k1));
+            lds[F::addr(1, zm)] = cmulf(conjf2(hh[2]), mix(sz0 * sx0, kc0, sz1 * sx1, kc1));
+            lds[F::addr(2, zm)] = cmulf(conjf2(hh[1]), mix(sz0, kc0, sz1, kc1));
+            lds[F::addr(3, zm)] = cmulf(conjf2(hh[0]), mix(sz0 * sx0, k0, sz1 * sx1, k1));
+        }
+    };
+    // radial factor of alias argument t for each order (one l, or l and l + 1 when packed)
+    auto radial = [&](double t, float* j) {
+        if (PACK) {
+            jl_pair_f32(M.l[0], t, j[0], j[MP - 1]);
+        } else {
+            const float jj = jl_f32(M.l[0], t);
+#pragma unroll
+            for (int mi = 0; mi < MP; ++mi) j[mi] = jj;
+        }
+    };
     for (int s = 0; s < A.n_shell; ++s) {
         const double rs = A.radii_over_L[s];
         if (zact) {
@@ -555,30 +593,53 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 if (!((live >> a) & 1u)) continue;
-                const float j = jl_f32(A.l, nrm[a] * rs);
+                float j[MP];
+                radial(nrm[a] * rs, j);
 #pragma unroll
                 for (int mi = 0; mi < MP; ++mi) {
-                    K[mi].x += j * wy[a][mi].x;
-                    K[mi].y += j * wy[a][mi].y;
+                    K[mi].x += j[mi] * wy[a][mi].x;
+                    K[mi].y += j[mi] * wy[a][mi].y;
                 }
             }
+            if (PACK) {
+                put_pack(z, h, K[0], K[MP - 1]);
+            } else {
 #pragma unroll
-            for (int mi = 0; mi < MP; ++mi) put(mi, z, h, K[mi]);
+                for (int mi = 0; mi < MP; ++mi) put(mi, z, h, K[mi]);
+            }
         }
         if (NYQ) {
             if (tid < 8) {
-                const float j = live_n ? jl_f32(A.l, nrm_n * rs) : 0.0f;
+                float j[MP];
+                if (live_n) {
+                    radial(nrm_n * rs, j);
+                } else {
 #pragma unroll
-                for (int mi = 0; mi < MP; ++mi) nyq[tid][mi] = make_float2(j * wn[mi].x, j * wn[mi].y);
+                    for (int mi = 0; mi < MP; ++mi) j[mi] = 0.0f;
+                }
+#pragma unroll
+                for (int mi = 0; mi < MP; ++mi) nyq[tid][mi] = make_float2(j[mi] * wn[mi].x, j[mi] * wn[mi].y);
             }
             __syncthreads();
-            if (tid < MP) {
-                float2 K = make_float2(0.0f, 0.0f);
+            if (tid < (PACK ? 1 : MP)) {
+                float2 K[MP];
+#pragma unroll
+                for (int mi = 0; mi < MP; ++mi) K[mi] = make_float2(0.0f, 0.0f);
                 for (int a = 0; a < 8; ++a) {
-                    K.x += nyq[a][tid].x;
-                    K.y += nyq[a][tid].y;
+#pragma unroll
+                    for (int mi = 0; mi < MP; ++mi) {
+                        if (!PACK && mi != tid) continue;
+                        K[mi].x += nyq[a][mi].x;
+                        K[mi].y += nyq[a][mi].y;
+                    }
                 }
-                put(tid, L / 2, hn, K);
+                if (PACK) {
+                    put_pack(L / 2, hn, K[0], K[MP - 1]);
+                } else {
+#pragma unroll
+                    for (int mi = 0; mi < MP; ++mi)
+                        if (mi == tid) put(mi, L / 2, hn, K[mi]);
+                }
             }
         }
         __syncthreads();
@@ -587,7 +648,7 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
         // line's base address are uniform, and a line's LDS reads are all issued before its stores
         constexpr int EPT = (L + NT - 1) / NT;  // elements per thread and line
 #pragma unroll
-        for (int line = 0; line < 4 * MP; ++line) {
+        for (int line = 0; line < NLINE; ++line) {
             const int mi = line >> 2, row = line & 3;
             const bool dx = row & 1, dy = row & 2;
             if ((dx && xm == x) || (dy && ym == y)) continue;
@@ -1041,6 +1102,7 @@ int hermes_multipole_window_zpass_multi_f32(hermes_ctx* ctx, int J, const void* 
     for (int i = 0; i < n_m; ++i) {
         const int m = ms_host[i], am = m < 0 ? -m : m;
         HERMES_REQUIRE(m >= -l && m <= l, "invalid multipole (l=%d, m=%d)", l, m);
+        M.l[i] = l;
         M.m[i] = m;
         M.cnorm[i] = (float)hermes_ylm_cnorm(l, am);
         M.sx[i] = (am & 1) ? -1.0f : 1.0f;
@@ -1069,6 +1131,7 @@ int hermes_multipole_window_zpass_multi_f32(hermes_ctx* ctx, int J, const void* 
             for (int i = 0; i < n_m; ++i) {
                 MultiM Mi{};
                 Mi.n = 1;
+                Mi.l[0] = M.l[i];
                 Mi.m[0] = M.m[i];
                 Mi.cnorm[0] = M.cnorm[i];
                 Mi.sx[0] = M.sx[i];
@@ -1082,6 +1145,44 @@ int hermes_multipole_window_zpass_multi_f32(hermes_ctx* ctx, int J, const void* 
 #undef HERMES_MPW
     HERMES_LAUNCH_CHECK();
     return 0;
+}
+
+int hermes_multipole_window_ifft_pack0_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
+                                           const double* radii, int n_shell, int l, double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_window_ifft_pack0_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 9, "hermes_multipole_window_ifft_pack0_f32: J=%d outside 3..9", J);
+    HERMES_REQUIRE(l >= 0 && l < 40, "hermes_multipole_window_ifft_pack0_f32: l=%d", l);
+    ApplyArgs A;
+    float* Pf = nullptr;
+    int rc = make_apply_args(ctx, J, P, radii, n_shell, l, 0, scale, &A, &Pf);
+    if (rc) return rc;
+    MultiM M{};
+    M.n = 2;
+    for (int i = 0; i < 2; ++i) {
+        M.l[i] = l + i;
+        M.m[i] = 0;
+        M.cnorm[i] = (float)hermes_ylm_cnorm(l + i, 0);
+        M.sx[i] = 1.0f;
+        M.sz[i] = ((l + i) & 1) ? -1.0f : 1.0f;
+    }
+    const float2* twg = hermes_twiddles(ctx, J, 1);
+    HERMES_REQUIRE(twg, "%s", hermes_last_error());
+    const unsigned nblk = (unsigned)(((1 << J) / 2 + 1) * ((1 << J) / 2 + 1));
+    const float2* h = (const float2*)H;
+    float2* o = (float2*)out;
+#define HERMES_MPWP(LG) k_mpwin_zfft_multi<LG, 2, true><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o);
+    switch (J) {
+        case 3: HERMES_MPWP(3) break;
+        case 4: HERMES_MPWP(4) break;
+        case 5: HERMES_MPWP(5) break;
+        case 6: HERMES_MPWP(6) break;
+        case 7: HERMES_MPWP(7) break;
+        case 8: HERMES_MPWP(8) break;
+        default: HERMES_MPWP(9) break;
+    }
+#undef HERMES_MPWP
+    HERMES_LAUNCH_CHECK();
+    return hermes_fft3_inverse_yx(ctx, J, n_shell, o);
 }
 
 int hermes_multipole_window_ifft_multi_f32(hermes_ctx* ctx, int J, const void* H, const double* P,

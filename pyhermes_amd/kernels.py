@@ -309,6 +309,30 @@ def multipole_gram_f32_dev(d1, U, Q, lam, out_dev, V=None, v_conj=False):
     return out_dev
 
 
+def multipole_gram_f32_dev_ri(d1, U, Q, lam, out_dev):
+    """Auto Gram with the real-part and imaginary-part products summed apart: out_dev (512 doubles on the
+    device) holds the real block then the imaginary block; see hermes_multipole_gram_f32_dev_ri."""
+    L = d1.shape[0]
+    if out_dev.dtype != torch.float64 or out_dev.numel() < 512 or not out_dev.is_contiguous():
+        raise ValueError("out_dev must be a contiguous float64 tensor of at least 512 elements")
+    q = _np_f64(Q)
+    lm = _np_f64(lam)
+    _lib.call("hermes_multipole_gram_f32_dev_ri", _lib.context(), _J_of(L), ptr(d1), ptr(U), U.shape[0],
+              q.ctypes.data_as(_DP), lm.ctypes.data_as(_DP), ptr(out_dev))
+    return out_dev
+
+
+def multipole_window_ifft_pack0_f32(H, P, radii_grid, l, scale, out):
+    """(S, L, L, L) complex64: the (l, 0) and (l + 1, 0) windows + inverse FFT of every shell packed into
+    one field per shell; the even degree's field is its real part, the odd degree's its imaginary part."""
+    r = _np_f64(radii_grid)
+    S = r.size
+    L = _check_multipole_inputs(H, P, S, out)
+    _lib.call("hermes_multipole_window_ifft_pack0_f32", _lib.context(), _J_of(L), ptr(H), ptr(P),
+              r.ctypes.data_as(_DP), S, int(l), float(scale), ptr(out))
+    return out
+
+
 def gram_block_to_matrix(block, n_u, n_v=None):
     """(n_u, n_v) Gram sums from a raw block of hermes_multipole_gram_f32_dev (auto when n_v is None)."""
     h = np.asarray(block, dtype=np.float64).reshape(16, 16)

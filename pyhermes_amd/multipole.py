@@ -135,6 +135,8 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     n_l = l_max - l_min + 1
     result = np.zeros((r12.size, r13.size, n_l), dtype=np.float64)
     if auto and r12.size <= 16:
+        if packs_m0(geo.L, r12.size, schedule):
+            return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm)
         return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm, schedule=schedule)
     blocks12 = _shell_blocks(r12.size)
     blocks13 = _shell_blocks(r13.size)
@@ -173,6 +175,81 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
 
 
 _SIDE = {}
+
+
+def packs_m0(L, n_shell, schedule="serial"):
+    """Whether the batched auto path packs the m = 0 terms of consecutive degrees in pairs (serial
+    schedule, 9..16 shells so that each term has its own Gram, L <= 512)."""
+    return schedule == "serial" and 8 < n_shell <= 16 and L <= 512
+
+
+def fields_per_set(l_min, l_max, L, n_shell, schedule="serial"):
+    """Shell-field groups (window + transform + Gram launches) the batched auto path computes."""
+    terms = sum(l + 1 for l in range(l_min, l_max + 1))
+    return terms - ((l_max - l_min + 1) // 2 if packs_m0(L, n_shell, schedule) else 0)
+
+
+def _packed_units(l_min, l_max):
+    """Work units of the packed auto path: the (l, 0) terms of consecutive degrees l_min, l_min + 1, ...
+    in pairs (one field per shell), a last unpaired (l_max, 0), then the orders m >= 1 of each l two at
+    a time (sharing the radial factors)."""
+    units = []
+    l = l_min
+    while l + 1 <= l_max:
+        units.append(("pack0", l, None))
+        l += 2
+    if l == l_max:
+        units.append(("multi", l, [0]))
+    for l in range(max(l_min, 1), l_max + 1):
+        ms = list(range(1, l + 1))
+        for i in range(0, len(ms), 2):
+            units.append(("multi", l, ms[i:i + 2]))
+    return units
+
+
+def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm):
+    """Auto case with 9..16 shells on the serial schedule, the m = 0 terms packed in pairs: the windows of
+    (l, 0) and (l + 1, 0) are real with opposite parity under k -> -k, so one transform yields the even
+    degree's fields as real parts and the odd degree's as imaginary parts, and one Gram with the real and
+    imaginary products summed apart separates them (for l <= 10: 61 field groups per set instead of 66).
+    Every other term is computed as in _ddd_auto_pipelined, one Gram per term."""
+    L = geo.L
+    N = L ** 3
+    S = r.size
+    units = _packed_units(l_min, l_max)
+    U = torch.empty((2 * S, L, L, L), dtype=torch.complex64, device=d1.device)
+    n_blocks = sum(1 if kind == "pack0" else len(ms) for kind, _, ms in units)
+    blocks = torch.zeros((n_blocks, 512), dtype=torch.float64, device=d1.device)
+    owners = []
+    for kind, l, ms in units:
+        t = tm.start("window_ifft")
+        if kind == "pack0":
+            kernels.multipole_window_ifft_pack0_f32(H, geo.P, r, l, 1.0 / N, U[:S])
+        else:
+            kernels.multipole_window_ifft_multi_f32(H, geo.P, r, l, ms, 1.0 / N, U[:len(ms) * S])
+        tm.stop(t)
+        t = tm.start("gram")
+        if kind == "pack0":
+            kernels.multipole_gram_f32_dev_ri(d1, U[:S], geo.Q, geo.lam, blocks[len(owners)])
+            owners.append(("pack0", l, None))
+        else:
+            for i, m in enumerate(ms):
+                kernels.multipole_gram_f32_dev(d1, U[i * S:(i + 1) * S], geo.Q, geo.lam, blocks[len(owners), :256])
+                owners.append(("term", l, m))
+        tm.stop(t)
+    host = blocks.cpu().numpy()
+    v_re = {}
+    for b, (kind, l, m) in enumerate(owners):
+        if kind == "pack0":
+            even, odd = (l, l + 1) if l % 2 == 0 else (l + 1, l)
+            v_re[(even, 0)] = 4.0 * np.pi * kernels.gram_block_to_matrix(host[b, :256], S) / N
+            v_re[(odd, 0)] = -4.0 * np.pi * kernels.gram_block_to_matrix(host[b, 256:], S) / N
+        else:
+            v_re[(l, m)] = 4.0 * np.pi * ((-1.0) ** (l + m)) * kernels.gram_block_to_matrix(host[b, :256], S) / N
+    result = np.zeros((S, S, l_max - l_min + 1), dtype=np.float64)
+    for li, l in enumerate(range(l_min, l_max + 1)):
+        result[:, :, li] = combine_m_terms([v_re[(l, m)] for m in range(l + 1)], l)
+    return result
 
 
 def _side_stream(device):

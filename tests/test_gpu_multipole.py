@@ -143,3 +143,61 @@ def test_schedules_and_split_window_agree(phi):
     torch.cuda.synchronize()
     assert np.array_equal(runs["serial"], runs["split"])
     assert np.array_equal(runs["serial"], runs["overlap"])
+
+
+def test_packed_m0_window_and_split_gram(phi):
+    """The packed (l, 0) + (l + 1, 0) window equals the sum of the two single windows; the split Gram's
+    real and imaginary blocks are the Grams of the real and imaginary parts."""
+    import torch
+    from pyhermes_amd import kernels, multipole
+    L = 32
+    rng = np.random.default_rng(23)
+    delta = rng.normal(size=(L, L, L))
+    d32 = kernels.as_device(delta - delta.mean(), torch.float32)
+    geo = multipole.MultipoleGeometry(phi, L)
+    H = kernels.rfftn(d32)
+    radii = np.array([1.5, 3.0, 4.5, 6.0, 8.0])
+    S = radii.size
+    for l in (0, 3, 6):
+        print(f"packed window l={l}", flush=True)
+        packed = torch.empty((S, L, L, L), dtype=torch.complex64, device=d32.device)
+        kernels.multipole_window_ifft_pack0_f32(H, geo.P, radii, l, 1.0 / L ** 3, packed)
+        a = kernels.multipole_window_ifft_f32(H, geo.P, radii, l, 0, 1.0 / L ** 3)
+        b = kernels.multipole_window_ifft_f32(H, geo.P, radii, l + 1, 0, 1.0 / L ** 3)
+        want = a + b
+        assert float((packed - want).abs().max()) <= 2e-6 * float(want.abs().max())
+        even, odd = (a, b) if l % 2 == 0 else (b, a)
+        # the even degree's field is real, the odd degree's imaginary
+        assert float(even.imag.abs().max()) <= 1e-6 * float(even.abs().max())
+        assert float(odd.real.abs().max()) <= 1e-6 * float(odd.abs().max())
+        blk = torch.zeros(512, dtype=torch.float64, device=d32.device)
+        kernels.multipole_gram_f32_dev_ri(d32, packed, geo.Q, geo.lam, blk)
+        h = blk.cpu().numpy()
+        re_only = torch.complex(packed.real.contiguous(), torch.zeros_like(packed.real))
+        im_only = torch.complex(packed.imag.contiguous(), torch.zeros_like(packed.imag))
+        g_re = kernels.multipole_gram_f32(d32, re_only, geo.Q, geo.lam)
+        g_im = kernels.multipole_gram_f32(d32, im_only, geo.Q, geo.lam)
+        assert np.array_equal(kernels.gram_block_to_matrix(h[:256], S), g_re)
+        assert np.allclose(kernels.gram_block_to_matrix(h[256:], S), g_im, rtol=1e-6, atol=1e-9 * np.abs(g_im).max())
+
+
+def test_packed_pipeline_matches_unpacked_and_fp64(phi):
+    """With 10 shells the batched auto path packs the m = 0 terms; it agrees with the unpacked schedule
+    and with the plain FP64 path."""
+    import torch
+    from pyhermes_amd import kernels, multipole
+    L = 32
+    rng = np.random.default_rng(29)
+    delta = rng.normal(size=(L, L, L))
+    delta -= delta.mean()
+    d32 = kernels.as_device(delta, torch.float32)
+    d64 = kernels.as_device(delta, torch.float64)
+    geo = multipole.MultipoleGeometry(phi, L)
+    radii = np.linspace(1.5, 9.0, 10)
+    lmax = 5
+    assert multipole.packs_m0(L, radii.size)
+    packed = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, lmax)
+    unpacked = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, lmax, schedule="overlap")
+    assert np.linalg.norm(packed - unpacked) <= 1e-5 * np.linalg.norm(unpacked)
+    plain = np.array(multipole.ddd_multipole_plain_f64(geo, d64, d64, d64, radii[2], radii[7], 0, lmax))
+    assert np.linalg.norm(packed[2, 7] - plain) <= 1e-4 * np.linalg.norm(plain)
