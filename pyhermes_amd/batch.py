@@ -61,10 +61,16 @@ class CatalogueBatch:
         if not 1 <= self.shells.size <= 16:
             raise ValueError("CatalogueBatch: 1..16 SH-3PCF shells")
         self.lm = [(l, m) for l in range(self.lmax + 1) for m in range(l + 1)]
-        # the Gram's MFMA tile is 16 shells wide: pack 16 // S (l, m) terms into one contraction and keep
-        # the diagonal blocks (the cross-term blocks are not used)
+        # the Gram's MFMA tile is 16 shells wide: pack 16 // S fields into one contraction and keep the
+        # diagonal blocks (the cross-term blocks are not used).  The m = 0 terms of consecutive degrees
+        # share one field ((l, 0) real part, (l + 1, 0) imaginary part; a Gram with the real and imaginary
+        # products apart separates them); the other terms have a field each.
         self.pack = max(1, 16 // self.shells.size)
-        self.groups = [self.lm[i:i + self.pack] for i in range(0, len(self.lm), self.pack)]
+        pack_m0 = max(c.J for c in self.cats) <= 9  # the packed window kernel covers L <= 512
+        pairs = [l for l in range(0, self.lmax, 2)] if pack_m0 else []  # (l, 0) with (l + 1, 0)
+        single = [(l, m) for l, m in self.lm if m > 0 or not pack_m0 or (l % 2 == 0 and l == self.lmax)]
+        self.groups = [("plain", single[i:i + self.pack]) for i in range(0, len(single), self.pack)]
+        self.groups += [("pack0", pairs[i:i + self.pack]) for i in range(0, len(pairs), self.pack)]
         phi = sample_scaling_function() if phi is None else np.asarray(phi, dtype=np.float64)
         self.phi = phi
         self.phi_d = kernels.as_device(phi, torch.float64)
@@ -103,7 +109,7 @@ class CatalogueBatch:
         self.H = torch.empty(Lm * Lm * (Lm // 2 + 1), dtype=torch.complex64, device=dev)
         self.U = torch.empty(self.pack * S * Lm ** 3, dtype=torch.complex64, device=dev)
         self.pp_out = torch.zeros((len(self.cats), self.s_bins.size), dtype=torch.float64, device=dev)
-        self.gram_out = torch.zeros((len(self.cats), len(self.groups), 256), dtype=torch.float64, device=dev)
+        self.gram_out = torch.zeros((len(self.cats), len(self.groups), 512), dtype=torch.float64, device=dev)
         self._warm = False
 
     # ------------------------------------------------------------ device work
@@ -125,14 +131,21 @@ class CatalogueBatch:
         kernels.half_spectrum_f32(dc, H)
         dv["plan"].reduce(H, H, self.P[J], self.pp_out[i])
         S = self.shells.size
-        for k, group in enumerate(self.groups):
+        for k, (kind, group) in enumerate(self.groups):
+            if kind == "pack0":
+                for pos, l in enumerate(group):
+                    Uj = self.U[pos * S * N:(pos + 1) * S * N].view(S, L, L, L)
+                    kernels.multipole_window_ifft_pack0_f32(H, self.P[J], dv["radii"], l, 1.0 / N, Uj)
+                U = self.U[:len(group) * S * N].view(len(group) * S, L, L, L)
+                kernels.multipole_gram_f32_dev_ri(d, U, self.Q, self.lam, self.gram_out[i, k])
+                continue
             pos = 0
             for l, ms in _window_runs(group):  # orders of one l share the radial factors
                 Uj = self.U[pos * S * N:(pos + len(ms)) * S * N].view(len(ms) * S, L, L, L)
                 kernels.multipole_window_ifft_multi_f32(H, self.P[J], dv["radii"], l, ms, 1.0 / N, Uj)
                 pos += len(ms)
             U = self.U[:len(group) * S * N].view(len(group) * S, L, L, L)
-            kernels.multipole_gram_f32_dev(d, U, self.Q, self.lam, self.gram_out[i, k])
+            kernels.multipole_gram_f32_dev(d, U, self.Q, self.lam, self.gram_out[i, k, :256])
 
     def warmup(self):
         """One eager catalogue per level (FFT tables, tuned tiles, workspaces) and the projection
@@ -191,8 +204,15 @@ class CatalogueBatch:
             xi = pp[i] / float(L) ** 6 / rho ** 2
             zeta = np.zeros((S, S, self.lmax + 1))
             v_re = {}
-            for k, group in enumerate(self.groups):
-                full = kernels.gram_block_to_matrix(gr[i, k], len(group) * S)
+            for k, (kind, group) in enumerate(self.groups):
+                if kind == "pack0":
+                    re = kernels.gram_block_to_matrix(gr[i, k, :256], len(group) * S)
+                    im = kernels.gram_block_to_matrix(gr[i, k, 256:], len(group) * S)
+                    for j, l in enumerate(group):  # l even: real part; l + 1 odd: imaginary part, sign (-1)^(l+1)
+                        v_re[(l, 0)] = 4.0 * np.pi * re[j * S:(j + 1) * S, j * S:(j + 1) * S] / N
+                        v_re[(l + 1, 0)] = -4.0 * np.pi * im[j * S:(j + 1) * S, j * S:(j + 1) * S] / N
+                    continue
+                full = kernels.gram_block_to_matrix(gr[i, k, :256], len(group) * S)
                 for j, (l, m) in enumerate(group):
                     G = ((-1.0) ** (l + m)) * full[j * S:(j + 1) * S, j * S:(j + 1) * S]
                     v_re[(l, m)] = 4.0 * np.pi * G / N
