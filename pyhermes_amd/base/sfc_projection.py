@@ -67,11 +67,15 @@ class SFCProjection(TaskBase):
         self.particle_count = p_pos.shape[0]
         n = self.particle_count
         cw = np.ones(n, dtype=np.float32) if self.catalog_weight is None else self.catalog_weight
-        fv = np.ones(n, dtype=np.float32) if self.field_value is None else self.field_value
         self.fin["catalog_weight_key"] = None if self.catalog_weight is None else "custom"
         self.fin["field_value_key"] = None if self.field_value is None else "custom"
-        factors = {}
-        for name, value in (("catalog_weight", cw), ("field_value", fv)):
+        # an absent field value is the unit value: returned as None, and the sums and weights that would
+        # multiply by 1.0 (exact in FP64) use the catalogue weights alone
+        factors = {"field_value": None}
+        inputs = [("catalog_weight", cw)]
+        if self.field_value is not None:
+            inputs.append(("field_value", self.field_value))
+        for name, value in inputs:
             if np.isscalar(value):
                 value = np.full(n, value, dtype=np.float64)
             if not isinstance(value, np.ndarray) or value.ndim != 1 or value.shape[0] != n:
@@ -101,13 +105,17 @@ class SFCProjection(TaskBase):
         self.phi_support = self.phi_array.shape[0] // self.phi_resolution
         self.scale_factor = self.L / self.box_size
         p_pos, cw, fv = self._load_particle_input()
+        unit_value = fv is None
+        if unit_value:  # read-only view of ones: the attribute keeps its shape without an O(N) array
+            fv = np.broadcast_to(np.float32(1.0), (cw.shape[0],))
         self.particle_pos, self.catalog_weight, self.field_value = p_pos, cw, fv
         self.catalog_weight_sum = float(np.sum(cw, dtype=np.float64))
         self.catalog_weight_sq_sum = float(np.sum(cw ** 2, dtype=np.float64))
         if not np.isfinite(self.catalog_weight_sum) or np.isclose(self.catalog_weight_sum, 0.0):
             self.logger.error(f"Catalogue weights must have a finite, non-zero sum, got {self.catalog_weight_sum!r}.")
             func_util.safe_exit(1)
-        self.raw_field_weighted_sum = float(np.sum(cw * fv, dtype=np.float64))
+        self.raw_field_weighted_sum = (self.catalog_weight_sum if unit_value
+                                       else float(np.sum(cw * fv, dtype=np.float64)))
         if not np.isfinite(self.raw_field_weighted_sum):
             self.logger.error("The product catalog_weight * field_value produced a non-finite summed field value.")
             func_util.safe_exit(1)
@@ -120,7 +128,7 @@ class SFCProjection(TaskBase):
             if np.isclose(den, 0.0):
                 self.logger.error("weight_normalization='field' requires a finite, non-zero raw_field_weighted_sum.")
                 func_util.safe_exit(1)
-        self.projection_weight = cw * fv / den
+        self.projection_weight = cw / den if unit_value else cw * fv / den
         self.field_integral = self.raw_field_weighted_sum / den
         if self.save_particle_data:
             path = self.particle_data_path or (os.path.splitext(self.fout_path)[0] + "_particles.npz"

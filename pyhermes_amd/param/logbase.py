@@ -34,7 +34,8 @@ class MPIRankFilter(logging.Filter):
 def setup_logger(module_name, class_name=None, level=logging.INFO, stream_handler=True):
     name = module_name + (f":{class_name}" if class_name else "")
     logger = logging.getLogger(name)
-    logger.setLevel(level)
+    if logger.level != level:  # setLevel clears every logger's cache, so only call it on a change
+        logger.setLevel(level)
     if stream_handler and not logger.handlers:
         h = logging.StreamHandler(stream=sys.stdout)
         h.addFilter(MPIRankFilter())

@@ -39,6 +39,19 @@ def loads_json_relaxed(text):
     return json.loads(_TRAILING_COMMA_RE.sub(r"\1", stripped))
 
 
+def copy_json(value):
+    """Copy of parsed JSON: dicts and lists are rebuilt, the (immutable) scalars and strings are shared."""
+    if isinstance(value, dict):
+        return {k: copy_json(v) for k, v in value.items()}
+    if isinstance(value, list):
+        return [copy_json(v) for v in value]
+    return value
+
+
+# parsed default_params.json per path, with the file's modification time; readers get copies
+_DEFAULTS_CACHE = {}
+
+
 def read_param(config_path=None):
     if not config_path:
         config_path = ParamBase.get_parser().parse_args().config
@@ -133,11 +146,16 @@ class ParamBase(object):
 
     def _read_default(self, default_fname):
         try:
-            with open(default_fname) as f:
-                params = loads_json_relaxed(f.read())
+            stamp = os.stat(default_fname).st_mtime_ns
         except FileNotFoundError:
             self.logger.error(f"Default parameter file not found: '{default_fname}'.")
             func_util.safe_exit(1)
+        cached = _DEFAULTS_CACHE.get(default_fname)
+        if cached is None or cached[0] != stamp:
+            with open(default_fname) as f:
+                cached = (stamp, loads_json_relaxed(f.read()))
+            _DEFAULTS_CACHE[default_fname] = cached
+        params = copy_json(cached[1])
         if params.get("enable_default_param") is not True:
             return
         for key in params:

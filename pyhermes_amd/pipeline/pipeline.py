@@ -1,7 +1,5 @@
 """TaskBase: parameter merge and runtime options (mirrors pyhermes.pipeline.pipeline:10-68)."""
 
-import copy
-
 from ..param.logbase import configure_mpi_logging, setup_logger
 from ..param.parambase import ParamBase
 from ..utils import func_util
@@ -28,7 +26,7 @@ class TaskBase(object):
         self.logger = setup_logger(__name__, self.__class__.__name__)
         pb = ParamBase()
         pb.read_default(self.__class__)
-        defaults = copy.deepcopy(pb.default_params)
+        defaults = pb.default_params  # read_default hands this ParamBase its own copy
         user = param_task
         if self.task_name not in list(user.keys()):
             self.logger.error(f"Mismatch of task name, expected <{self.task_name}>, but it was not found in the "

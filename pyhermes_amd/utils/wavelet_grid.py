@@ -37,8 +37,21 @@ def _cascade_step(coeffs, filt):
     return out
 
 
+_PHI_CACHE = {}
+
+
 def sample_scaling_function(wavelet_name="db2", level=10):
-    """Scaling-function samples phi(i / 2^level), i = 0 .. (support * 2^level - 1)."""
+    """Scaling-function samples phi(i / 2^level), i = 0 .. (support * 2^level - 1).
+
+    The cascade runs once per (wavelet, level) in a process; callers get their own copy.
+    """
+    key = (wavelet_name, int(level))
+    if key not in _PHI_CACHE:
+        _PHI_CACHE[key] = _cascade_samples(wavelet_name, int(level))
+    return _PHI_CACHE[key].copy()
+
+
+def _cascade_samples(wavelet_name, level):
     if wavelet_name not in _REC_LO:
         raise ValueError(f"Unsupported wavelet_mode '{wavelet_name}'. Supported: {sorted(_REC_LO)}.")
     filt = np.asarray(_REC_LO[wavelet_name], dtype=np.float64)

@@ -168,6 +168,48 @@ def test_corr2pcf_products_and_sampling():
     np.testing.assert_allclose(t.sampling_arrays["s"], [5, 10, 15, 20, 25])
 
 
+def test_cached_defaults_and_phi_are_private_copies():
+    """Tasks read default_params.json and sample the scaling function once per process; every task must
+    still own its parameters, and every caller its phi samples."""
+    from pyhermes_amd.theory import Corr_2PCF
+    from pyhermes_amd.utils.wavelet_grid import sample_scaling_function
+    spec = {"sampling": {"s": {"min": 5.0, "max": 25.0, "n": 5}}}
+    a = Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform", **spec}})
+    a.task_params["products"] = "mutated"
+    a.task_params["sampling"]["s"]["n"] = 99
+    b = Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform", **spec}})
+    assert b.task_params["products"] != "mutated"
+    assert b.task_params["sampling"]["s"]["n"] == 5
+    p1 = sample_scaling_function()
+    ref = p1.copy()
+    p1[:] = -1.0
+    p2 = sample_scaling_function()
+    np.testing.assert_array_equal(p2, ref)
+    assert abs(ref.sum() / 1024 - 1.0) < 1e-12  # the samples integrate to one at level 10
+
+
+@pytest.mark.parametrize("norm", ["raw", "catalog", "field"])
+def test_absent_field_value_equals_explicit_ones(norm):
+    """No field value means the unit value: the sums and projection weights match explicit ones bit for bit."""
+    from pyhermes_amd.base import SFCProjection
+    rng = np.random.default_rng(3)
+    pos = rng.uniform(0.0, 100.0, (5000, 3))
+    w = rng.lognormal(0.0, 0.5, 5000).astype(np.float32)
+
+    def prepared(fv):
+        t = SFCProjection({"SFCProjection": {"J": 5, "box_size": 100.0, "particle_pos": pos, "catalog_weight": w,
+                                             "field_value": fv, "weight_normalization": norm, "fout_path": ""}})
+        t.prepare_input_fields()
+        return t
+
+    a, b = prepared(None), prepared(np.ones(5000, dtype=np.float32))
+    for key in ("catalog_weight_sum", "catalog_weight_sq_sum", "raw_field_weighted_sum", "field_integral"):
+        assert getattr(a, key) == getattr(b, key), key
+    assert np.array_equal(a.projection_weight, b.projection_weight)
+    assert a.field_value.shape == (5000,) and np.all(a.field_value == 1.0)
+    assert a.fin["field_value_key"] is None and b.fin["field_value_key"] == "custom"
+
+
 def test_result_save_load_roundtrip(tmp_path):
     from pyhermes_amd.io import Corr2PCFData, Corr3PCFData
     d = Corr2PCFData()
