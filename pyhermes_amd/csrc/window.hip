@@ -231,13 +231,21 @@ __device__ __forceinline__ void radial_jl(double t, double* j) {
 // accumulates sum_k K_c(k) Re F1 conj F2 in FP64; chan[u * (LMAX + 1) + l] is the
 // channel of (u, l) or -1.
 // T = float: FP32 windows (FP64 power and accumulation); T = double: all FP64.
-template <int LMAX, typename T, typename C>
+//
+// ROT (FP64 thin shells, equally spaced radii R_u = R0 + u dR): sin and cos of x = 2 pi |a| R_u / L
+// come from those at R0 by rotating through the angle 2 pi |a| dR / L (both reduced from exact turn
+// fractions once per alias), and 1 / x from 1 / (2 pi |a|) times 1 / R_u: four FMAs and two
+// multiplies per (alias, radius) in place of an FP64 sincos and a division.  x < 1 takes radial_jl
+// as before.  Each rotation adds about two ulps to |sin|, |cos| <= 1.
+template <int LMAX, typename T, typename C, bool ROT = false>
 __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce_radial(int L, const C* __restrict__ F1,
                                                                     const C* __restrict__ F2,
                                                                     const double* __restrict__ P,
                                                                     const double* __restrict__ radii_over_L, int nu,
+                                                                    double R0, double dR,
                                                                     const int* __restrict__ chan, int nch,
                                                                     double* __restrict__ partial) {
+    static_assert(!ROT || (LMAX == 0 && sizeof(T) == 8), "rotated phases: FP64 thin shells");
     extern __shared__ double wacc[];  // [waves][nch]
     const int Lh = L / 2 + 1;
     const int wave = threadIdx.x / 64, lane = threadIdx.x % 64;
@@ -287,17 +295,53 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce_radial(int L, const
                 wl[a][n + 1] = n2 ? p * pn : (T)0;
             }
         }
+        // ROT: per alias sin, cos at the current radius, the rotation step and 1 / (2 pi |a|) (+inf for
+        // a = 0, which then always takes radial_jl)
+        double sn[8], cs[8], sd[8], cd[8], iqa[8];
+        if (ROT) {
+#pragma unroll
+            for (int a = 0; a < 8; ++a) {
+                const double t0 = sa[a] * R0, td = sa[a] * dR;
+                sincospi(2.0 * (t0 - floor(t0)), &sn[a], &cs[a]);
+                sincospi(2.0 * (td - floor(td)), &sd[a], &cd[a]);
+                iqa[a] = sa[a] > 0.0 ? 1.0 / (2.0 * HERMES_PI * sa[a]) : __builtin_inf();
+            }
+        }
         for (int u = 0; u < nu; ++u) {
             const double R = radii_over_L[u];
+            const double iR = ROT ? 1.0 / R : 0.0;
             T kb[LMAX + 1];
 #pragma unroll
             for (int l = 0; l <= LMAX; ++l) kb[l] = 0;
+            unsigned small = 0;  // ROT: aliases with x < 1, evaluated below
 #pragma unroll
             for (int a = 0; a < 8; ++a) {
                 T j[LMAX + 1];
-                radial_jl<LMAX>(sa[a] * R, j);
+                if (ROT) {
+                    if (u > 0) {
+                        const double s2 = fma(sn[a], cd[a], cs[a] * sd[a]);
+                        cs[a] = fma(cs[a], cd[a], -(sn[a] * sd[a]));
+                        sn[a] = s2;
+                    }
+                    if (R >= iqa[a]) {  // x = R / iqa >= 1
+                        kb[0] += sn[a] * (iqa[a] * iR) * wl[a][0];
+                    } else {
+                        small |= 1u << a;
+                    }
+                } else {
+                    radial_jl<LMAX>(sa[a] * R, j);
 #pragma unroll
-                for (int l = 0; l <= LMAX; ++l) kb[l] += j[l] * wl[a][l];
+                    for (int l = 0; l <= LMAX; ++l) kb[l] += j[l] * wl[a][l];
+                }
+            }
+            // one copy of the small-argument path (it is rare: |a| R / L < 1 / (2 pi)), outside the
+            // unrolled alias loop, so its registers do not add to the rotation state's
+            for (; ROT && small; small &= small - 1) {
+                const int a = __builtin_ctz(small);
+                const int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+                const int ix = ax < 0 ? -ax : ax, iy = ay < 0 ? -ay : ay, iz = az < 0 ? -az : az;
+                const double t = sqrt((double)(ax * ax + ay * ay + az * az)) * R;
+                kb[0] += (T)(jl_f64(0, 2.0 * HERMES_PI * t) * (P[ix] * P[iy] * P[iz]));
             }
 #pragma unroll
             for (int l = 0; l <= LMAX; ++l) {
@@ -787,6 +831,8 @@ int hermes_spectrum_mul(hermes_ctx* ctx, int J, int half, int dbl, void* spec, c
 // of even order <= 10 (FP32) or <= 4 (FP64), or the channels do not fit the kernel's LDS.
 struct RadialChannels {
     int LM = 0, nu = 0, nch = 0;
+    int rot = 0;                 // FP64 thin shells at >= 3 equally spaced radii R0 + u dR: rotated phases
+    double R0 = 0.0, dR = 0.0;
     std::vector<double> radii;   // R / L per radius
     std::vector<int> chan;       // nu x (LM + 1): channel or -1
     std::vector<int> ch_of;      // per descriptor
@@ -821,24 +867,37 @@ static bool radial_channels(int J, int dbl, const hermes_window_desc* descs, int
         rc.factor[b] = d.type == HERMES_WIN_SHELL ? 1.0 : hermes_ylm_cnorm(d.l, 0);
     }
     rc.nu = (int)rc.radii.size();
+    if (dbl && rc.LM == 0 && rc.nu >= 3) {
+        const double r0 = rc.radii[0], d = (rc.radii[rc.nu - 1] - r0) / (rc.nu - 1);
+        double rmax = 0.0;
+        for (double r : rc.radii) rmax = std::max(rmax, std::fabs(r));
+        bool uniform = d != 0.0;
+        for (int u = 0; u < rc.nu && uniform; ++u) uniform = std::fabs(rc.radii[u] - (r0 + u * d)) <= 1e-12 * rmax;
+        if (uniform) {
+            rc.rot = 1;
+            rc.R0 = r0;
+            rc.dR = d;
+        }
+    }
     return sizeof(double) * (RED_THREADS / 64) * rc.nch <= 64 * 1024;
 }
 
 static hipError_t launch_radial(hipStream_t st, int J, int dbl, int LM, int nblocks, const void* F1, const void* F2,
-                                const double* P, const double* d_radii, int nu, const int* d_chan, int nch,
-                                double* partial) {
+                                const double* P, const double* d_radii, int nu, int rot, double R0, double dR,
+                                const int* d_chan, int nch, double* partial) {
     const int L = 1 << J;
     const size_t shmem = sizeof(double) * (RED_THREADS / 64) * nch;
-#define HERMES_RADIAL(LMV, T, C)                                                                      \
-    k_pair_reduce_radial<LMV, T, C><<<nblocks, RED_THREADS, shmem, st>>>(L, (const C*)F1, (const C*)F2, P, d_radii, \
-                                                                        nu, d_chan, nch, partial)
+#define HERMES_RADIAL(LMV, T, C, ROT)                                                                      \
+    k_pair_reduce_radial<LMV, T, C, ROT><<<nblocks, RED_THREADS, shmem, st>>>(L, (const C*)F1, (const C*)F2, P, \
+                                                                             d_radii, nu, R0, dR, d_chan, nch, partial)
     if (dbl) {
-        if (LM == 0) HERMES_RADIAL(0, double, double2);
-        else HERMES_RADIAL(4, double, double2);
+        if (LM == 0 && rot) HERMES_RADIAL(0, double, double2, true);
+        else if (LM == 0) HERMES_RADIAL(0, double, double2, false);
+        else HERMES_RADIAL(4, double, double2, false);
     } else {
-        if (LM == 0) HERMES_RADIAL(0, float, float2);
-        else if (LM == 4) HERMES_RADIAL(4, float, float2);
-        else HERMES_RADIAL(10, float, float2);
+        if (LM == 0) HERMES_RADIAL(0, float, float2, false);
+        else if (LM == 4) HERMES_RADIAL(4, float, float2, false);
+        else HERMES_RADIAL(10, float, float2, false);
     }
 #undef HERMES_RADIAL
     return hipGetLastError();
@@ -880,8 +939,8 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
     if (e != hipSuccess) return fail(e);
     e = hipMemcpyAsync(d_chan, ch.chan.data(), sizeof(int) * ch.chan.size(), hipMemcpyHostToDevice, ctx->stream);
     if (e != hipSuccess) return fail(e);
-    if ((e = launch_radial(ctx->stream, J, dbl, ch.LM, nblocks, F1, F2, P, d_radii, nu, d_chan, nch, partial)) !=
-        hipSuccess)
+    if ((e = launch_radial(ctx->stream, J, dbl, ch.LM, nblocks, F1, F2, P, d_radii, nu, ch.rot, ch.R0, ch.dR, d_chan,
+                           nch, partial)) != hipSuccess)
         return fail(e);
     k_sum_partial_groups<<<hermes_blocks((int64_t)ng * nch, 256), 256, 0, ctx->stream>>>(partial, nblocks, nch, gsum);
     if ((e = hipGetLastError()) != hipSuccess) return fail(e);
@@ -897,6 +956,8 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
 
 struct hermes_pair_plan {
     int J = 0, dbl = 0, LM = 0, nu = 0, nch = 0, n_desc = 0, nblocks = 0;
+    int rot = 0;
+    double R0 = 0.0, dR = 0.0;
     char* mem = nullptr;  // radii | chan | ch_of | factor | partial | group sums
     double* radii = nullptr;
     int* chan = nullptr;
@@ -936,6 +997,9 @@ int hermes_pair_plan_create(hermes_ctx* ctx, int J, int dbl, const hermes_window
     p->LM = ch.LM;
     p->nu = ch.nu;
     p->nch = ch.nch;
+    p->rot = ch.rot;
+    p->R0 = ch.R0;
+    p->dR = ch.dR;
     p->n_desc = n_desc;
     p->nblocks = pair_blocks(J);
     auto al = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -972,7 +1036,7 @@ int hermes_pair_reduce_planned(hermes_ctx* ctx, const hermes_pair_plan* plan, co
     HERMES_REQUIRE(ctx && plan && F1 && F2 && P && out_dev, "hermes_pair_reduce_planned: null argument");
     HERMES_CHECK_HIP(hipSetDevice(ctx->device));
     HERMES_CHECK_HIP(launch_radial(ctx->stream, plan->J, plan->dbl, plan->LM, plan->nblocks, F1, F2, P, plan->radii,
-                                   plan->nu, plan->chan, plan->nch, plan->partial));
+                                   plan->nu, plan->rot, plan->R0, plan->dR, plan->chan, plan->nch, plan->partial));
     const int ng = part_groups(plan->nblocks);
     k_sum_partial_groups<<<hermes_blocks((int64_t)ng * plan->nch, 256), 256, 0, ctx->stream>>>(
         plan->partial, plan->nblocks, plan->nch, plan->gsum);

@@ -113,6 +113,35 @@ def test_pair_reduce_radial_path(phi, case, J):
         assert abs(g / L ** 6 - ref) <= 2e-5 * scale / L ** 6, (d.type, d.a0, d.l)
 
 
+@pytest.mark.parametrize("J", [5, 6])
+def test_pair_reduce_uniform_shells_fp64(phi, J):
+    """FP64 thin shells at equally spaced radii (linear binning) take the kernel with rotated phases:
+    it matches the same kernel without rotation (one radius moved by 1e-9, which turns it off) and the
+    convolution oracle to FP64 rounding."""
+    import torch
+    from pyhermes_amd import kernels
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    rng = np.random.default_rng(70 + J)
+    f1 = rng.normal(size=(L, L, L))
+    f2 = f1 + 0.5 * rng.normal(size=(L, L, L))
+    radii = np.linspace(0.05, 0.45 * L, 20)
+    bumped = radii.copy()
+    bumped[7] *= 1.0 + 1e-9
+    print(f"uniform FP64 shells J={J}", flush=True)
+    F1 = kernels.rfftn(kernels.as_device(f1, torch.float64))
+    F2 = kernels.rfftn(kernels.as_device(f2, torch.float64))
+    got = kernels.pair_reduce(F1, F2, Pd, [kernels.window_desc("shell", R) for R in radii])
+    plain = kernels.pair_reduce(F1, F2, Pd, [kernels.window_desc("shell", R) for R in bumped])
+    scale = np.sqrt(np.mean(f1 ** 2) * np.mean(f2 ** 2)) * L ** 6
+    keep = np.arange(20) != 7
+    assert np.abs(got - plain)[keep].max() <= 1e-12 * scale
+    assert abs(got[7] - plain[7]) <= 1e-6 * scale
+    for i in (0, 1, 10, 19):
+        ref = ow.pair_product(f1, f2, ow.octant_rfft_kernel(L, P, "shell", radii[i], 0.0))
+        assert abs(got[i] / L ** 6 - ref) <= 1e-11 * scale / L ** 6, (i, radii[i])
+
+
 def test_pair_reduce_rejects_odd_multipoles():
     import torch
     from pyhermes_amd import kernels
