@@ -26,10 +26,12 @@ class HermesData(object):
         self.logger.info(f"Writing data to ---> {path} <---")
         return path
 
-    def load(self, f_in, **_ignored):
+    def load(self, f_in, trusted=False, **_ignored):
+        """trusted=False resolves only the data types PyHermes files hold (arrays, scalars, containers,
+        paths); True loads any pickle, for files whose origin you trust."""
         try:
             with open(f_in, "rb") as f:
-                dataset = read_numpy_pickle(f)
+                dataset = read_numpy_pickle(f, trusted=trusted)
         except Exception as exc:
             self.logger.error(f"An error occurred while loading the file '{f_in}': {exc}")
             func_util.safe_exit(1)

@@ -53,8 +53,8 @@ class Corr2PCFData(_Result):
     def save_corr2pcf(self, f_out, single=True, overwrite=False):
         return self.save(f_out, overwrite=overwrite)
 
-    def load_corr2pcf(self, f_in, single=True):
-        return self.load(f_in)
+    def load_corr2pcf(self, f_in, single=True, trusted=False):
+        return self.load(f_in, trusted=trusted)
 
 
 class Corr3PCFData(_Result):
@@ -66,8 +66,8 @@ class Corr3PCFData(_Result):
     def save_corr3pcf(self, f_out, single=True, overwrite=False):
         return self.save(f_out, overwrite=overwrite)
 
-    def load_corr3pcf(self, f_in, single=True):
-        return self.load(f_in)
+    def load_corr3pcf(self, f_in, single=True, trusted=False):
+        return self.load(f_in, trusted=trusted)
 
 
 class Corr3PCFMultipoleData(_Result):
@@ -80,5 +80,5 @@ class Corr3PCFMultipoleData(_Result):
     def save_corr3pcf_multipole(self, f_out, single=True, overwrite=False):
         return self.save(f_out, overwrite=overwrite)
 
-    def load_corr3pcf_multipole(self, f_in, single=True):
-        return self.load(f_in)
+    def load_corr3pcf_multipole(self, f_in, single=True, trusted=False):
+        return self.load(f_in, trusted=trusted)

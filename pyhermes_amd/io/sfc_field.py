@@ -388,8 +388,8 @@ class SFCField(HermesData):
             self.sfc_info.update(dataset["sfc_info"])
         self.format_sfc_params()
 
-    def load_sfc_field(self, f_in, single=True):
-        return self.load(f_in)
+    def load_sfc_field(self, f_in, single=True, trusted=False):
+        return self.load(f_in, trusted=trusted)
 
     def save_sfc_field(self, f_out, single=True, overwrite=False):
         return self.save(f_out, overwrite=overwrite)
