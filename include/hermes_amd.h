@@ -109,9 +109,12 @@ int hermes_fft3_tile_choice(hermes_ctx *ctx, int J, int inverse, int *tw_out, in
 /* Pins the strided-pass tile (must be one of the candidates) instead of autotuning. */
 int hermes_fft3_set_tile(hermes_ctx *ctx, int J, int inverse, int tw, int threads);
 
-/* ---- window descriptors (window_functions.py:29-129, :426-438;
+/* ---- window descriptors (window_functions.py:29-129, :164-319, :426-438;
  *      radial_multipole_windows.py:128-144 thin-shell profile).
- * Length arguments a0, a1 are in grid units (len_args * L / box_size). */
+ * Length arguments a0, a1 are in grid units (len_args * L / box_size).  The line-of-sight
+ * windows (ring .. cylinder) take k_par = k[axis], k_perp = |k - k_par e_axis|,
+ * q_perp = 2 pi k_perp R, q_par = 2 pi k_par H; their line of sight is the coordinate
+ * axis `axis` (0 x, 1 y, 2 z: the reference's los_args (nx, ny, nz) along an axis). */
 enum {
     HERMES_WIN_SPHERE = 0,            /* a0 = R */
     HERMES_WIN_GAUSSIAN = 1,          /* a0 = R */
@@ -119,11 +122,16 @@ enum {
     HERMES_WIN_THICK_SHELL = 3,       /* a0 = R, a1 = delta_R */
     HERMES_WIN_GAUSSIAN_SHELL = 4,    /* a0 = R_shell, a1 = R_smooth */
     HERMES_WIN_INVERSE_LAPLACIAN = 5, /* no arguments; zero mode -> 0 */
-    HERMES_WIN_SHELL_MULTIPOLE = 6    /* a0 = R: j_l(2 pi |k| R) Y_l^m(k-hat) */
+    HERMES_WIN_SHELL_MULTIPOLE = 6,   /* a0 = R: j_l(2 pi |k| R) Y_l^m(k-hat) */
+    HERMES_WIN_RING = 7,              /* a0 = R, a1 = H: J0(q_perp) cos(q_par) */
+    HERMES_WIN_DISK = 8,              /* a0 = R, a1 = H: 2 J1(q_perp) / q_perp cos(q_par) */
+    HERMES_WIN_CYLSHELL = 9,          /* a0 = R, a1 = H (half height): J0(q_perp) sin(q_par) / q_par */
+    HERMES_WIN_CYLINDER = 10          /* a0 = R, a1 = H (half height): 2 J1(q_perp) / q_perp sin(q_par) / q_par */
 };
 
 typedef struct {
-    int32_t type, l, m, pad;
+    int32_t type, l, m;
+    int32_t axis; /* line-of-sight axis of the ring .. cylinder windows; 0 for the others */
     double a0, a1;
 } hermes_window_desc;
 

@@ -113,3 +113,53 @@ def pair_product_realspace(field1, field2, win_real):
     L = field1.shape[0]
     conv = np.real(np.fft.ifftn(np.fft.fftn(win_real) * np.fft.fftn(field1)))
     return float(np.sum(conv * field2) / L ** 3)
+
+
+# ---------------------------------------------------------------- line-of-sight windows
+# window_functions.py:164-319: ring, disk, cylshell and cylinder windows about a line of sight n
+# (normalised), with k_par = k . n, k_perp = sqrt(|k|^2 - k_par^2), q_perp = 2 pi k_perp R,
+# q_par = 2 pi k_par H.  J0 and J1 are scipy.special's (the reference evaluates 8-digit rational
+# approximations of them, special_functions.py:131-215).
+LOS_WINDOW_TYPES = ("ring", "disk", "cylshell", "cylinder")
+
+
+def los_window(win_type, ki, kj, kk, R, H, los=(0.0, 0.0, 1.0)):
+    """Vectorised transfer function of one line-of-sight window (k in cycles per grid cell)."""
+    from scipy.special import j0, j1
+    n = np.asarray(los, dtype=np.float64)
+    n = n / np.linalg.norm(n)
+    ki, kj, kk = (np.asarray(a, dtype=np.float64) for a in (ki, kj, kk))
+    kpar = ki * n[0] + kj * n[1] + kk * n[2]
+    kperp = np.sqrt(np.maximum(ki * ki + kj * kj + kk * kk - kpar * kpar, 0.0))
+    qp = 2.0 * np.pi * kperp * R
+    qz = 2.0 * np.pi * kpar * H
+    if win_type in ("ring", "cylshell"):
+        perp = j0(qp)
+    else:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            perp = np.where(qp == 0.0, 1.0, 2.0 * j1(qp) / np.where(qp == 0.0, 1.0, qp))
+    if win_type in ("ring", "disk"):
+        par = np.cos(qz)
+    else:
+        with np.errstate(invalid="ignore", divide="ignore"):
+            par = np.where(qz == 0.0, 1.0, np.sin(qz) / np.where(qz == 0.0, 1.0, qz))
+    return perp * par
+
+
+def los_rfft_kernel(L, P, win_type, R, H, los=(0.0, 0.0, 1.0)):
+    """convolution.py:73-157 (bandwidth 1): K[x, y, z] = sum over the 8 signed aliases
+    (x/L or -(L-x)/L per axis, the reference's summation order) of P P P W -> (L, L, L//2+1)."""
+    P = np.asarray(P, dtype=np.float64)
+    x = np.arange(L)[:, None, None]
+    y = np.arange(L)[None, :, None]
+    z = np.arange(L // 2 + 1)[None, None, :]
+    pos = [(x, x / L), (y, y / L), (z, z / L)]
+    neg = [(L - x, -(L - x) / L), (L - y, -(L - y) / L), (L - z, -(L - z) / L)]
+    order = ((0, 0, 0), (1, 0, 0), (0, 1, 0), (0, 0, 1), (1, 1, 0), (1, 0, 1), (0, 1, 1), (1, 1, 1))
+    out = np.zeros((L, L, L // 2 + 1))
+    for sx, sy, sz in order:
+        (ix, kx) = neg[0] if sx else pos[0]
+        (iy, ky) = neg[1] if sy else pos[1]
+        (iz, kz) = neg[2] if sz else pos[2]
+        out = out + P[ix] * P[iy] * P[iz] * los_window(win_type, kx, ky, kz, R, H, los)
+    return out

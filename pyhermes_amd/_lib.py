@@ -49,7 +49,7 @@ _SIGNATURES = {
 
 class WindowDesc(ctypes.Structure):
     """hermes_window_desc (include/hermes_amd.h)."""
-    _fields_ = [("type", ctypes.c_int32), ("l", ctypes.c_int32), ("m", ctypes.c_int32), ("pad", ctypes.c_int32),
+    _fields_ = [("type", ctypes.c_int32), ("l", ctypes.c_int32), ("m", ctypes.c_int32), ("axis", ctypes.c_int32),
                 ("a0", ctypes.c_double), ("a1", ctypes.c_double)]
 
 

@@ -16,6 +16,72 @@
 #define HERMES_PI 3.14159265358979323846
 
 // ------------------------------------------------------------------ FP64 faithful
+// Cylindrical Bessel J0(x) and J1(x), x >= 0, to FP64 rounding (the reference's jn_numba,
+// special_functions.py:131-270, uses 8-digit rational approximations instead):
+//   x <= 4       the power series sum_k (-x^2/4)^k / (k! (k+nu)!) (x/2)^nu (largest term 4, so little
+//                cancellation);
+//   4 < x < 25   Miller's backward recurrence J_{n-1} = (2n/x) J_n - J_{n+1} from n = x + 10 sqrt(x) + 30,
+//                normalised by J0 + 2 (J2 + J4 + ...) = 1 (values stay below 1e60 on the way down);
+//   x >= 25      Hankel's asymptotic expansion J_nu = sqrt(2/(pi x)) (P cos w - Q sin w),
+//                w = x - (2 nu + 1) pi/4, with a_k = a_{k-1} (4 nu^2 - (2k-1)^2) / (8k) and
+//                P = sum (-1)^k a_{2k} / x^{2k}, Q = sum (-1)^k a_{2k+1} / x^{2k+1}, summed while the
+//                terms fall (their smallest is about e^{-2x}).
+__host__ __device__ inline void bessel_j01_f64(double x, double& j0, double& j1) {
+    if (x <= 4.0) {
+        const double h = -0.25 * x * x;
+        double t0 = 1.0, t1 = 0.5 * x, s0 = 1.0, s1 = t1;
+        for (int k = 1; k <= 24; ++k) {
+            t0 *= h / ((double)k * k);
+            t1 *= h / ((double)k * (k + 1));
+            s0 += t0;
+            s1 += t1;
+        }
+        j0 = s0;
+        j1 = s1;
+        return;
+    }
+    if (x < 25.0) {
+        const int n0 = 2 * (((int)(x + 10.0 * sqrt(x)) + 30) / 2);  // even start
+        const double tox = 2.0 / x;
+        double jp = 0.0, jc = 1e-30, even_sum = 0.0, a0 = 0.0, a1 = 0.0;
+        for (int n = n0; n >= 1; --n) {
+            const double jm = n * tox * jc - jp;  // J_{n-1}
+            jp = jc;
+            jc = jm;
+            if (n == 1) {
+                a0 = jc;
+                a1 = jp;
+            } else if (((n - 1) & 1) == 0) {
+                even_sum += jc;  // J_{n-1} with n - 1 even and >= 2
+            }
+        }
+        const double norm = a0 + 2.0 * even_sum;
+        j0 = a0 / norm;
+        j1 = a1 / norm;
+        return;
+    }
+    double out[2];
+    for (int nu = 0; nu < 2; ++nu) {
+        const double mu4 = 4.0 * nu * nu;
+        double a = 1.0, P = 1.0, Q = 0.0, prev = 1e300, xp = 1.0;
+        for (int k = 1; k <= 60; ++k) {
+            a *= (mu4 - (2.0 * k - 1.0) * (2.0 * k - 1.0)) / (8.0 * k);
+            xp *= x;
+            const double term = a / xp;
+            if (fabs(term) >= prev || fabs(term) < 1e-17) break;
+            prev = fabs(term);
+            // k odd -> Q, k even -> P; sign (-1)^{floor(k/2)}
+            const double sg = ((k / 2) & 1) ? -1.0 : 1.0;
+            if (k & 1) Q += sg * term;
+            else P += sg * term;
+        }
+        const double w = x - (2.0 * nu + 1.0) * 0.25 * HERMES_PI;
+        out[nu] = sqrt(2.0 / (HERMES_PI * x)) * (P * cos(w) - Q * sin(w));
+    }
+    j0 = out[0];
+    j1 = out[1];
+}
+
 __device__ inline double jl_f64(int l, double x) {
     if (l < 0) return __builtin_nan("");
     if (x == 0.0) return l == 0 ? 1.0 : 0.0;
@@ -126,8 +192,29 @@ __device__ inline double win_sphere_f64(double k, double R) {
     return 3 * (s - q * c) / (q * q * q);
 }
 
-// Isotropic transfer function of one window at |k| (cycles per grid cell).
-__device__ inline double iso_window_f64(int type, double ki, double kj, double kk, double a0, double a1) {
+// Line-of-sight windows (window_functions.py:164-319) along coordinate axis `axis`: even in every
+// component of k, so the octant folds and the half-spectrum sums apply to them as to the isotropic ones.
+__device__ inline double los_window_f64(int type, double ki, double kj, double kk, double R, double H, int axis) {
+    const double kpar = axis == 0 ? ki : (axis == 1 ? kj : kk);
+    const double u = axis == 0 ? kj : ki, v = axis == 2 ? kj : kk;  // the two perpendicular components
+    const double qperp = 2.0 * HERMES_PI * sqrt(u * u + v * v) * R;
+    const double qpar = 2.0 * HERMES_PI * kpar * H;
+    double j0, j1;
+    bessel_j01_f64(qperp, j0, j1);
+    const double perp = (type == HERMES_WIN_RING || type == HERMES_WIN_CYLSHELL)
+                            ? j0
+                            : (qperp == 0.0 ? 1.0 : 2.0 * j1 / qperp);
+    const double par = (type == HERMES_WIN_RING || type == HERMES_WIN_DISK)
+                           ? cos(qpar)
+                           : (qpar == 0.0 ? 1.0 : sin(qpar) / qpar);
+    return perp * par;
+}
+
+// Transfer function of one real, even window at k (cycles per grid cell): the isotropic ones and the
+// line-of-sight ones.
+__device__ inline double iso_window_f64(int type, double ki, double kj, double kk, double a0, double a1,
+                                        int axis = 2) {
+    if (type >= HERMES_WIN_RING && type <= HERMES_WIN_CYLINDER) return los_window_f64(type, ki, kj, kk, a0, a1, axis);
     switch (type) {
     case HERMES_WIN_SPHERE: return win_sphere_f64(sqrt(ki * ki + kj * kj + kk * kk), a0);
     case HERMES_WIN_GAUSSIAN: {
@@ -177,7 +264,7 @@ __device__ inline double2 window_value_f64(const hermes_window_desc& d, double k
         double2 y = ylm_f64(d.l, d.m, ki, kj, kk);
         return make_double2(radial * y.x, radial * y.y);
     }
-    return make_double2(iso_window_f64(d.type, ki, kj, kk, d.a0, d.a1), 0.0);
+    return make_double2(iso_window_f64(d.type, ki, kj, kk, d.a0, d.a1, d.axis), 0.0);
 }
 
 // ------------------------------------------------------------------ FP32 fast path

@@ -36,7 +36,7 @@ __global__ void k_build_rfft(int L, const double* __restrict__ P, hermes_window_
         for (int a = 0; a < 8; ++a) {
             int ii = xs[ord[a][0]], jj = ys[ord[a][1]], kk = zs[ord[a][2]];
             double wa = 0.0;
-            wa += P[ii] * P[jj] * P[kk] * iso_window_f64(d.type, ii * inv_L, jj * inv_L, kk * inv_L, d.a0, d.a1);
+            wa += P[ii] * P[jj] * P[kk] * iso_window_f64(d.type, ii * inv_L, jj * inv_L, kk * inv_L, d.a0, d.a1, d.axis);
             t = (a == 0) ? wa : t + wa;
         }
         if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && i == 0) t = 0.0;
@@ -109,7 +109,7 @@ __device__ __forceinline__ float window_value_f32(const hermes_window_desc& d, i
         return j * y.x;  // m = 0 windows are real
     }
     double inv_L = 1.0 / L;
-    return (float)iso_window_f64(d.type, ax * inv_L, ay * inv_L, az * inv_L, d.a0, d.a1);
+    return (float)iso_window_f64(d.type, ax * inv_L, ay * inv_L, az * inv_L, d.a0, d.a1, d.axis);
 }
 
 // One thread per octant representative (x, y, z) in [0, L/2]^3; its kernel value is
@@ -751,8 +751,12 @@ __global__ void k_reduce_final(const double* __restrict__ partial, int nb, doubl
 }
 
 int check_desc(const hermes_window_desc& d) {
-    if (d.type < 0 || d.type > HERMES_WIN_SHELL_MULTIPOLE) {
+    if (d.type < 0 || d.type > HERMES_WIN_CYLINDER) {
         hermes_set_error("unknown window type %d", d.type);
+        return 2;
+    }
+    if (d.type >= HERMES_WIN_RING && (d.axis < 0 || d.axis > 2)) {
+        hermes_set_error("line-of-sight window: axis %d is not 0, 1 or 2", d.axis);
         return 2;
     }
     if (d.type == HERMES_WIN_SHELL_MULTIPOLE && (d.l < 0 || d.l > 40 || d.m > d.l || d.m < -d.l)) {

@@ -1,13 +1,14 @@
 """WindowFunc: Fourier window kernels built on the HIP device (drop-in for pyhermes.io.WindowFunc).
 
 Built-in isotropic windows (sphere, gaussian, shell, thick_shell, gaussian_shell,
-inverse_laplacian) are built by hermes_window_build_rfft in the octant / rFFT
+inverse_laplacian) and the line-of-sight windows (ring, disk, cylshell, cylinder,
+window_functions.py:164-319) are built by hermes_window_build_rfft in the rFFT
 layout (window.py:274-332); thin-shell ``radial_multipole`` and
 ``legendre_multipole`` windows by hermes_window_build_full in the complex
 full-FFT layout (window.py:197-264).  A user ``func`` is evaluated on the host
-over the same alias sums and then used on the device.  The anisotropic,
-derivative and tabulated-profile windows of the reference are outside this
-package's scope and raise NotImplementedError.
+over the same alias sums and then used on the device.  Line-of-sight windows take
+a line of sight along a coordinate axis (any sign); other directions, and the
+derivative and tabulated-profile windows of the reference, raise NotImplementedError.
 """
 
 import copy
@@ -25,8 +26,8 @@ from ..utils.window_params import (ANISOTROPIC_AUTO_WINDOW_TYPES, COMPLEX_FULL_F
 from .sfc_field import SFCField
 
 ISOTROPIC_TYPES = {"sphere", "gaussian", "shell", "thick_shell", "gaussian_shell", "inverse_laplacian"}
-KNOWN_UNSUPPORTED = {"cubic", "ring", "disk", "cylshell", "cylinder", "cw", "cws", "gdw", "directional_derivative",
-                     "laplacian"}
+LOS_TYPES = set(kernels.LOS_TYPES)
+KNOWN_UNSUPPORTED = {"cubic", "cw", "cws", "gdw", "directional_derivative", "laplacian"}
 ZERO_MODE_ZERO_WINDOW_TYPES = {"inverse_laplacian"}
 
 
@@ -38,6 +39,27 @@ def _iso_desc(wtype, rescaled):
     if wtype == "gaussian_shell":
         return kernels.window_desc(wtype, float(rescaled["R_shell"]), float(rescaled["R_smooth"]))
     return kernels.window_desc("inverse_laplacian")
+
+
+def los_axis(los_args):
+    """The coordinate axis (0 x, 1 y, 2 z) of a line of sight {nx, ny, nz} along one axis (either sign)."""
+    vec = np.array([los_args["nx"], los_args["ny"], los_args["nz"]], dtype=np.float64)
+    nz = np.nonzero(vec)[0]
+    if nz.size != 1:
+        raise NotImplementedError("pyhermes_amd builds line-of-sight windows with the line of sight along a "
+                                  f"coordinate axis; got (nx, ny, nz) = {tuple(vec)}")
+    return int(nz[0])
+
+
+def los_desc(wtype, rescaled, los_args):
+    """Descriptor of a ring / disk / cylshell / cylinder window (len_args R, H in grid units)."""
+    try:
+        R, H = float(rescaled["R"]), float(rescaled["H"])
+    except (KeyError, TypeError, ValueError) as exc:
+        raise ValueError(f"{wtype} requires finite len_args {{'R': ..., 'H': ...}}.") from exc
+    if not (np.isfinite(R) and np.isfinite(H)) or R < 0.0:
+        raise ValueError(f"{wtype} requires a finite non-negative R and a finite H.")
+    return kernels.window_desc(wtype, R, H, axis=los_axis(los_args))
 
 
 class WindowFunc(SFCField):
@@ -84,7 +106,8 @@ class WindowFunc(SFCField):
             if self.type in KNOWN_UNSUPPORTED:
                 raise NotImplementedError(f"Window type '{self.type}' is outside the field-pipeline scope of "
                                           "pyhermes_amd.")
-            if self.type not in ISOTROPIC_TYPES and self.type not in COMPLEX_FULL_FFT_WINDOW_TYPES:
+            if (self.type not in ISOTROPIC_TYPES and self.type not in LOS_TYPES
+                    and self.type not in COMPLEX_FULL_FFT_WINDOW_TYPES):
                 self.logger.error(f"Unsupported input window function type: {self.type}")
                 func_util.safe_exit(1)
         mode = win_params.get("kernel_mode", None) or default_kernel_mode(self.type, self.has_custom_func)
@@ -109,7 +132,10 @@ class WindowFunc(SFCField):
         self.is_composite_window = False
 
     def _validate_builtin_arguments(self):
-        """window.py:131-142."""
+        """window.py:131-142; line-of-sight windows: R, H and an axis line of sight, checked up front."""
+        if not self.has_custom_func and self.type in LOS_TYPES:
+            los_desc(self.type, self.rescale_len_args, self.los_args)
+            return
         if self.has_custom_func or self.type != "thick_shell":
             return
         try:
@@ -194,6 +220,11 @@ class WindowFunc(SFCField):
                                       "kernel_mode='auto'.")
                     func_util.safe_exit(1)
                 t = kernels.window_build_full(self.J, P, self._multipole_desc(), dbl=dbl)
+            elif self.type in LOS_TYPES:
+                # even in every component of k for an axis line of sight: the rFFT kernel of the reference's
+                # "auto" (full_rfft) mode and the octant fold sum the same alias values
+                t = kernels.window_build_rfft(self.J, P, los_desc(self.type, self.rescale_len_args, self.los_args),
+                                              dbl=dbl)
             else:
                 t = kernels.window_build_rfft(self.J, P, _iso_desc(self.type, self.rescale_len_args), dbl=dbl)
         self._dev_kernel[key] = t

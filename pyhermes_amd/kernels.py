@@ -133,12 +133,17 @@ def fft3_set_tile(J, tw, threads, inverse=True):
 
 # ---------------------------------------------------------------- windows
 WIN_CODES = {"sphere": 0, "gaussian": 1, "shell": 2, "thick_shell": 3, "gaussian_shell": 4,
-             "inverse_laplacian": 5, "shell_multipole": 6}
+             "inverse_laplacian": 5, "shell_multipole": 6, "ring": 7, "disk": 8, "cylshell": 9, "cylinder": 10}
+LOS_TYPES = ("ring", "disk", "cylshell", "cylinder")
 
 
-def window_desc(wtype, a0=0.0, a1=0.0, l=0, m=0):
-    """A hermes_window_desc; lengths in grid units."""
-    return _lib.WindowDesc(WIN_CODES[wtype], int(l), int(m), 0, float(a0), float(a1))
+def window_desc(wtype, a0=0.0, a1=0.0, l=0, m=0, axis=2):
+    """A hermes_window_desc; lengths in grid units.  axis: line of sight (0 x, 1 y, 2 z) of the ring,
+    disk, cylshell and cylinder windows (a0 = R, a1 = H); ignored by the others."""
+    if wtype in LOS_TYPES and int(axis) not in (0, 1, 2):
+        raise ValueError(f"line-of-sight axis must be 0, 1 or 2, got {axis!r}")
+    return _lib.WindowDesc(WIN_CODES[wtype], int(l), int(m), int(axis) if wtype in LOS_TYPES else 0, float(a0),
+                           float(a1))
 
 
 def _np_f64(a):
@@ -173,15 +178,23 @@ def spectrum_mul(spec, kernel, scale=1.0):
     return spec
 
 
+PAIR_WINDOWS_PER_PASS = 2048  # one fused pass keeps a sum per window and wave in 64 KB of LDS
+
+
 def pair_reduce(F1, F2, P, descs):
-    """sum_k K_b(k) Re(F1 conj F2) over the full grid, for every descriptor (host float64 array)."""
+    """sum_k K_b(k) Re(F1 conj F2) over the full grid, for every descriptor (host float64 array); longer
+    descriptor lists run as several fused passes of at most PAIR_WINDOWS_PER_PASS windows."""
     L = F1.shape[0]
-    n = len(descs)
-    arr = (_lib.WindowDesc * n)(*descs)
-    out = np.zeros(n, dtype=np.float64)
+    descs = list(descs)
+    out = np.zeros(len(descs), dtype=np.float64)
     dbl = 1 if F1.dtype == torch.complex128 else 0
-    _lib.call("hermes_pair_reduce", _lib.context(), _J_of(L), dbl, ptr(F1), ptr(F2), ptr(P), arr, n,
-              out.ctypes.data_as(_DP))
+    for b0 in range(0, len(descs), PAIR_WINDOWS_PER_PASS):
+        part = descs[b0:b0 + PAIR_WINDOWS_PER_PASS]
+        arr = (_lib.WindowDesc * len(part))(*part)
+        chunk = np.zeros(len(part), dtype=np.float64)
+        _lib.call("hermes_pair_reduce", _lib.context(), _J_of(L), dbl, ptr(F1), ptr(F2), ptr(P), arr, len(part),
+                  chunk.ctypes.data_as(_DP))
+        out[b0:b0 + len(part)] = chunk
     return out
 
 

@@ -1,16 +1,20 @@
 """Corr_2PCF: pair products with shell binning windows on the HIP device.
 
-Drop-in for pyhermes.theory.Corr_2PCF on its isotropic path (mapping 's_to_R';
-corr2pcf.py:226-382, :597-731, :936-1264).  Instead of one window build and two
-FFTs per bin (corr2pcf.py:366-372), each field is transformed once and every bin
-is evaluated in one fused Parseval pass (hermes_pair_reduce):
+Drop-in for pyhermes.theory.Corr_2PCF (corr2pcf.py:226-382, :597-731, :936-1264):
+samplings 's' (mapping 's_to_R', isotropic binning windows), ('s', 'mu') ('smu_to_RH')
+and ('rp', 'pi') ('rppi_to_RH') with the line-of-sight ring / disk / cylshell /
+cylinder windows, or a callable mapping; results have the shape of the sampling
+grid (np.ndindex order over the mapping's sampling names, corr2pcf.py:226-232).
+Instead of one window build and two FFTs per sample (corr2pcf.py:366-372), each
+field is transformed once and every sample is evaluated in one fused Parseval pass
+(hermes_pair_reduce):
   pp_b = sum_x (f1 @ W_b)(x) f2(x) / N = (1/N^2) sum_k W_b(k) Re F1(k) conj F2(k).
 Products dd, dr, rd, delta_dd, rr, xi follow the reference; uniform randoms are
 the scalar density rho; values are scaled by (L / box)^6.
 
 Extension (no reference counterpart): ``multipoles: [0, 2, 4]`` adds
 xi_l(s) = (-i)^l sqrt(4 pi (2l + 1)) pp_l(s) / rr from j_l(2 pi k s) Y_l0 windows,
-evaluated in the same fused pass.
+evaluated in the same fused pass (s sampling only).
 """
 
 import copy
@@ -23,12 +27,72 @@ from .. import kernels
 from ..io import Corr2PCFData, SFCField, WindowFunc, normalize_task_weight_normalization
 from ..pipeline import TaskBase
 from ..utils import func_util
-from ..utils.window_params import default_binning_window, normalize_binning_window_template
+from ..io.window import los_desc
+from ..utils.window_params import LOS_AWARE_WINDOW_TYPES, default_binning_window, normalize_binning_window_template
 
 PRODUCT_NAMES = ("dd", "dr", "rd", "delta_dd", "rr", "xi")
 PRODUCT_INPUT_FLAGS = {"dd": (True, False), "dr": (True, True), "rd": (True, True), "delta_dd": (True, True),
                        "rr": (False, True), "xi": (True, True)}
-ISO_BINNING = {"shell", "thick_shell", "sphere", "gaussian", "gaussian_shell"}
+NONNEGATIVE_SAMPLING_ARGS = ("s", "rp", "pi")
+
+
+# Built-in binning-window mappings (corr2pcf.py:267-322): a sample fills the window's unset lengths.
+def _fill(params, key, value):
+    if params["len_args"].get(key) is None:
+        params["len_args"][key] = value
+
+
+def map_s_to_R(sample, bw):
+    p = copy.deepcopy(bw)
+    p.setdefault("len_args", {})
+    _fill(p, "R", sample["s"])
+    return p
+
+
+def map_smu_to_RH(sample, bw):
+    p = copy.deepcopy(bw)
+    p.setdefault("len_args", {})
+    s, mu = sample["s"], sample["mu"]
+    _fill(p, "R", s * np.sqrt(max(0.0, 1.0 - mu * mu)))
+    _fill(p, "H", s * mu)
+    return p
+
+
+def map_rppi_to_RH(sample, bw):
+    p = copy.deepcopy(bw)
+    p.setdefault("len_args", {})
+    _fill(p, "R", sample["rp"])
+    _fill(p, "H", sample["pi"])
+    return p
+
+
+BINNING_MAPPINGS = {"s_to_R": (("s",), map_s_to_R), "smu_to_RH": (("s", "mu"), map_smu_to_RH),
+                    "rppi_to_RH": (("rp", "pi"), map_rppi_to_RH)}
+
+
+def binning_desc(params, L, box):
+    """Window descriptor of one sample's binning-window parameters (lengths in box units)."""
+    wtype = str(params.get("type", "")).strip().lower()
+    if params.get("func") is not None:
+        raise NotImplementedError("custom binning-window functions are outside pyhermes_amd's fused pair pass.")
+    la = {k: (None if v is None else float(v) * L / box) for k, v in (params.get("len_args") or {}).items()}
+
+    def need(*keys):
+        missing = [k for k in keys if la.get(k) is None]
+        if missing:
+            raise ValueError(f"binning window '{wtype}' needs len_args {missing} (set them or use a mapping).")
+        return [la[k] for k in keys]
+
+    if wtype in ("shell", "sphere", "gaussian"):
+        return kernels.window_desc(wtype, *need("R"))
+    if wtype == "thick_shell":
+        return kernels.window_desc(wtype, *need("R", "delta_R"))
+    if wtype == "gaussian_shell":
+        return kernels.window_desc(wtype, *need("R_shell", "R_smooth"))
+    if wtype in LOS_AWARE_WINDOW_TYPES and wtype in kernels.LOS_TYPES:
+        R, H = need("R", "H")
+        return los_desc(wtype, {"R": R, "H": H}, params.get("los_args") or {"nx": 0.0, "ny": 0.0, "nz": 1.0})
+    raise NotImplementedError(f"binning window type '{wtype}' is not available in pyhermes_amd's pair pass.")
 
 
 def normalize_products(products):
@@ -60,14 +124,22 @@ def expand_products(products):
 
 
 def normalize_sampling_spec(name, spec):
+    """corr2pcf.py:87-99, :210-223: {min, max, n} -> linspace, or a monotonic 1-D array."""
     if isinstance(spec, dict):
         n = int(spec["n"])
         if n <= 0:
             raise ValueError(f"'{name}' sampling n must be positive.")
         arr = np.linspace(float(spec["min"]), float(spec["max"]), n)
     else:
-        arr = np.atleast_1d(np.asarray(spec, dtype=np.float64))
-    if name in ("s", "rp", "pi") and np.any(arr < 0):
+        arr = np.asarray(spec, dtype=np.float64)
+        if arr.ndim != 1:
+            raise TypeError(f"'{name}' must be a 1D array-like input.")
+        if arr.size == 0:
+            raise ValueError(f"'{name}' must contain at least one sampling point.")
+        d = np.diff(arr)
+        if np.any(d < 0.0) and np.any(d > 0.0):
+            raise ValueError(f"'{name}' values must be monotonic.")
+    if name in NONNEGATIVE_SAMPLING_ARGS and np.any(arr < 0):
         raise ValueError(f"'{name}' sampling values must be non-negative.")
     return np.ascontiguousarray(arr, dtype=np.float64)
 
@@ -144,15 +216,30 @@ class Corr_2PCF(TaskBase):
 
     # ------------------------------------------------------------ inputs
     def _resolve_sampling(self):
+        """corr2pcf.py:453-473: the mapping names the sampling axes (a callable mapping takes the
+        sampling dict's own keys)."""
+        if not isinstance(self.sampling, dict) or not self.sampling:
+            raise ValueError("Corr_2PCF requires a non-empty 'sampling' dictionary.")
         mapping = self.binning_window.get("mapping")
-        if mapping != "s_to_R":
-            raise NotImplementedError(f"binning_window mapping '{mapping}' is outside pyhermes_amd's isotropic scope.")
-        if self.binning_window["type"] not in ISO_BINNING and self.binning_window.get("func") is None:
-            raise NotImplementedError(f"binning window type '{self.binning_window['type']}' is not isotropic.")
-        if not isinstance(self.sampling, dict) or set(self.sampling) != {"s"}:
-            raise ValueError(f"sampling keys must match mapping 's_to_R': expected ['s'], got {list(self.sampling)}.")
-        self.sampling_names = ("s",)
-        self.sampling_arrays = {"s": normalize_sampling_spec("s", self.sampling["s"])}
+        if isinstance(mapping, str):
+            if mapping not in BINNING_MAPPINGS:
+                raise ValueError(f"Unsupported binning_window mapping '{mapping}'. Supported built-in mappings: "
+                                 f"{sorted(BINNING_MAPPINGS)}.")
+            required, self._mapper = BINNING_MAPPINGS[mapping]
+        elif callable(mapping):
+            required, self._mapper = tuple(self.sampling.keys()), mapping
+        else:
+            raise TypeError("binning_window mapping must be a string or callable.")
+        missing = [n for n in required if n not in self.sampling]
+        extra = [n for n in self.sampling if n not in required]
+        if missing or extra:
+            raise ValueError(f"sampling keys must match mapping '{mapping}': expected {list(required)}, "
+                             f"missing={missing}, extra={extra}.")
+        self.sampling_names = tuple(required)
+        self.sampling_arrays = {n: normalize_sampling_spec(n, self.sampling[n]) for n in self.sampling_names}
+        self.sampling_shape = tuple(self.sampling_arrays[n].size for n in self.sampling_names)
+        if self.multipoles and self.sampling_names != ("s",):
+            raise ValueError("Corr_2PCF multipoles need the 's' sampling (mapping 's_to_R').")
 
     def _load(self, value, label):
         if isinstance(value, SFCField):
@@ -218,20 +305,17 @@ class Corr_2PCF(TaskBase):
 
     # ------------------------------------------------------------ products
     def _descs(self, L, box):
-        s = self.sampling_arrays["s"]
-        bw = self.binning_window
-        wtype = bw["type"]
+        """One descriptor per sample, in np.ndindex order over the sampling axes, then the multipoles."""
+        bw = {k: v for k, v in self.binning_window.items() if k != "mapping"}
         descs = []
-        for sv in s:
-            R = float(sv if bw["len_args"].get("R") is None else bw["len_args"]["R"]) * L / box
-            if wtype == "thick_shell":
-                descs.append(kernels.window_desc(wtype, R, float(bw["len_args"]["delta_R"]) * L / box))
-            elif wtype == "gaussian_shell":
-                descs.append(kernels.window_desc(wtype, float(bw["len_args"]["R_shell"]) * L / box,
-                                                 float(bw["len_args"]["R_smooth"]) * L / box))
-            else:
-                descs.append(kernels.window_desc(wtype, R))
+        for idx in np.ndindex(self.sampling_shape):
+            sample = {n: float(self.sampling_arrays[n][i]) for n, i in zip(self.sampling_names, idx)}
+            params = self._mapper(sample, bw)
+            if not isinstance(params, dict):
+                raise TypeError("binning_window mapping must return a binning_window dictionary.")
+            descs.append(binning_desc(params, L, box))
         ells = list(self.multipoles or [])
+        s = self.sampling_arrays.get("s")
         for l in ells:
             if int(l) % 2:
                 raise ValueError("Corr_2PCF multipoles supports even l only (xi_l of an auto-correlation is real).")
@@ -251,7 +335,7 @@ class Corr_2PCF(TaskBase):
             v for v in self.randoms.values() if not _is_density(v))
         L, box = int(ref.L), float(ref.box_size)
         descs, ells = self._descs(L, box)
-        ns = self.sampling_arrays["s"].size
+        ns = int(np.prod(self.sampling_shape))
         cache = {}
         delta = {}
         if "delta_dd" in expanded:
@@ -274,7 +358,7 @@ class Corr_2PCF(TaskBase):
         data.set_sampling(self.sampling_names, self.sampling_arrays)
         for p in PRODUCT_NAMES:
             v = out.get(p)
-            setattr(data, p, None if v is None else v[:ns].copy())
+            setattr(data, p, None if v is None else v[:ns].reshape(self.sampling_shape).copy())
         if "xi" in expanded:
             data.xi = data.delta_dd / data.rr
         if ells:

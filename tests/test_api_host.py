@@ -67,8 +67,14 @@ def test_thick_shell_validation_and_custom_override():
     assert np.all(np.isfinite(w._host_custom_kernel()))
     with pytest.raises(ValueError, match="requires finite len_args"):
         WindowFunc({"type": "thick_shell", "len_args": {"R_in": 8.0, "R_out": 12.0}}, sfc)
+    cyl = WindowFunc({"type": "cylinder", "len_args": {"R": 1.0, "H": 2.0}}, sfc)
+    assert cyl.los_args == {"nx": 0.0, "ny": 0.0, "nz": 1.0}  # the default line of sight
+    with pytest.raises(NotImplementedError, match="coordinate axis"):
+        WindowFunc({"type": "ring", "len_args": {"R": 1.0, "H": 2.0}, "los_args": [1.0, 1.0, 0.0]}, sfc)
+    with pytest.raises(ValueError, match="requires finite len_args"):
+        WindowFunc({"type": "disk", "len_args": {"R": 1.0}}, sfc)
     with pytest.raises(NotImplementedError):
-        WindowFunc({"type": "cylinder", "len_args": {"R": 1.0, "H": 2.0}}, sfc)
+        WindowFunc({"type": "cubic", "len_args": {"R": 1.0}}, sfc)
 
 
 def test_custom_octant_kernel_equals_alias_sum(phi):
@@ -232,3 +238,36 @@ def test_result_save_load_roundtrip(tmp_path):
     f.load_corr3pcf(q)
     np.testing.assert_array_equal(f.zeta, c.zeta)
     np.testing.assert_array_equal(f.mu, c.mu)
+
+
+def test_corr2pcf_anisotropic_sampling_and_descriptors():
+    """(s, mu) and (rp, pi) mappings (corr2pcf.py:267-322): sampling axes, grid order and the descriptors of
+    the fused pass, built on the host."""
+    from pyhermes_amd import kernels
+    from pyhermes_amd.theory import Corr_2PCF
+    t = Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform", "binning_window": "ring",
+                                 "sampling": {"mu": [0.0, 0.5, 1.0], "s": {"min": 10.0, "max": 20.0, "n": 2}}}})
+    t._resolve_sampling()
+    assert t.sampling_names == ("s", "mu") and t.sampling_shape == (2, 3)
+    descs, ells = t._descs(64, 128.0)  # grid units = 0.5 x box units
+    assert ells == [] and len(descs) == 6
+    want = [(s * np.sqrt(1 - m * m) / 2, s * m / 2) for s in (10.0, 20.0) for m in (0.0, 0.5, 1.0)]
+    for d, (R, H) in zip(descs, want):
+        assert d.type == kernels.WIN_CODES["ring"] and d.axis == 2
+        assert d.a0 == pytest.approx(R, abs=1e-12) and d.a1 == pytest.approx(H, abs=1e-12)
+    t = Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform",
+                                 "binning_window": {"type": "cylinder", "mapping": "rppi_to_RH",
+                                                    "los_args": {"nx": 0.0, "ny": -1.0, "nz": 0.0}},
+                                 "sampling": {"rp": [5.0, 10.0], "pi": [0.0, 8.0]}}})
+    t._resolve_sampling()
+    descs, _ = t._descs(64, 64.0)
+    assert [(d.a0, d.a1, d.axis) for d in descs] == [(5.0, 0.0, 1), (5.0, 8.0, 1), (10.0, 0.0, 1), (10.0, 8.0, 1)]
+    with pytest.raises(ValueError, match="must match mapping"):
+        Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform", "binning_window": "ring",
+                                 "sampling": {"s": [10.0]}}})._resolve_sampling()
+    with pytest.raises(ValueError, match="non-negative"):
+        Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform", "binning_window": "disk",
+                                 "sampling": {"s": [-1.0, 2.0], "mu": [0.5]}}})._resolve_sampling()
+    with pytest.raises(ValueError, match="multipoles"):
+        Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform", "binning_window": "ring",
+                                 "multipoles": [0, 2], "sampling": {"s": [10.0], "mu": [0.5]}}})._resolve_sampling()

@@ -107,6 +107,47 @@ def test_corr2pcf_task(phi):
         assert res.xi_l[1][j] == pytest.approx(ref2, rel=1e-8, abs=1e-12)
 
 
+def test_corr2pcf_smu_and_rppi_binning(phi):
+    """Anisotropic binning (corr2pcf.py:267-322 mappings): xi(s, mu) with ring windows and xi(rp, pi) with
+    cylinder-shell windows against the oracle, result shapes in the sampling grid, and the geometry's
+    identity: the Gauss-Legendre mu-average of the ring pair products is the shell pair product."""
+    from pyhermes_amd.theory import Corr_2PCF
+    print("Corr_2PCF J=5 (s, mu) rings and (rp, pi) cylinder shells", flush=True)
+    box, J = 400.0, 5
+    L = 1 << J
+    pos, w = _catalogue(30000, box, 21)
+    field = _project(pos, w, J, box)
+    s = np.array([40.0, 75.0])
+    mu, wmu = np.polynomial.legendre.leggauss(64)  # exact to degree 127: enough for 2 pi |k| s <= 35
+    mu, wmu = 0.5 * (mu + 1.0), 0.5 * wmu
+    smu = Corr_2PCF({"Corr_2PCF": {"sfc_field": field, "random": "uniform", "binning_window": "ring",
+                                   "sampling": {"s": s, "mu": mu}, "products": ["xi", "dd"], "fout_path": ""}})
+    r = smu.run(save_result=False)
+    assert r.xi.shape == (2, 64) and r.dd.shape == (2, 64)
+    iso = Corr_2PCF({"Corr_2PCF": {"sfc_field": field, "random": "uniform", "sampling": {"s": s},
+                                   "products": ["xi", "dd"], "fout_path": ""}}).run(save_result=False)
+    np.testing.assert_allclose(r.xi @ wmu, iso.xi, rtol=1e-9, atol=1e-12)
+    np.testing.assert_allclose(r.dd @ wmu, iso.dd, rtol=1e-11)
+    eps = field.epsilon
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    sc = (L / box) ** 6
+    for i, j in ((0, 0), (1, 7), (1, 63)):
+        R, H = s[i] * np.sqrt(1 - mu[j] ** 2) * L / box, s[i] * mu[j] * L / box
+        ref = ow.pair_product(eps, eps, ow.los_rfft_kernel(L, P, "ring", R, H)) * sc
+        assert r.dd[i, j] == pytest.approx(ref, rel=1e-10)
+    rp = np.linspace(20.0, 80.0, 4)
+    pi = np.linspace(0.0, 60.0, 3)
+    cyl = Corr_2PCF({"Corr_2PCF": {"sfc_field": field, "random": "uniform",
+                                   "binning_window": {"type": "cylshell", "mapping": "rppi_to_RH",
+                                                      "los_args": [0.0, 0.0, 1.0]},
+                                   "sampling": {"rp": rp, "pi": pi}, "products": ["dd"], "fout_path": ""}})
+    rc = cyl.run(save_result=False)
+    assert rc.dd.shape == (4, 3) and tuple(rc.sampling_names) == ("rp", "pi")
+    for i, j in ((0, 0), (2, 1), (3, 2)):
+        ref = ow.pair_product(eps, eps, ow.los_rfft_kernel(L, P, "cylshell", rp[i] * L / box, pi[j] * L / box)) * sc
+        assert rc.dd[i, j] == pytest.approx(ref, rel=1e-10)
+
+
 def _multipole_case(phi, backend, products):
     from pyhermes_amd.theory import Corr_3PCF_Multipole
     box, J = 160.0, 4

@@ -142,6 +142,64 @@ def test_pair_reduce_uniform_shells_fp64(phi, J):
         assert abs(got[i] / L ** 6 - ref) <= 1e-11 * scale / L ** 6, (i, radii[i])
 
 
+LOS_CASES = [("ring", 1.5, 2.5, 2), ("disk", 3.0, 1.25, 2), ("cylshell", 2.25, 3.5, 0), ("cylinder", 6.0, 0.75, 1),
+             ("ring", 40.0, 9.0, 2)]  # the last reaches J0/J1 arguments of several hundred (asymptotic range)
+
+
+@pytest.mark.parametrize("wt,R,H,axis", LOS_CASES)
+def test_build_rfft_los_windows_match_oracle(phi, wt, R, H, axis):
+    """Line-of-sight windows (window_functions.py:164-319) in the rFFT layout against the oracle's signed
+    8-alias sums (convolution.py:73-157) with scipy's J0/J1, FP64 and FP32."""
+    from pyhermes_amd import kernels
+    J = 5
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    los = tuple(1.0 if i == axis else 0.0 for i in range(3))
+    ref = ow.los_rfft_kernel(L, P, wt, R, H, los)
+    desc = kernels.window_desc(wt, R, H, axis=axis)
+    got = kernels.window_build_rfft(J, Pd, desc, dbl=True).cpu().numpy()
+    assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), np.abs(got - ref).max()
+    got32 = kernels.window_build_rfft(J, Pd, desc, dbl=False).cpu().numpy()
+    assert np.abs(got32 - ref).max() <= 1e-6 * np.abs(ref).max()
+
+
+@pytest.mark.parametrize("dbl", [True, False])
+def test_pair_reduce_los_windows_match_oracle(phi, dbl):
+    """The fused pair pass over ring / disk / cylshell / cylinder windows (several axes) equals the oracle's
+    convolution pair products."""
+    import torch
+    from pyhermes_amd import kernels
+    J = 5
+    L = 1 << J
+    P, Pd = _P(phi, L)
+    rng = np.random.default_rng(17)
+    f1 = rng.normal(size=(L, L, L))
+    f2 = f1 + 0.4 * rng.normal(size=(L, L, L))
+    td = torch.float64 if dbl else torch.float32
+    F1 = kernels.rfftn(kernels.as_device(f1, td))
+    F2 = kernels.rfftn(kernels.as_device(f2, td))
+    print(f"LOS pair reduce J={J} dbl={dbl}", flush=True)
+    got = kernels.pair_reduce(F1, F2, Pd, [kernels.window_desc(wt, R, H, axis=a) for wt, R, H, a in LOS_CASES])
+    got /= L ** 6
+    scale = np.sqrt(np.mean(f1 ** 2) * np.mean(f2 ** 2))
+    tol = 1e-11 if dbl else 2e-5
+    for (wt, R, H, a), g in zip(LOS_CASES, got):
+        los = tuple(1.0 if i == a else 0.0 for i in range(3))
+        ref = ow.pair_product(f1, f2, ow.los_rfft_kernel(L, P, wt, R, H, los))
+        assert abs(g - ref) <= tol * scale, (wt, R, H, a, g, ref)
+
+
+def test_rejects_bad_los_axis():
+    import torch
+    from pyhermes_amd import kernels
+    L = 16
+    Pd = kernels.as_device(np.ones(L + 1), torch.float64)
+    bad = kernels.window_desc("ring", 1.0, 1.0)
+    bad.axis = 3
+    with pytest.raises(RuntimeError, match="axis"):
+        kernels.window_build_rfft(4, Pd, bad)
+
+
 def test_pair_reduce_rejects_odd_multipoles():
     import torch
     from pyhermes_amd import kernels

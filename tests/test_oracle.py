@@ -444,3 +444,48 @@ def test_bucketize_oracle_matches_definition(phi):
     key = cb[cells].astype(np.int64) * Ls ** 3 + cells
     assert np.all(np.diff(key) > 0)
     assert off[-1] == Ls ** 3 and np.array_equal(np.diff(off), np.bincount(cb, minlength=4))
+
+
+def test_los_windows_obey_integral_identities():
+    """Each line-of-sight window is pinned to another through an identity of the geometry:
+    the mu-average of rings (R = s sqrt(1 - mu^2), H = s mu) is the spherical shell sin(q)/q; the
+    area average of rings of radius r <= R is the disk; the average over heights |h| <= H of a ring
+    (disk) is the cylinder shell (cylinder)."""
+    rng = np.random.default_rng(8)
+    k = rng.uniform(-0.45, 0.45, (40, 3))
+    ki, kj, kk = k.T
+    kn = np.linalg.norm(k, axis=1)
+    mu, wmu = np.polynomial.legendre.leggauss(160)
+    mu, wmu = 0.5 * (mu + 1.0), 0.5 * wmu  # [0, 1]
+    for s in (1.7, 6.0):
+        ring = sum(w * windows.los_window("ring", ki, kj, kk, s * np.sqrt(1 - m * m), s * m) for m, w in zip(mu, wmu))
+        q = 2 * np.pi * kn * s
+        np.testing.assert_allclose(ring, np.sin(q) / q, rtol=0, atol=1e-12)
+    R, H = 3.1, 2.4
+    t, wt = np.polynomial.legendre.leggauss(160)
+    r, wr = 0.5 * R * (t + 1.0), 0.5 * R * wt
+    disk = sum(w * 2 * rr / R ** 2 * windows.los_window("ring", ki, kj, kk, rr, H) for rr, w in zip(r, wr))
+    np.testing.assert_allclose(windows.los_window("disk", ki, kj, kk, R, H), disk, rtol=0, atol=1e-12)
+    h, wh = H * t, H * wt
+    for avg, base in (("cylshell", "ring"), ("cylinder", "disk")):
+        ref = sum(w / (2 * H) * windows.los_window(base, ki, kj, kk, R, hh) for hh, w in zip(h, wh))
+        np.testing.assert_allclose(windows.los_window(avg, ki, kj, kk, R, H), ref, rtol=0, atol=1e-12)
+    # the line of sight is a direction: along x it is the z-axis window with the axes exchanged
+    for wt_ in windows.LOS_WINDOW_TYPES:
+        np.testing.assert_allclose(windows.los_window(wt_, ki, kj, kk, R, H, los=(2.0, 0.0, 0.0)),
+                                   windows.los_window(wt_, kk, kj, ki, R, H), rtol=0, atol=1e-15)
+    assert windows.los_window("disk", 0.0, 0.0, 0.0, R, H) == 1.0
+    assert windows.los_window("cylinder", 0.0, 0.0, 0.0, R, H) == 1.0
+
+
+def test_los_kernel_mu_average_is_the_shell_kernel(phi):
+    """At kernel level the Gauss-Legendre mu-average of the ring kernels (convolution.py:73-157 alias sums)
+    equals the octant shell kernel of the C oracle (convolution.py:14-38 + :263-281)."""
+    L = 8
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    s = 2.6
+    mu, wmu = np.polynomial.legendre.leggauss(96)
+    mu, wmu = 0.5 * (mu + 1.0), 0.5 * wmu
+    avg = sum(w * windows.los_rfft_kernel(L, P, "ring", s * np.sqrt(1 - m * m), s * m) for m, w in zip(mu, wmu))
+    shell = windows.octant_rfft_kernel(L, P, "shell", s)
+    assert np.abs(avg - shell).max() <= 1e-11 * np.abs(shell).max()
