@@ -142,3 +142,17 @@ def bucketize(grid, phi, sub, edges, phi_res=1024):
     counts = np.bincount(cb, minlength=len(edges) + 1)
     off = np.concatenate([[0], np.cumsum(counts)]).astype(np.int64)
     return cb, cells, off, v
+
+
+def triplet_particle_centers(R1, R2, mu, centers, weights, rho1, abc, e2, e3, phi, phi_res=1024):
+    """corr3pcf_kernels.py:66-97 with the rotation uniforms supplied: the first vertex sits on each centre
+    with its weight, rho1 sum_rot sum_i w_i n2(c_i + d12) n3(c_i + d13) / (n_rot sum_i w_i)."""
+    c = np.asarray(centers, dtype=np.float64)
+    w = np.asarray(weights, dtype=np.float64)
+    total = 0.0
+    for a, b, g in np.asarray(abc, dtype=np.float64):
+        o = triangle_offsets(R1, R2, mu, 2.0 * np.pi * a, 2.0 * b - 1.0, 2.0 * np.pi * g)
+        n2 = interpolate(e2, c, phi, offset=(o[0], o[1], o[2]), phi_res=phi_res)
+        n3 = interpolate(e3, c, phi, offset=(o[3], o[4], o[5]), phi_res=phi_res)
+        total += float(np.sum(w * n2 * n3))
+    return rho1 * total / (len(abc) * float(np.sum(w)))

@@ -1,6 +1,6 @@
-"""Corr_3PCF: real-space 3PCF by Monte-Carlo triangles around box-random centres.
+"""Corr_3PCF: real-space 3PCF by Monte-Carlo triangles around box-random or particle centres.
 
-Drop-in for pyhermes.theory.Corr_3PCF with center='box_random' (corr3pcf.py
+Drop-in for pyhermes.theory.Corr_3PCF (corr3pcf.py
 :154-203 parameters, :288-322 angle sampling, :1105-1132 per-mu products,
 :1183-1215 pair cache, :1400-1620 run).  For every mu the reference loops over
 n_rot rotations and all centres in numba; here one HIP launch
@@ -15,7 +15,17 @@ n_rot rotations and all centres in numba; here one HIP launch
 Products ddd, rrr, delta_ddd, xi12, xi13, xi23, zeta, zeta_H, Q follow the
 reference's dependency rules; triplet products are scaled by (L / box)^9 and
 the pair caches use thin-shell windows evaluated in one fused pass.
-center='particle' (particle-position centres) is not provided by this package.
+
+center='particle' (corr3pcf.py:1135-1180, :1405-1460, :1596-1614; the estimator of
+corr3pcf_kernels.py:66-97): vertex 1 sits on the catalogue's own points with their
+weights, rho1 sum_rot sum_i w_i n2(c_i + d12) n3(c_i + d13) / (n_rot sum_i w_i),
+rho1 = sum_i w_i / L^3 -- the same HIP launch with the weights in place of n1.
+Products add d_delta_dd (data centres on delta2, delta3) and r_delta_dd (random
+centres; rrr * xi23 when random1 is a uniform density and no random_pos1 is
+given; the data centres when random1 is a field), and delta_ddd = d_delta_dd -
+r_delta_dd.  Centres come as particle_pos1 / particle_weight1 (and optionally
+random_pos1 / random_weight1) in box units; reading them from a field's saved
+particle data is not provided.
 """
 
 import copy
@@ -31,10 +41,19 @@ from ..utils import func_util
 from .corr2pcf import shell_pair_products
 
 PRODUCT_RULES = {
-    "allowed": {"ddd", "rrr", "delta_ddd", "xi12", "xi13", "xi23", "zeta", "zeta_H", "Q"},
-    "deps": {"Q": ["zeta", "zeta_H"], "zeta_H": ["xi12", "xi13", "xi23"], "zeta": ["delta_ddd", "rrr"]},
+    "box_random": {
+        "allowed": {"ddd", "rrr", "delta_ddd", "xi12", "xi13", "xi23", "zeta", "zeta_H", "Q"},
+        "deps": {"Q": ["zeta", "zeta_H"], "zeta_H": ["xi12", "xi13", "xi23"], "zeta": ["delta_ddd", "rrr"]},
+    },
+    "particle": {
+        "allowed": {"ddd", "rrr", "d_delta_dd", "r_delta_dd", "delta_ddd", "xi12", "xi13", "xi23", "zeta", "zeta_H",
+                    "Q"},
+        "deps": {"Q": ["zeta", "zeta_H"], "zeta_H": ["xi12", "xi13", "xi23"], "zeta": ["delta_ddd", "rrr"],
+                 "delta_ddd": ["d_delta_dd", "r_delta_dd"]},
+    },
 }
-PRODUCT_INPUT_FLAGS = {"ddd": (True, False), "rrr": (False, True), "delta_ddd": (True, True), "xi12": (True, True),
+PRODUCT_INPUT_FLAGS = {"ddd": (True, False), "rrr": (False, True), "d_delta_dd": (True, True),
+                       "r_delta_dd": (True, True), "delta_ddd": (True, True), "xi12": (True, True),
                        "xi13": (True, True), "xi23": (True, True), "zeta": (True, True), "zeta_H": (True, True),
                        "Q": (True, True)}
 ROTATION_SEED_STRIDE = 1000003
@@ -91,10 +110,10 @@ class Corr_3PCF(TaskBase):
         self.mu = copy.deepcopy(tp.get("mu"))
         self.n_rot = int(tp["n_rot"])
         self.center = str(tp.get("center", "box_random")).strip().lower()
-        if self.center == "particle":
-            raise NotImplementedError("Corr_3PCF center='particle' is not provided by pyhermes_amd; use 'box_random'.")
-        if self.center != "box_random":
+        if self.center not in ("box_random", "particle"):
             raise ValueError(f"Unknown center='{self.center}'. Use 'box_random' or 'particle'.")
+        for key in ("particle_pos1", "particle_weight1", "random_pos1", "random_weight1"):
+            setattr(self, key, tp.get(key, None))
         self.n_box_centers = int(tp["n_box_centers"])
         self.base_seed = int(tp["base_seed"])
         self.threads = int(tp["threads"])
@@ -115,9 +134,9 @@ class Corr_3PCF(TaskBase):
                 raise TypeError("Each product name must be a string.")
             raw = item.strip()
             name = "Q" if raw.upper() == "Q" else ("zeta_H" if raw.lower() == "zeta_h" else raw.lower())
-            if name not in PRODUCT_RULES["allowed"]:
+            if name not in PRODUCT_RULES[self.center]["allowed"]:
                 raise ValueError(f"Unsupported product '{item}' for center='{self.center}'. Allowed values are "
-                                 f"{sorted(PRODUCT_RULES['allowed'])}.")
+                                 f"{sorted(PRODUCT_RULES[self.center]['allowed'])}.")
             if name not in out:
                 out.append(name)
         return out
@@ -126,7 +145,7 @@ class Corr_3PCF(TaskBase):
         out = list(self.products)
         i = 0
         while i < len(out):
-            for dep in PRODUCT_RULES["deps"].get(out[i], []):
+            for dep in PRODUCT_RULES[self.center]["deps"].get(out[i], []):
                 if dep not in out:
                     out.append(dep)
             i += 1
@@ -203,9 +222,38 @@ class Corr_3PCF(TaskBase):
             func_util.safe_exit(1)
         return field
 
+    def _centre_arrays(self, pos, weight, pos_name, weight_name):
+        """corr3pcf.py:550-572: (N, 3) positions and a positive-sum weight array, given together."""
+        if (pos is None) != (weight is None):
+            raise ValueError(f"{pos_name} and {weight_name} must be provided together for center='particle'.")
+        if pos is None:
+            return None
+        p = np.asarray(pos, dtype=np.float64)
+        if p.ndim != 2 or p.shape[1] != 3:
+            raise ValueError(f"{pos_name} must be an array-like with shape (N, 3), got shape {p.shape}.")
+        w = np.asarray(weight, dtype=np.float64)
+        if w.ndim != 1 or w.shape[0] != p.shape[0]:
+            raise ValueError(f"{weight_name} must be a 1D array with length {p.shape[0]}, got shape {w.shape}.")
+        if not float(np.sum(w)) > 0.0:
+            raise ValueError(f"{weight_name} must have a positive sum.")
+        return np.ascontiguousarray(p), np.ascontiguousarray(w)
+
     def prepare_input_fields(self, sfc_field1=None, sfc_field2=None, sfc_field3=None, random1=None, random2=None,
-                             random3=None, window1=None, window2=None, window3=None):
+                             random3=None, window1=None, window2=None, window3=None, particle_pos1=None,
+                             particle_weight1=None, random_pos1=None, random_weight1=None):
         self.corr3pcf_data = Corr3PCFData()
+        if self.center == "particle":
+            for key, val in (("particle_pos1", particle_pos1), ("particle_weight1", particle_weight1),
+                             ("random_pos1", random_pos1), ("random_weight1", random_weight1)):
+                if val is not None:
+                    setattr(self, key, val)
+            self._data_centres = self._centre_arrays(self.particle_pos1, self.particle_weight1, "particle_pos1",
+                                                     "particle_weight1")
+            self._random_centres = self._centre_arrays(self.random_pos1, self.random_weight1, "random_pos1",
+                                                       "random_weight1")
+            if self._data_centres is None and ({"ddd", "d_delta_dd", "r_delta_dd"} & set(self._expanded_products())):
+                raise ValueError("center='particle' needs particle_pos1 and particle_weight1 (box units); reading "
+                                 "centres from a field's saved particle data is not provided by pyhermes_amd.")
         self._resolve_angle_sampling()
         exp = self._expanded_products()
         needs_data = any(PRODUCT_INPUT_FLAGS[p][0] for p in exp)
@@ -307,21 +355,32 @@ class Corr_3PCF(TaskBase):
         geo = self.geometry
         L = int(geo.L)
         dev = kernels._dev()
-        centres = np.random.default_rng(seed=self.base_seed + CENTRE_SEED_OFFSET).uniform(
-            0.0, float(L), size=(self.n_box_centers, 3))
-        self._centres = kernels.as_device(centres, torch.float64)
+        particle = self.center == "particle"
+        if not particle:
+            centres = np.random.default_rng(seed=self.base_seed + CENTRE_SEED_OFFSET).uniform(
+                0.0, float(L), size=(self.n_box_centers, 3))
+            self._centres = kernels.as_device(centres, torch.float64)
         self._phi = kernels.as_device(np.asarray(geo.phi_array, dtype=np.float64), torch.float64)
-        defer_rrr = "rrr" in exp and "xi23" in exp and _is_density(self.randoms.get(1))
+        # particle centres with a uniform random vertex 1 and no random positions: r_delta_dd = rrr xi23
+        # (corr3pcf.py:1236-1244, :1607-1610), which needs both
+        r_shortcut = (particle and "r_delta_dd" in exp and self._random_centres is None
+                      and _is_density(self.randoms.get(1)))
+        want_rrr = "rrr" in exp or r_shortcut
+        want_xi23 = "xi23" in exp or r_shortcut
+        defer_rrr = want_rrr and want_xi23 and _is_density(self.randoms.get(1))
         results = {}
         cache = {}
-        if "ddd" in exp:
-            g = [self._grid(self.fields[i]) for i in (1, 2, 3)]
-            results["ddd"] = self._triplet(*g)
-        if "delta_ddd" in exp:
-            d = [self._delta_cached(self.fields[i], self.randoms[i], cache) for i in (1, 2, 3)]
-            results["delta_ddd"] = self._triplet(*(self._grid(x) for x in d))
-        if "rrr" in exp and not defer_rrr:
-            results["rrr"] = self._rrr(cache)
+        if particle:
+            results = self._particle_products(exp, r_shortcut, want_rrr and not defer_rrr, cache)
+        else:
+            if "ddd" in exp:
+                g = [self._grid(self.fields[i]) for i in (1, 2, 3)]
+                results["ddd"] = self._triplet(*g)
+            if "delta_ddd" in exp:
+                d = [self._delta_cached(self.fields[i], self.randoms[i], cache) for i in (1, 2, 3)]
+                results["delta_ddd"] = self._triplet(*(self._grid(x) for x in d))
+            if "rrr" in exp and not defer_rrr:
+                results["rrr"] = self._rrr(cache)
         scale = getattr(self, "_density_scale", 1.0) ** 3
         if not np.isclose(scale, 1.0):
             results = {k: v * scale for k, v in results.items()}
@@ -334,8 +393,8 @@ class Corr_3PCF(TaskBase):
             pair["xi13"] = {k: v[0] for k, v in self._pair_stats(self.fields[1], self.fields[3], self.randoms[1],
                                                                  self.randoms[3], [self.r13], cache).items()}
         rr23 = None
-        if "xi23" in exp or "rrr" in exp:
-            if "xi23" in exp or defer_rrr:
+        if want_xi23 or want_rrr:
+            if want_xi23 or defer_rrr:
                 pair["xi23"] = self._pair_stats(self.fields[2], self.fields[3], self.randoms[2], self.randoms[3],
                                                 r23, cache)
                 rr23 = pair["xi23"]["rr"]
@@ -345,31 +404,80 @@ class Corr_3PCF(TaskBase):
         data.ddd = results.get("ddd")
         data.rrr = results.get("rrr")
         data.delta_ddd = results.get("delta_ddd")
+        data.d_delta_dd = results.get("d_delta_dd")
+        data.r_delta_dd = results.get("r_delta_dd")
         data.xi12 = pair.get("xi12", {}).get("xi")
         data.xi13 = pair.get("xi13", {}).get("xi")
-        data.xi23 = pair.get("xi23", {}).get("xi") if "xi23" in exp else None
-        if "rrr" in exp and data.rrr is None:
-            data.rrr = float(self.randoms[1]) * rr23 * scale
+        xi23 = pair.get("xi23", {}).get("xi")
+        data.xi23 = xi23 if "xi23" in exp else None
+        rrr = data.rrr
+        if want_rrr and rrr is None:
+            rrr = float(self.randoms[1]) * rr23 * scale
+        data.rrr = rrr if "rrr" in exp else None
+        if r_shortcut:
+            data.r_delta_dd = rrr * xi23
+        if particle and "delta_ddd" in exp:
+            data.delta_ddd = data.d_delta_dd - data.r_delta_dd
         if "zeta" in exp:
-            data.zeta = data.delta_ddd / data.rrr
+            data.zeta = data.delta_ddd / rrr
         if "zeta_H" in exp:
             x12, x13, x23 = data.xi12, data.xi13, data.xi23
             data.zeta_H = x12 * x13 + x12 * x23 + x13 * x23
         if "Q" in exp:
             data.Q = data.zeta / data.zeta_H
         data.corr3pcf_info = {"center": self.center, "r12": self.r12, "r13": self.r13, "n_rot": self.n_rot,
-                              "n_box_centers": self.n_box_centers, "base_seed": self.base_seed,
+                              "n_box_centers": None if particle else self.n_box_centers, "base_seed": self.base_seed,
                               "products": list(self.products), "angle_param": self.angle_param}
         if save_result and self.fout_path:
             data.save_corr3pcf(self.fout_path, overwrite=overwrite)
         self.logger.info(f"The time for task: {time.perf_counter() - t0:.4f} sec")
         return data
 
-    def _rrr(self, cache):
-        """corr3pcf.py:736-772 shortcuts; three explicit random fields run the triplet kernel."""
+    def _particle_estimate(self, centres, g2, g3, rho1):
+        """corr3pcf_kernels.py:66-97 per mu: rho1 sum_rot sum_i w_i g2 g3 / (n_rot sum w) (grid units)."""
+        pos, w = centres
+        sf = float(self.geometry.scale_factor)
+        R1, R2 = self.r12 * sf, self.r13 * sf
+        pos_d = kernels.as_device(pos * sf, torch.float64)
+        w_d = kernels.as_device(w, torch.float64)
+        n, wsum = pos.shape[0], float(np.sum(w))
+        out = np.empty(self.mu_arr.size)
+        for it, mu in enumerate(self.mu_arr):
+            abc = rotation_draws(self.base_seed + 1, self.n_rot, it)
+            mean = kernels.mc_triplet_box(R1, R2, float(mu), pos_d, w_d, abc, g2, g3, self._phi)
+            out[it] = rho1 * mean * n / wsum
+        return out
+
+    def _particle_products(self, exp, r_shortcut, want_rrr, cache):
+        """corr3pcf.py:1135-1180 with the densities of :1430-1450 (V = L^3 in grid units)."""
+        V = float(self.geometry.L) ** 3
+        data_c = self._data_centres
+        rand_c = self._random_centres if self._random_centres is not None else data_c
+        rho1_data = None if data_c is None else float(np.sum(data_c[1])) / V
+        rho1_rand = float(np.sum(rand_c[1])) / V if rand_c is not None else None
+        out = {}
+        if "ddd" in exp:
+            out["ddd"] = self._particle_estimate(data_c, self._grid(self.fields[2]), self._grid(self.fields[3]),
+                                                 rho1_data)
+        if "d_delta_dd" in exp or ("r_delta_dd" in exp and not r_shortcut):
+            d2 = self._grid(self._delta_cached(self.fields[2], self.randoms[2], cache))
+            d3 = self._grid(self._delta_cached(self.fields[3], self.randoms[3], cache))
+            if "d_delta_dd" in exp:
+                out["d_delta_dd"] = self._particle_estimate(data_c, d2, d3, rho1_data)
+            if "r_delta_dd" in exp and not r_shortcut:
+                out["r_delta_dd"] = self._particle_estimate(rand_c, d2, d3, rho1_rand)
+        if want_rrr:
+            out["rrr"] = self._rrr(cache, rho1=rho1_rand, centres=rand_c)
+        return out
+
+    def _rrr(self, cache, rho1=None, centres=None):
+        """corr3pcf.py:736-772 shortcuts; three explicit random fields run the triplet kernel (box-random
+        centres, or the particle centres with their weights and rho1)."""
         r = [self.randoms[i] for i in (1, 2, 3)]
         dens = [_is_density(x) for x in r]
         rho = [float(x) if _is_density(x) else float(x.field_mean_density(value_unit="grid")) for x in r]
+        if rho1 is not None:
+            rho[0] = float(rho1)
         L, box = int(self.geometry.L), float(self.geometry.box_size)
         n = self.mu_arr.size
         if sum(dens) >= 2:
@@ -381,4 +489,6 @@ class Corr_3PCF(TaskBase):
             return np.full(n, rho[1] * shell_pair_products(r[0], r[2], [self.r13], L, box, cache)[0])
         if dens[2]:
             return np.full(n, rho[2] * shell_pair_products(r[0], r[1], [self.r12], L, box, cache)[0])
+        if centres is not None:
+            return self._particle_estimate(centres, self._grid(r[1]), self._grid(r[2]), rho[0])
         return self._triplet(*(self._grid(x) for x in r))

@@ -148,8 +148,14 @@ def test_corr3pcf_products_and_angles():
     assert t._expanded_products() == ["Q", "zeta", "zeta_H", "delta_ddd", "rrr", "xi12", "xi13", "xi23"]
     t._resolve_angle_sampling()
     np.testing.assert_allclose(t.mu_arr, np.cos(np.linspace(0, 3.0, 4)))
-    with pytest.raises(NotImplementedError):
-        Corr_3PCF(_3pcf_params(center="particle"))
+    tp = Corr_3PCF(_3pcf_params(center="particle", products=["zeta"]))
+    assert tp._expanded_products() == ["zeta", "delta_ddd", "rrr", "d_delta_dd", "r_delta_dd"]
+    with pytest.raises(ValueError, match="provided together"):
+        tp._centre_arrays(np.zeros((3, 3)), None, "particle_pos1", "particle_weight1")
+    with pytest.raises(ValueError, match="positive sum"):
+        tp._centre_arrays(np.zeros((3, 3)), np.zeros(3), "particle_pos1", "particle_weight1")
+    with pytest.raises(ValueError, match="Unsupported product"):
+        Corr_3PCF(_3pcf_params(products=["d_delta_dd"]))  # a particle-centre product
     with pytest.raises(ValueError, match="monotonic"):
         t2 = Corr_3PCF(_3pcf_params(angle_param="mu", mu=[0.1, 0.5, 0.2]))
         t2._resolve_angle_sampling()

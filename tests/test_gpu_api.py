@@ -271,6 +271,54 @@ def test_corr3pcf_box_random_task(phi):
     np.testing.assert_allclose(res.Q, zeta / zh, rtol=1e-8, atol=1e-10)
 
 
+def test_corr3pcf_particle_centre_task(phi):
+    """center='particle' (corr3pcf_kernels.py:66-97; corr3pcf.py:1135-1180, :1430-1450, :1596-1614): ddd and
+    d_delta_dd on the catalogue's own points against the oracle; r_delta_dd from rrr * xi23 with a uniform
+    random vertex 1, and sampled on explicit random centres when they are given; delta_ddd and zeta."""
+    from pyhermes_amd.theory import Corr_3PCF
+    print("Corr_3PCF particle centres J=5", flush=True)
+    box, J = 320.0, 5
+    L = 1 << J
+    pos, w = _catalogue(30000, box, 19)
+    field = _project(pos, w, J, box)
+    cpos, cw = pos[:2500], w[:2500] / w.sum()
+    rpos = np.random.default_rng(4).uniform(0.0, box, (2000, 3))
+    rw = np.full(2000, 0.5)
+    seed, n_rot = 7, 4
+    base = {"sfc_field": field, "random": "uniform", "r12": 50.0, "r13": 70.0, "angle_param": "mu",
+            "mu": {"mu_min": -0.8, "mu_max": 0.8, "n_mu": 3}, "n_rot": n_rot, "center": "particle",
+            "n_box_centers": 1, "base_seed": seed, "particle_pos1": cpos, "particle_weight1": cw, "fout_path": ""}
+    res = Corr_3PCF({"Corr_3PCF": {**base, "products": ["zeta", "ddd"]}}).run(save_result=False)
+    exp_r = Corr_3PCF({"Corr_3PCF": {**base, "products": ["r_delta_dd"], "random_pos1": rpos,
+                                     "random_weight1": rw}}).run(save_result=False)
+    eps = field.epsilon
+    rho = field.field_integral / L ** 3
+    delta = eps - rho
+    sf = L / box
+    s9 = sf ** 9
+    mu = np.linspace(-0.8, 0.8, 3)
+    rho1, rho1r = cw.sum() / L ** 3, rw.sum() / L ** 3
+    ddd, ddel, rdel = [], [], []
+    for it, m in enumerate(mu):
+        abc = omc.rotation_uniforms(seed + 1, n_rot, it)
+        ddd.append(omc.triplet_particle_centers(50 * sf, 70 * sf, m, cpos * sf, cw, rho1, abc, eps, eps, phi))
+        ddel.append(omc.triplet_particle_centers(50 * sf, 70 * sf, m, cpos * sf, cw, rho1, abc, delta, delta, phi))
+        rdel.append(omc.triplet_particle_centers(50 * sf, 70 * sf, m, rpos * sf, rw, rho1r, abc, delta, delta, phi))
+    ddd, ddel, rdel = np.array(ddd) * s9, np.array(ddel) * s9, np.array(rdel) * s9
+    np.testing.assert_allclose(res.ddd, ddd, rtol=1e-10)
+    np.testing.assert_allclose(res.d_delta_dd, ddel, rtol=1e-9, atol=1e-12 * np.abs(ddd).max())
+    np.testing.assert_allclose(exp_r.r_delta_dd, rdel, rtol=1e-9, atol=1e-12 * np.abs(ddd).max())
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    r23 = np.sqrt(50.0 ** 2 + 70.0 ** 2 - 2 * 50.0 * 70.0 * mu)
+    x23 = np.array([ow.pair_product(delta, delta, ow.octant_rfft_kernel(L, P, "shell", r * sf)) / rho ** 2
+                    for r in r23])
+    rrr = rho ** 3 * s9
+    np.testing.assert_allclose(res.rrr, rrr * np.ones(3), rtol=1e-12)
+    np.testing.assert_allclose(res.r_delta_dd, rrr * x23, rtol=1e-9, atol=1e-15)
+    np.testing.assert_allclose(res.delta_ddd, ddel - rrr * x23, rtol=1e-8, atol=1e-12 * np.abs(ddd).max())
+    np.testing.assert_allclose(res.zeta, (ddel - rrr * x23) / rrr, rtol=1e-8, atol=1e-10)
+
+
 def test_sfc_field_and_result_roundtrip(phi, tmp_path):
     from pyhermes_amd.io import Corr3PCFMultipoleData, SFCField
     box, J = 400.0, 4

@@ -489,3 +489,24 @@ def test_los_kernel_mu_average_is_the_shell_kernel(phi):
     avg = sum(w * windows.los_rfft_kernel(L, P, "ring", s * np.sqrt(1 - m * m), s * m) for m, w in zip(mu, wmu))
     shell = windows.octant_rfft_kernel(L, P, "shell", s)
     assert np.abs(avg - shell).max() <= 1e-11 * np.abs(shell).max()
+
+
+def test_particle_centre_estimator_pins(phi):
+    """corr3pcf_kernels.py:66-97 restated: with unit weights and rho1 = 1 it is the box-random estimator of a
+    constant first field (the scaling function sums to one, so n1 = 1); a doubled weight equals a repeated
+    centre; a common weight factor cancels."""
+    L = 16
+    rng = np.random.default_rng(31)
+    e2, e3 = rng.normal(size=(L, L, L)), rng.normal(size=(L, L, L))
+    cen = rng.uniform(0, L, (40, 3))
+    abc = rng.uniform(size=(5, 3))
+    R1, R2, mu = 2.5, 3.75, 0.3
+    box = mc.triplet_box_random(R1, R2, mu, cen, abc, np.ones((L, L, L)), e2, e3, phi)
+    part = mc.triplet_particle_centers(R1, R2, mu, cen, np.ones(40), 1.0, abc, e2, e3, phi)
+    assert part == pytest.approx(box, rel=1e-12, abs=1e-15)
+    w = rng.uniform(0.5, 2.0, 40)
+    a = mc.triplet_particle_centers(R1, R2, mu, cen, np.r_[w[:-1], 2 * w[-1]], 0.7, abc, e2, e3, phi)
+    b = mc.triplet_particle_centers(R1, R2, mu, np.r_[cen, cen[-1:]], np.r_[w, w[-1]], 0.7, abc, e2, e3, phi)
+    assert a == pytest.approx(b, rel=1e-12)
+    c = mc.triplet_particle_centers(R1, R2, mu, cen, 3.0 * w, 0.7, abc, e2, e3, phi)
+    assert c == pytest.approx(mc.triplet_particle_centers(R1, R2, mu, cen, w, 0.7, abc, e2, e3, phi), rel=1e-12)
