@@ -82,3 +82,16 @@ class Corr3PCFMultipoleData(_Result):
 
     def load_corr3pcf_multipole(self, f_in, single=True, trusted=False):
         return self.load(f_in, trusted=trusted)
+
+
+class CountingData(_Result):
+    """io/counting.py:9-60 fields: the field density at random positions (nx) with the field's sfc_info."""
+    _FIELDS = ("sfc_info", "nx")
+    _REQUIRED = ("nx",)
+    _INFO = "counting_info"
+
+    def save_counting(self, f_out, single=True, overwrite=False):
+        return self.save(f_out, overwrite=overwrite)
+
+    def load_counting(self, f_in, single=True, trusted=False):
+        return self.load(f_in, trusted=trusted)

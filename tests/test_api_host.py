@@ -277,3 +277,13 @@ def test_corr2pcf_anisotropic_sampling_and_descriptors():
     with pytest.raises(ValueError, match="multipoles"):
         Corr_2PCF({"Corr_2PCF": {"sfc_field": "unused", "random": "uniform", "binning_window": "ring",
                                  "multipoles": [0, 2], "sampling": {"s": [10.0], "mu": [0.5]}}})._resolve_sampling()
+
+
+def test_counting_defaults_and_validation():
+    """Counting parameters (theory/default_params.json 'Counting', counting.py:27-37)."""
+    from pyhermes_amd.theory import Counting
+    t = Counting({"Counting": {"sfc_field": "unused"}})
+    assert t.random_count == 1000000 and t.seed == 42 and t.window is None
+    assert t.weight_normalization == "catalog"
+    with pytest.raises(ValueError, match="non-negative"):
+        Counting({"Counting": {"sfc_field": "unused", "random_count": -1}})

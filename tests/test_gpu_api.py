@@ -319,6 +319,34 @@ def test_corr3pcf_particle_centre_task(phi):
     np.testing.assert_allclose(res.zeta, (ddel - rrr * x23) / rrr, rtol=1e-8, atol=1e-10)
 
 
+def test_counting_task(phi, tmp_path):
+    """Counting (counting.py:185-258): the field density at default_rng(seed) box points, physical units,
+    against the oracle's interpolation, with and without a window; the result file round-trips."""
+    from pyhermes_amd.io import CountingData
+    from pyhermes_amd.theory import Counting
+    print("Counting J=5", flush=True)
+    box, J = 300.0, 5
+    L = 1 << J
+    pos, w = _catalogue(20000, box, 23)
+    field = _project(pos, w, J, box)
+    sf = L / box
+    pts = np.random.default_rng(5).uniform(0.0, box, (4000, 3))
+    res = Counting({"Counting": {"sfc_field": field, "random_count": 4000, "seed": 5,
+                                 "fout_path": str(tmp_path / "nx.pkl")}}).run(save_result=True)
+    eps = field.epsilon
+    ref = omc.interpolate(eps, pts * sf, phi) * sf ** 3
+    np.testing.assert_allclose(res.nx, ref, rtol=1e-12, atol=1e-12 * np.abs(ref).max())
+    back = CountingData().load_counting(str(tmp_path / "nx.pkl"))
+    np.testing.assert_array_equal(back.nx, res.nx)
+    win = {"type": "gaussian", "len_args": {"R": 20.0}}
+    rw = Counting({"Counting": {"sfc_field": field, "random_count": 4000, "seed": 5, "window": win,
+                                "fout_path": ""}}).run(save_result=False)
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    conv = ow.convolve_rfft(eps, ow.octant_rfft_kernel(L, P, "gaussian", 20.0 * sf))
+    refw = omc.interpolate(conv, pts * sf, phi) * sf ** 3
+    np.testing.assert_allclose(rw.nx, refw, rtol=1e-10, atol=1e-12 * np.abs(refw).max())
+
+
 def test_sfc_field_and_result_roundtrip(phi, tmp_path):
     from pyhermes_amd.io import Corr3PCFMultipoleData, SFCField
     box, J = 400.0, 4
