@@ -273,6 +273,33 @@ class SFCField(HermesData):
             raise ValueError(f"Cannot use weight_normalization='{mode}' without a finite, non-zero {label}.")
         return float(value)
 
+    def get_particle_data(self):
+        """sfc_field.py:425-453: the catalogue behind a catalogue field, from the particle file SFCProjection
+        saved (save_particle_data=True): pos, catalog_weight, normalized_catalog_weight, field_value and
+        projection_weight (in the field's weight normalization).  Re-reading the original input catalogue
+        (sfc_field.py:455-480) needs the reference's file readers and is not provided."""
+        if getattr(self, "field_kind", None) != "catalog_field":
+            raise ValueError("Only catalog fields can provide particle data. Derived fields do not retain a unique "
+                             "particle catalogue.")
+        if not bool(getattr(self, "particle_data_retrievable", True)):
+            raise ValueError("This field no longer maps to one retrievable particle catalogue. Provide "
+                             "particle_pos1 and particle_weight1 explicitly for particle-centred statistics.")
+        den = self._normalization_denominator(getattr(self, "weight_normalization", "raw"))
+        path = getattr(self, "particle_data_path", "")
+        if not path:
+            raise ValueError("This field has no saved particle file (project with save_particle_data=True); "
+                             "reading the input catalogue again is not provided by pyhermes_amd.")
+        with np.load(path, allow_pickle=False) as pd:
+            required = {"pos", "catalog_weight", "field_value"}
+            if not required.issubset(pd.files):
+                self.logger.error(f"Particle dataset '{path}' must contain {sorted(required)} arrays.")
+                func_util.safe_exit(1)
+            cw = np.asarray(pd["catalog_weight"], dtype=np.float64)
+            fv = np.asarray(pd["field_value"], dtype=np.float64)
+            return {"pos": np.asarray(pd["pos"]), "catalog_weight": cw,
+                    "normalized_catalog_weight": cw / float(self.catalog_weight_sum), "field_value": fv,
+                    "projection_weight": cw * fv / den}
+
     def _field_integral_for_normalization(self, weight_normalization):
         raw = getattr(self, "raw_field_weighted_sum", None)
         if raw is None or not np.isfinite(raw):

@@ -24,8 +24,8 @@ Products add d_delta_dd (data centres on delta2, delta3) and r_delta_dd (random
 centres; rrr * xi23 when random1 is a uniform density and no random_pos1 is
 given; the data centres when random1 is a field), and delta_ddd = d_delta_dd -
 r_delta_dd.  Centres come as particle_pos1 / particle_weight1 (and optionally
-random_pos1 / random_weight1) in box units; reading them from a field's saved
-particle data is not provided.
+random_pos1 / random_weight1) in box units, or, for the data centres, from the
+particle file SFCProjection saved for sfc_field1 (SFCField.get_particle_data).
 """
 
 import copy
@@ -251,9 +251,16 @@ class Corr_3PCF(TaskBase):
                                                      "particle_weight1")
             self._random_centres = self._centre_arrays(self.random_pos1, self.random_weight1, "random_pos1",
                                                        "random_weight1")
-            if self._data_centres is None and ({"ddd", "d_delta_dd", "r_delta_dd"} & set(self._expanded_products())):
-                raise ValueError("center='particle' needs particle_pos1 and particle_weight1 (box units); reading "
-                                 "centres from a field's saved particle data is not provided by pyhermes_amd.")
+            needs_centres = {"ddd", "d_delta_dd", "r_delta_dd"} & set(self._expanded_products())
+            if self._data_centres is None and needs_centres:
+                # corr3pcf.py:600-607: the centres of vertex 1's catalogue field, with its projection weights
+                f1 = sfc_field1 if sfc_field1 is not None else self.sfc_field1
+                if not isinstance(f1, SFCField):
+                    raise ValueError("center='particle' needs particle_pos1 and particle_weight1, or sfc_field1 as a "
+                                     "catalogue field with saved particle data.")
+                pdata = self._normalized(f1).get_particle_data()
+                self._data_centres = self._centre_arrays(pdata["pos"], pdata["projection_weight"],
+                                                         "sfc_field1 particle positions", "sfc_field1 particle weight")
         self._resolve_angle_sampling()
         exp = self._expanded_products()
         needs_data = any(PRODUCT_INPUT_FLAGS[p][0] for p in exp)

@@ -319,6 +319,33 @@ def test_corr3pcf_particle_centre_task(phi):
     np.testing.assert_allclose(res.zeta, (ddel - rrr * x23) / rrr, rtol=1e-8, atol=1e-10)
 
 
+def test_particle_centres_from_saved_particle_data(phi, tmp_path):
+    """SFCField.get_particle_data (sfc_field.py:425-453) from the file SFCProjection saves, and Corr_3PCF
+    center='particle' taking its data centres from it (corr3pcf.py:600-607) equals passing them explicitly."""
+    from pyhermes_amd.base import SFCProjection
+    from pyhermes_amd.theory import Corr_3PCF
+    print("particle data file J=5", flush=True)
+    box, J = 320.0, 5
+    pos, w = _catalogue(8000, box, 27)
+    field = SFCProjection({"SFCProjection": {"J": J, "box_size": box, "particle_pos": pos, "catalog_weight": w,
+                                             "weight_normalization": "catalog", "save_particle_data": True,
+                                             "particle_data_path": str(tmp_path / "cat.npz"),
+                                             "fout_path": ""}}).run(save_result=False)
+    pd = field.get_particle_data()
+    np.testing.assert_allclose(pd["pos"], pos.astype(np.float32))
+    np.testing.assert_allclose(pd["projection_weight"], w / w.sum(), rtol=1e-15)
+    base = {"sfc_field": field, "random": "uniform", "r12": 40.0, "r13": 60.0, "angle_param": "mu",
+            "mu": {"mu_min": -0.5, "mu_max": 0.5, "n_mu": 2}, "n_rot": 3, "center": "particle",
+            "n_box_centers": 1, "base_seed": 3, "products": ["ddd", "d_delta_dd"], "fout_path": ""}
+    a = Corr_3PCF({"Corr_3PCF": base}).run(save_result=False)
+    b = Corr_3PCF({"Corr_3PCF": {**base, "particle_pos1": pd["pos"].astype(np.float64),
+                                 "particle_weight1": pd["projection_weight"]}}).run(save_result=False)
+    np.testing.assert_array_equal(a.ddd, b.ddd)
+    np.testing.assert_array_equal(a.d_delta_dd, b.d_delta_dd)
+    with pytest.raises(ValueError, match="Derived fields"):
+        (field - 1.0).get_particle_data()
+
+
 def test_counting_task(phi, tmp_path):
     """Counting (counting.py:185-258): the field density at default_rng(seed) box points, physical units,
     against the oracle's interpolation, with and without a window; the result file round-trips."""
