@@ -114,7 +114,8 @@ int hermes_fft3_set_tile(hermes_ctx *ctx, int J, int inverse, int tw, int thread
  * Length arguments a0, a1 are in grid units (len_args * L / box_size).  The line-of-sight
  * windows (ring .. cylinder) take k_par = k[axis], k_perp = |k - k_par e_axis|,
  * q_perp = 2 pi k_perp R, q_par = 2 pi k_par H; their line of sight is the coordinate
- * axis `axis` (0 x, 1 y, 2 z: the reference's los_args (nx, ny, nz) along an axis). */
+ * axis `axis` (0 x, 1 y, 2 z: the reference's los_args (nx, ny, nz) along an axis), or, with
+ * axis = 3, the unit vector los (k_par = k . los, k_perp = sqrt(|k|^2 - k_par^2)). */
 enum {
     HERMES_WIN_SPHERE = 0,            /* a0 = R */
     HERMES_WIN_GAUSSIAN = 1,          /* a0 = R */
@@ -131,8 +132,9 @@ enum {
 
 typedef struct {
     int32_t type, l, m;
-    int32_t axis; /* line-of-sight axis of the ring .. cylinder windows; 0 for the others */
+    int32_t axis;  /* line of sight of the ring .. cylinder windows: 0..2 an axis, 3 the vector los; 0 otherwise */
     double a0, a1;
+    double los[3]; /* unit line-of-sight vector when axis = 3 */
 } hermes_window_desc;
 
 /* Real rFFT-layout kernel (L, L, L/2+1) of one isotropic window, summed over the

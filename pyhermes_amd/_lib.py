@@ -50,7 +50,7 @@ _SIGNATURES = {
 class WindowDesc(ctypes.Structure):
     """hermes_window_desc (include/hermes_amd.h)."""
     _fields_ = [("type", ctypes.c_int32), ("l", ctypes.c_int32), ("m", ctypes.c_int32), ("axis", ctypes.c_int32),
-                ("a0", ctypes.c_double), ("a1", ctypes.c_double)]
+                ("a0", ctypes.c_double), ("a1", ctypes.c_double), ("los", ctypes.c_double * 3)]
 
 
 _dpp = ctypes.POINTER(ctypes.c_double)

@@ -192,12 +192,22 @@ __device__ inline double win_sphere_f64(double k, double R) {
     return 3 * (s - q * c) / (q * q * q);
 }
 
-// Line-of-sight windows (window_functions.py:164-319) along coordinate axis `axis`: even in every
-// component of k, so the octant folds and the half-spectrum sums apply to them as to the isotropic ones.
-__device__ inline double los_window_f64(int type, double ki, double kj, double kk, double R, double H, int axis) {
-    const double kpar = axis == 0 ? ki : (axis == 1 ? kj : kk);
-    const double u = axis == 0 ? kj : ki, v = axis == 2 ? kj : kk;  // the two perpendicular components
-    const double qperp = 2.0 * HERMES_PI * sqrt(u * u + v * v) * R;
+// Line-of-sight windows (window_functions.py:164-319).  Along a coordinate axis they are even in every
+// component of k, so the octant folds apply to them as to the isotropic ones; along any other direction
+// they are only even under k -> -k (the half-spectrum sums still apply, the octant sharing does not).
+// axis 3: the unit vector n (k_par = k . n, k_perp from |k|^2 - k_par^2, window_functions.py:164-319).
+__device__ inline double los_window_f64(int type, double ki, double kj, double kk, double R, double H, int axis,
+                                        const double* n = nullptr) {
+    double kpar, kperp2;
+    if (axis == 3) {
+        kpar = ki * n[0] + kj * n[1] + kk * n[2];
+        kperp2 = fmax(ki * ki + kj * kj + kk * kk - kpar * kpar, 0.0);
+    } else {
+        kpar = axis == 0 ? ki : (axis == 1 ? kj : kk);
+        const double u = axis == 0 ? kj : ki, v = axis == 2 ? kj : kk;  // the two perpendicular components
+        kperp2 = u * u + v * v;
+    }
+    const double qperp = 2.0 * HERMES_PI * sqrt(kperp2) * R;
     const double qpar = 2.0 * HERMES_PI * kpar * H;
     double j0, j1;
     bessel_j01_f64(qperp, j0, j1);
@@ -213,8 +223,9 @@ __device__ inline double los_window_f64(int type, double ki, double kj, double k
 // Transfer function of one real, even window at k (cycles per grid cell): the isotropic ones and the
 // line-of-sight ones.
 __device__ inline double iso_window_f64(int type, double ki, double kj, double kk, double a0, double a1,
-                                        int axis = 2) {
-    if (type >= HERMES_WIN_RING && type <= HERMES_WIN_CYLINDER) return los_window_f64(type, ki, kj, kk, a0, a1, axis);
+                                        int axis = 2, const double* los = nullptr) {
+    if (type >= HERMES_WIN_RING && type <= HERMES_WIN_CYLINDER)
+        return los_window_f64(type, ki, kj, kk, a0, a1, axis, los);
     switch (type) {
     case HERMES_WIN_SPHERE: return win_sphere_f64(sqrt(ki * ki + kj * kj + kk * kk), a0);
     case HERMES_WIN_GAUSSIAN: {
@@ -264,7 +275,7 @@ __device__ inline double2 window_value_f64(const hermes_window_desc& d, double k
         double2 y = ylm_f64(d.l, d.m, ki, kj, kk);
         return make_double2(radial * y.x, radial * y.y);
     }
-    return make_double2(iso_window_f64(d.type, ki, kj, kk, d.a0, d.a1, d.axis), 0.0);
+    return make_double2(iso_window_f64(d.type, ki, kj, kk, d.a0, d.a1, d.axis, d.los), 0.0);
 }
 
 // ------------------------------------------------------------------ FP32 fast path

@@ -36,7 +36,10 @@ __global__ void k_build_rfft(int L, const double* __restrict__ P, hermes_window_
         for (int a = 0; a < 8; ++a) {
             int ii = xs[ord[a][0]], jj = ys[ord[a][1]], kk = zs[ord[a][2]];
             double wa = 0.0;
-            wa += P[ii] * P[jj] * P[kk] * iso_window_f64(d.type, ii * inv_L, jj * inv_L, kk * inv_L, d.a0, d.a1, d.axis);
+            // the mirror alias L - x sits at k = -(L - x) / L (the sign matters for a non-axis line of sight)
+            const double sx = ord[a][0] ? -1.0 : 1.0, sy = ord[a][1] ? -1.0 : 1.0, sz = ord[a][2] ? -1.0 : 1.0;
+            wa += P[ii] * P[jj] * P[kk] *
+                  iso_window_f64(d.type, sx * ii * inv_L, sy * jj * inv_L, sz * kk * inv_L, d.a0, d.a1, d.axis, d.los);
             t = (a == 0) ? wa : t + wa;
         }
         if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && i == 0) t = 0.0;
@@ -109,7 +112,7 @@ __device__ __forceinline__ float window_value_f32(const hermes_window_desc& d, i
         return j * y.x;  // m = 0 windows are real
     }
     double inv_L = 1.0 / L;
-    return (float)iso_window_f64(d.type, ax * inv_L, ay * inv_L, az * inv_L, d.a0, d.a1, d.axis);
+    return (float)iso_window_f64(d.type, ax * inv_L, ay * inv_L, az * inv_L, d.a0, d.a1, d.axis, d.los);
 }
 
 // One thread per octant representative (x, y, z) in [0, L/2]^3; its kernel value is
@@ -132,6 +135,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __r
         int64_t r = base + threadIdx.x;
         bool active = r < nrep;
         double power = 0.0;
+        double pw[2][2] = {{0.0, 0.0}, {0.0, 0.0}};  // per mirror mode (x | L-x, y | L-y)
         int x = 0, y = 0, z = 0;
         if (active) {
             z = (int)(r % Lh);
@@ -145,27 +149,37 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __r
                     int xx = a ? xm : x, yy = b ? ym : y;
                     int64_t idx = ((int64_t)xx * L + yy) * Lh + z;
                     C f1 = F1[idx], f2 = F2[idx];
-                    power += wz * ((double)f1.x * (double)f2.x + (double)f1.y * (double)f2.y);
+                    pw[a][b] = wz * ((double)f1.x * (double)f2.x + (double)f1.y * (double)f2.y);
+                    power += pw[a][b];
                 }
         }
         for (int b = 0; b < nd; ++b) {
             double kb = 0.0;
             if (active) {
                 const hermes_window_desc d = descs[b];
-                for (int a = 0; a < 8; ++a) {
-                    int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
-                    int ix = ax < 0 ? -ax : ax, iy = ay < 0 ? -ay : ay, iz = az < 0 ? -az : az;
-                    double p = P[ix] * P[iy] * P[iz];
-                    if (FP64) {
-                        double inv_L = 1.0 / L;
-                        kb += p * window_value_f64(d, ax * inv_L, ay * inv_L, az * inv_L).x;
-                    } else {
-                        kb += (double)((float)p * window_value_f32(d, ax, ay, az, L, cnorms[b]));
+                // a window along a non-axis line of sight differs between the mirror modes: the aliases of
+                // (L - x, y, z) are those of (x, y, z) with k_x negated, so each mode takes its own sum
+                const bool general = d.type >= HERMES_WIN_RING && d.axis == 3;
+                for (int m = 0; m < (general ? 4 : 1); ++m) {
+                    const double sx = (m & 2) ? -1.0 : 1.0, sy = (m & 1) ? -1.0 : 1.0;
+                    double km = 0.0;
+                    for (int a = 0; a < 8; ++a) {
+                        int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+                        int ix = ax < 0 ? -ax : ax, iy = ay < 0 ? -ay : ay, iz = az < 0 ? -az : az;
+                        double p = P[ix] * P[iy] * P[iz];
+                        if (FP64 || general) {
+                            double inv_L = 1.0 / L;
+                            const double w = window_value_f64(d, sx * ax * inv_L, sy * ay * inv_L, az * inv_L).x;
+                            km += FP64 ? p * w : (double)((float)p * (float)w);
+                        } else {
+                            km += (double)((float)p * window_value_f32(d, ax, ay, az, L, cnorms[b]));
+                        }
                     }
+                    kb += general ? pw[m >> 1][m & 1] * km : power * km;
                 }
                 if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && x == 0 && y == 0 && z == 0) kb = 0.0;
             }
-            double v = wave_sum(power * kb);
+            double v = wave_sum(kb);
             if (lane == 0) wacc[wave * nd + b] += v;
         }
     }
@@ -755,9 +769,16 @@ int check_desc(const hermes_window_desc& d) {
         hermes_set_error("unknown window type %d", d.type);
         return 2;
     }
-    if (d.type >= HERMES_WIN_RING && (d.axis < 0 || d.axis > 2)) {
-        hermes_set_error("line-of-sight window: axis %d is not 0, 1 or 2", d.axis);
+    if (d.type >= HERMES_WIN_RING && (d.axis < 0 || d.axis > 3)) {
+        hermes_set_error("line-of-sight window: axis %d is not 0, 1, 2 or 3", d.axis);
         return 2;
+    }
+    if (d.type >= HERMES_WIN_RING && d.axis == 3) {
+        const double n2 = d.los[0] * d.los[0] + d.los[1] * d.los[1] + d.los[2] * d.los[2];
+        if (!(std::fabs(n2 - 1.0) < 1e-12)) {
+            hermes_set_error("line-of-sight window: los is not a unit vector (|los|^2 = %g)", n2);
+            return 2;
+        }
     }
     if (d.type == HERMES_WIN_SHELL_MULTIPOLE && (d.l < 0 || d.l > 40 || d.m > d.l || d.m < -d.l)) {
         hermes_set_error("invalid multipole (l=%d, m=%d)", d.l, d.m);

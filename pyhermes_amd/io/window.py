@@ -42,13 +42,17 @@ def _iso_desc(wtype, rescaled):
 
 
 def los_axis(los_args):
-    """The coordinate axis (0 x, 1 y, 2 z) of a line of sight {nx, ny, nz} along one axis (either sign)."""
+    """(axis, unit vector) of a line of sight {nx, ny, nz}: axis 0 x, 1 y, 2 z when it lies along one
+    coordinate axis (either sign; the windows are even in k), else 3 with the normalised direction, as
+    the reference divides k . n by |n| (convolution.py:73-157)."""
     vec = np.array([los_args["nx"], los_args["ny"], los_args["nz"]], dtype=np.float64)
+    norm = float(np.linalg.norm(vec))
+    if not np.isfinite(norm) or norm == 0.0:
+        raise ValueError(f"line-of-sight vector must be finite and non-zero, got {tuple(vec)}")
     nz = np.nonzero(vec)[0]
-    if nz.size != 1:
-        raise NotImplementedError("pyhermes_amd builds line-of-sight windows with the line of sight along a "
-                                  f"coordinate axis; got (nx, ny, nz) = {tuple(vec)}")
-    return int(nz[0])
+    if nz.size == 1:
+        return int(nz[0]), None
+    return 3, vec / norm
 
 
 def los_desc(wtype, rescaled, los_args):
@@ -59,7 +63,8 @@ def los_desc(wtype, rescaled, los_args):
         raise ValueError(f"{wtype} requires finite len_args {{'R': ..., 'H': ...}}.") from exc
     if not (np.isfinite(R) and np.isfinite(H)) or R < 0.0:
         raise ValueError(f"{wtype} requires a finite non-negative R and a finite H.")
-    return kernels.window_desc(wtype, R, H, axis=los_axis(los_args))
+    axis, n = los_axis(los_args)
+    return kernels.window_desc(wtype, R, H, axis=axis, los=n)
 
 
 class WindowFunc(SFCField):
@@ -222,9 +227,13 @@ class WindowFunc(SFCField):
                 t = kernels.window_build_full(self.J, P, self._multipole_desc(), dbl=dbl)
             elif self.type in LOS_TYPES:
                 # even in every component of k for an axis line of sight: the rFFT kernel of the reference's
-                # "auto" (full_rfft) mode and the octant fold sum the same alias values
-                t = kernels.window_build_rfft(self.J, P, los_desc(self.type, self.rescale_len_args, self.los_args),
-                                              dbl=dbl)
+                # "auto" (full_rfft) mode and the octant fold sum the same alias values.  Along any other
+                # direction only the signed-alias (full_rfft) sum is built.
+                desc = los_desc(self.type, self.rescale_len_args, self.los_args)
+                if desc.axis == 3 and self.kernel_mode == "octant":
+                    raise NotImplementedError("pyhermes_amd builds line-of-sight windows along a non-axis "
+                                              "direction with kernel_mode 'auto' / 'full_rfft' only")
+                t = kernels.window_build_rfft(self.J, P, desc, dbl=dbl)
             else:
                 t = kernels.window_build_rfft(self.J, P, _iso_desc(self.type, self.rescale_len_args), dbl=dbl)
         self._dev_kernel[key] = t

@@ -137,13 +137,22 @@ WIN_CODES = {"sphere": 0, "gaussian": 1, "shell": 2, "thick_shell": 3, "gaussian
 LOS_TYPES = ("ring", "disk", "cylshell", "cylinder")
 
 
-def window_desc(wtype, a0=0.0, a1=0.0, l=0, m=0, axis=2):
-    """A hermes_window_desc; lengths in grid units.  axis: line of sight (0 x, 1 y, 2 z) of the ring,
-    disk, cylshell and cylinder windows (a0 = R, a1 = H); ignored by the others."""
-    if wtype in LOS_TYPES and int(axis) not in (0, 1, 2):
-        raise ValueError(f"line-of-sight axis must be 0, 1 or 2, got {axis!r}")
-    return _lib.WindowDesc(WIN_CODES[wtype], int(l), int(m), int(axis) if wtype in LOS_TYPES else 0, float(a0),
-                           float(a1))
+def window_desc(wtype, a0=0.0, a1=0.0, l=0, m=0, axis=2, los=None):
+    """A hermes_window_desc; lengths in grid units.  axis: line of sight (0 x, 1 y, 2 z, or 3 along the
+    unit vector ``los``) of the ring, disk, cylshell and cylinder windows (a0 = R, a1 = H); ignored by the
+    others."""
+    if wtype in LOS_TYPES and int(axis) not in (0, 1, 2, 3):
+        raise ValueError(f"line-of-sight axis must be 0, 1, 2 or 3, got {axis!r}")
+    general = wtype in LOS_TYPES and int(axis) == 3
+    if general:
+        n = np.asarray(los, dtype=np.float64).reshape(3)
+        if not np.all(np.isfinite(n)) or abs(float(n @ n) - 1.0) > 1e-12:
+            raise ValueError(f"axis 3 needs a unit line-of-sight vector, got {los!r}")
+    d = _lib.WindowDesc(WIN_CODES[wtype], int(l), int(m), int(axis) if wtype in LOS_TYPES else 0, float(a0),
+                        float(a1))
+    if general:
+        d.los[0], d.los[1], d.los[2] = (float(v) for v in n)
+    return d
 
 
 def _np_f64(a):

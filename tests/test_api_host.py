@@ -69,8 +69,16 @@ def test_thick_shell_validation_and_custom_override():
         WindowFunc({"type": "thick_shell", "len_args": {"R_in": 8.0, "R_out": 12.0}}, sfc)
     cyl = WindowFunc({"type": "cylinder", "len_args": {"R": 1.0, "H": 2.0}}, sfc)
     assert cyl.los_args == {"nx": 0.0, "ny": 0.0, "nz": 1.0}  # the default line of sight
-    with pytest.raises(NotImplementedError, match="coordinate axis"):
-        WindowFunc({"type": "ring", "len_args": {"R": 1.0, "H": 2.0}, "los_args": [1.0, 1.0, 0.0]}, sfc)
+    diag = WindowFunc({"type": "ring", "len_args": {"R": 1.0, "H": 2.0}, "los_args": [1.0, 1.0, 0.0]}, sfc)
+    assert diag.los_args == {"nx": 1.0, "ny": 1.0, "nz": 0.0}
+    from pyhermes_amd.io.window import los_axis, los_desc
+    axis, n = los_axis({"nx": 2.0, "ny": -2.0, "nz": 1.0})
+    assert axis == 3 and np.allclose(n, [2 / 3, -2 / 3, 1 / 3], rtol=0, atol=1e-15)
+    assert los_axis({"nx": 0.0, "ny": -3.0, "nz": 0.0}) == (1, None)
+    d = los_desc("disk", {"R": 1.0, "H": 2.0}, {"nx": 1.0, "ny": 1.0, "nz": 0.0})
+    assert d.axis == 3 and abs(d.los[0] - 0.5 ** 0.5) < 1e-15 and d.los[2] == 0.0
+    with pytest.raises(ValueError, match="non-zero"):
+        los_axis({"nx": 0.0, "ny": 0.0, "nz": 0.0})
     with pytest.raises(ValueError, match="requires finite len_args"):
         WindowFunc({"type": "disk", "len_args": {"R": 1.0}}, sfc)
     with pytest.raises(NotImplementedError):

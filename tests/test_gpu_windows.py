@@ -142,8 +142,20 @@ def test_pair_reduce_uniform_shells_fp64(phi, J):
         assert abs(got[i] / L ** 6 - ref) <= 1e-11 * scale / L ** 6, (i, radii[i])
 
 
+# line of sight: a coordinate axis (0, 1, 2) or a direction vector (axis 3 descriptors)
 LOS_CASES = [("ring", 1.5, 2.5, 2), ("disk", 3.0, 1.25, 2), ("cylshell", 2.25, 3.5, 0), ("cylinder", 6.0, 0.75, 1),
-             ("ring", 40.0, 9.0, 2)]  # the last reaches J0/J1 arguments of several hundred (asymptotic range)
+             ("ring", 40.0, 9.0, 2),  # reaches J0/J1 arguments of several hundred (asymptotic range)
+             ("ring", 2.5, 3.0, (1.0, 1.0, 0.0)), ("disk", 1.75, 2.0, (1.0, -2.0, 2.0)),
+             ("cylshell", 3.0, 1.5, (0.3, 0.0, -1.0)), ("cylinder", 2.0, 4.0, (-1.0, 2.0, 0.5))]
+
+
+def _los_case(wt, R, H, spec):
+    """(descriptor, line-of-sight vector) of one LOS case."""
+    from pyhermes_amd import kernels
+    if isinstance(spec, tuple):
+        n = np.asarray(spec) / np.linalg.norm(spec)
+        return kernels.window_desc(wt, R, H, axis=3, los=n), tuple(spec)
+    return kernels.window_desc(wt, R, H, axis=spec), tuple(1.0 if i == spec else 0.0 for i in range(3))
 
 
 @pytest.mark.parametrize("wt,R,H,axis", LOS_CASES)
@@ -154,9 +166,8 @@ def test_build_rfft_los_windows_match_oracle(phi, wt, R, H, axis):
     J = 5
     L = 1 << J
     P, Pd = _P(phi, L)
-    los = tuple(1.0 if i == axis else 0.0 for i in range(3))
+    desc, los = _los_case(wt, R, H, axis)
     ref = ow.los_rfft_kernel(L, P, wt, R, H, los)
-    desc = kernels.window_desc(wt, R, H, axis=axis)
     got = kernels.window_build_rfft(J, Pd, desc, dbl=True).cpu().numpy()
     assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), np.abs(got - ref).max()
     got32 = kernels.window_build_rfft(J, Pd, desc, dbl=False).cpu().numpy()
@@ -179,12 +190,12 @@ def test_pair_reduce_los_windows_match_oracle(phi, dbl):
     F1 = kernels.rfftn(kernels.as_device(f1, td))
     F2 = kernels.rfftn(kernels.as_device(f2, td))
     print(f"LOS pair reduce J={J} dbl={dbl}", flush=True)
-    got = kernels.pair_reduce(F1, F2, Pd, [kernels.window_desc(wt, R, H, axis=a) for wt, R, H, a in LOS_CASES])
+    cases = [_los_case(*c) for c in LOS_CASES]
+    got = kernels.pair_reduce(F1, F2, Pd, [d for d, _ in cases])
     got /= L ** 6
     scale = np.sqrt(np.mean(f1 ** 2) * np.mean(f2 ** 2))
     tol = 1e-11 if dbl else 2e-5
-    for (wt, R, H, a), g in zip(LOS_CASES, got):
-        los = tuple(1.0 if i == a else 0.0 for i in range(3))
+    for (wt, R, H, a), (_, los), g in zip(LOS_CASES, cases, got):
         ref = ow.pair_product(f1, f2, ow.los_rfft_kernel(L, P, wt, R, H, los))
         assert abs(g - ref) <= tol * scale, (wt, R, H, a, g, ref)
 
@@ -195,8 +206,11 @@ def test_rejects_bad_los_axis():
     L = 16
     Pd = kernels.as_device(np.ones(L + 1), torch.float64)
     bad = kernels.window_desc("ring", 1.0, 1.0)
-    bad.axis = 3
+    bad.axis = 4
     with pytest.raises(RuntimeError, match="axis"):
+        kernels.window_build_rfft(4, Pd, bad)
+    bad.axis = 3  # with a zero los vector
+    with pytest.raises(RuntimeError, match="unit vector"):
         kernels.window_build_rfft(4, Pd, bad)
 
 
