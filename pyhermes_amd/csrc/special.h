@@ -199,7 +199,7 @@ __device__ inline double win_sphere_f64(double k, double R) {
 __device__ inline double los_window_f64(int type, double ki, double kj, double kk, double R, double H, int axis,
                                         const double* n = nullptr) {
     double kpar, kperp2;
-    if (axis == 3) {
+    if (n != nullptr && axis == 3) {
         kpar = ki * n[0] + kj * n[1] + kk * n[2];
         kperp2 = fmax(ki * ki + kj * kj + kk * kk - kpar * kpar, 0.0);
     } else {
@@ -267,15 +267,17 @@ __device__ inline double iso_window_f64(int type, double ki, double kj, double k
     return __builtin_nan("");
 }
 
-// Complex transfer function of a window at signed k (isotropic windows are real).
-__device__ inline double2 window_value_f64(const hermes_window_desc& d, double ki, double kj, double kk) {
+// Complex transfer function of a window at signed k (isotropic windows are real).  general = false
+// compiles out the non-axis line-of-sight branch (for callers that never see axis-3 descriptors).
+__device__ inline double2 window_value_f64(const hermes_window_desc& d, double ki, double kj, double kk,
+                                           bool general = true) {
     if (d.type == HERMES_WIN_SHELL_MULTIPOLE) {
         double k = sqrt(ki * ki + kj * kj + kk * kk);
         double radial = jl_f64(d.l, 2.0 * HERMES_PI * k * d.a0);
         double2 y = ylm_f64(d.l, d.m, ki, kj, kk);
         return make_double2(radial * y.x, radial * y.y);
     }
-    return make_double2(iso_window_f64(d.type, ki, kj, kk, d.a0, d.a1, d.axis, d.los), 0.0);
+    return make_double2(iso_window_f64(d.type, ki, kj, kk, d.a0, d.a1, d.axis, general ? d.los : nullptr), 0.0);
 }
 
 // ------------------------------------------------------------------ FP32 fast path

@@ -112,12 +112,15 @@ __device__ __forceinline__ float window_value_f32(const hermes_window_desc& d, i
         return j * y.x;  // m = 0 windows are real
     }
     double inv_L = 1.0 / L;
-    return (float)iso_window_f64(d.type, ax * inv_L, ay * inv_L, az * inv_L, d.a0, d.a1, d.axis, d.los);
+    // reached only by passes without non-axis line-of-sight windows
+    return (float)iso_window_f64(d.type, ax * inv_L, ay * inv_L, az * inv_L, d.a0, d.a1, d.axis, nullptr);
 }
 
 // One thread per octant representative (x, y, z) in [0, L/2]^3; its kernel value is
-// shared by the (up to) 4 modes (x|L-x, y|L-y, z) of the half spectrum.
-template <typename C, bool FP64>
+// shared by the (up to) 4 modes (x|L-x, y|L-y, z) of the half spectrum.  GEN: the pass holds a
+// line-of-sight window along a non-axis direction (descriptor axis 3), whose value differs between the 4
+// modes; it is a separate instantiation so the other passes keep their register budget.
+template <typename C, bool FP64, bool GEN>
 __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __restrict__ F1,
                                                              const C* __restrict__ F2, const double* __restrict__ P,
                                                              const hermes_window_desc* __restrict__ descs,
@@ -135,7 +138,7 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __r
         int64_t r = base + threadIdx.x;
         bool active = r < nrep;
         double power = 0.0;
-        double pw[2][2] = {{0.0, 0.0}, {0.0, 0.0}};  // per mirror mode (x | L-x, y | L-y)
+        double pw00 = 0.0, pw01 = 0.0, pw10 = 0.0, pw11 = 0.0;  // per mirror mode (x | L-x, y | L-y)
         int x = 0, y = 0, z = 0;
         if (active) {
             z = (int)(r % Lh);
@@ -149,8 +152,12 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __r
                     int xx = a ? xm : x, yy = b ? ym : y;
                     int64_t idx = ((int64_t)xx * L + yy) * Lh + z;
                     C f1 = F1[idx], f2 = F2[idx];
-                    pw[a][b] = wz * ((double)f1.x * (double)f2.x + (double)f1.y * (double)f2.y);
-                    power += pw[a][b];
+                    const double pab = wz * ((double)f1.x * (double)f2.x + (double)f1.y * (double)f2.y);
+                    if (a == 0 && b == 0) pw00 = pab;
+                    else if (a == 0) pw01 = pab;
+                    else if (b == 0) pw10 = pab;
+                    else pw11 = pab;
+                    power += pab;
                 }
         }
         for (int b = 0; b < nd; ++b) {
@@ -159,23 +166,35 @@ __global__ __launch_bounds__(RED_THREADS) void k_pair_reduce(int L, const C* __r
                 const hermes_window_desc d = descs[b];
                 // a window along a non-axis line of sight differs between the mirror modes: the aliases of
                 // (L - x, y, z) are those of (x, y, z) with k_x negated, so each mode takes its own sum
-                const bool general = d.type >= HERMES_WIN_RING && d.axis == 3;
-                for (int m = 0; m < (general ? 4 : 1); ++m) {
-                    const double sx = (m & 2) ? -1.0 : 1.0, sy = (m & 1) ? -1.0 : 1.0;
-                    double km = 0.0;
+                if (GEN && d.type >= HERMES_WIN_RING && d.axis == 3) {
+                    const double inv_L = 1.0 / L;
+                    for (int m = 0; m < 4; ++m) {
+                        const double sx = (m & 2) ? -1.0 : 1.0, sy = (m & 1) ? -1.0 : 1.0;
+                        const double pm = m == 0 ? pw00 : (m == 1 ? pw01 : (m == 2 ? pw10 : pw11));
+                        if (pm == 0.0) continue;  // a mode that is its own mirror, or a zero product
+                        double km = 0.0;
+                        for (int a = 0; a < 8; ++a) {
+                            int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
+                            int ix = ax < 0 ? -ax : ax, iy = ay < 0 ? -ay : ay, iz = az < 0 ? -az : az;
+                            double p = P[ix] * P[iy] * P[iz];
+                            const double w = window_value_f64(d, sx * ax * inv_L, sy * ay * inv_L, az * inv_L).x;
+                            km += FP64 ? p * w : (double)((float)p * (float)w);
+                        }
+                        kb += pm * km;
+                    }
+                } else {
                     for (int a = 0; a < 8; ++a) {
                         int ax = (a & 4) ? x - L : x, ay = (a & 2) ? y - L : y, az = (a & 1) ? z - L : z;
                         int ix = ax < 0 ? -ax : ax, iy = ay < 0 ? -ay : ay, iz = az < 0 ? -az : az;
                         double p = P[ix] * P[iy] * P[iz];
-                        if (FP64 || general) {
+                        if (FP64) {
                             double inv_L = 1.0 / L;
-                            const double w = window_value_f64(d, sx * ax * inv_L, sy * ay * inv_L, az * inv_L).x;
-                            km += FP64 ? p * w : (double)((float)p * (float)w);
+                            kb += p * window_value_f64(d, ax * inv_L, ay * inv_L, az * inv_L, GEN).x;
                         } else {
-                            km += (double)((float)p * window_value_f32(d, ax, ay, az, L, cnorms[b]));
+                            kb += (double)((float)p * window_value_f32(d, ax, ay, az, L, cnorms[b]));
                         }
                     }
-                    kb += general ? pw[m >> 1][m & 1] * km : power * km;
+                    kb *= power;
                 }
                 if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && x == 0 && y == 0 && z == 0) kb = 0.0;
             }
@@ -1112,11 +1131,19 @@ int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const vo
     HERMES_CHECK_HIP(hipMemcpyAsync(d_cn, cn.data(), sizeof(float) * n_desc, hipMemcpyHostToDevice, ctx->stream));
     size_t shmem = sizeof(double) * (RED_THREADS / 64) * n_desc;
     HERMES_REQUIRE(shmem <= 64 * 1024, "too many windows for one fused pass");
-    if (dbl)
-        k_pair_reduce<double2, true><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
+    bool gen = false;
+    for (int b = 0; b < n_desc; ++b) gen = gen || (descs[b].type >= HERMES_WIN_RING && descs[b].axis == 3);
+    if (dbl && gen)
+        k_pair_reduce<double2, true, true><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
             L, (const double2*)F1, (const double2*)F2, P, d_desc, d_cn, n_desc, partial);
+    else if (dbl)
+        k_pair_reduce<double2, true, false><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
+            L, (const double2*)F1, (const double2*)F2, P, d_desc, d_cn, n_desc, partial);
+    else if (gen)
+        k_pair_reduce<float2, false, true><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
+            L, (const float2*)F1, (const float2*)F2, P, d_desc, d_cn, n_desc, partial);
     else
-        k_pair_reduce<float2, false><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
+        k_pair_reduce<float2, false, false><<<nblocks, RED_THREADS, shmem, ctx->stream>>>(
             L, (const float2*)F1, (const float2*)F2, P, d_desc, d_cn, n_desc, partial);
     HERMES_LAUNCH_CHECK();
     k_sum_partial_groups<<<hermes_blocks((int64_t)ng * n_desc, 256), 256, 0, ctx->stream>>>(partial, nblocks, n_desc,
