@@ -137,9 +137,12 @@ class WindowFunc(SFCField):
         self.is_composite_window = False
 
     def _validate_builtin_arguments(self):
-        """window.py:131-142; line-of-sight windows: R, H and an axis line of sight, checked up front."""
+        """window.py:131-142; line-of-sight windows: R, H, a non-zero line of sight (octant mode only along
+        an axis), checked up front."""
         if not self.has_custom_func and self.type in LOS_TYPES:
-            los_desc(self.type, self.rescale_len_args, self.los_args)
+            if los_desc(self.type, self.rescale_len_args, self.los_args).axis == 3 and self.kernel_mode == "octant":
+                raise NotImplementedError("pyhermes_amd builds line-of-sight windows along a non-axis "
+                                          "direction with kernel_mode 'auto' / 'full_rfft' only")
             return
         if self.has_custom_func or self.type != "thick_shell":
             return
@@ -229,11 +232,8 @@ class WindowFunc(SFCField):
                 # even in every component of k for an axis line of sight: the rFFT kernel of the reference's
                 # "auto" (full_rfft) mode and the octant fold sum the same alias values.  Along any other
                 # direction only the signed-alias (full_rfft) sum is built.
-                desc = los_desc(self.type, self.rescale_len_args, self.los_args)
-                if desc.axis == 3 and self.kernel_mode == "octant":
-                    raise NotImplementedError("pyhermes_amd builds line-of-sight windows along a non-axis "
-                                              "direction with kernel_mode 'auto' / 'full_rfft' only")
-                t = kernels.window_build_rfft(self.J, P, desc, dbl=dbl)
+                t = kernels.window_build_rfft(self.J, P, los_desc(self.type, self.rescale_len_args, self.los_args),
+                                              dbl=dbl)
             else:
                 t = kernels.window_build_rfft(self.J, P, _iso_desc(self.type, self.rescale_len_args), dbl=dbl)
         self._dev_kernel[key] = t

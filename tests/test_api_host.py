@@ -79,6 +79,11 @@ def test_thick_shell_validation_and_custom_override():
     assert d.axis == 3 and abs(d.los[0] - 0.5 ** 0.5) < 1e-15 and d.los[2] == 0.0
     with pytest.raises(ValueError, match="non-zero"):
         los_axis({"nx": 0.0, "ny": 0.0, "nz": 0.0})
+    with pytest.raises(NotImplementedError, match="non-axis"):
+        WindowFunc({"type": "disk", "len_args": {"R": 1.0, "H": 2.0}, "los_args": [0.0, 1.0, 1.0],
+                    "kernel_mode": "octant"}, sfc)
+    WindowFunc({"type": "disk", "len_args": {"R": 1.0, "H": 2.0}, "los_args": [0.0, 0.0, -1.0],
+                "kernel_mode": "octant"}, sfc)  # an axis line of sight folds to the octant exactly
     with pytest.raises(ValueError, match="requires finite len_args"):
         WindowFunc({"type": "disk", "len_args": {"R": 1.0}}, sfc)
     with pytest.raises(NotImplementedError):
