@@ -22,10 +22,12 @@ weights, rho1 sum_rot sum_i w_i n2(c_i + d12) n3(c_i + d13) / (n_rot sum_i w_i),
 rho1 = sum_i w_i / L^3 -- the same HIP launch with the weights in place of n1.
 Products add d_delta_dd (data centres on delta2, delta3) and r_delta_dd (random
 centres; rrr * xi23 when random1 is a uniform density and no random_pos1 is
-given; the data centres when random1 is a field), and delta_ddd = d_delta_dd -
-r_delta_dd.  Centres come as particle_pos1 / particle_weight1 (and optionally
-random_pos1 / random_weight1) in box units, or, for the data centres, from the
-particle file SFCProjection saved for sfc_field1 (SFCField.get_particle_data).
+given), and delta_ddd = d_delta_dd - r_delta_dd.  Centres come as particle_pos1 /
+particle_weight1 (and random_pos1 / random_weight1) in box units, or from the
+particle files SFCProjection saved: sfc_field1's for the data centres, random1's
+for the random centres of r_delta_dd (SFCField.get_particle_data); a random1 field
+without one ends the task.  rrr without r_delta_dd samples on the data centres
+(corr3pcf.py:1527-1535 passes them when no random centres were resolved).
 """
 
 import copy
@@ -277,6 +279,8 @@ class Corr_3PCF(TaskBase):
                               "Please set shared 'random' or vertex-specific 'random1/2/3'.")
             func_util.safe_exit(1)
         cache, built = {}, {}
+        if self.center == "particle" and "r_delta_dd" in exp and self._random_centres is None:
+            self._random_centres = self._catalogue_random_centres(rsrc[1], cache)
         self.fields, self.randoms = {}, {}
         compat = []
         if needs_data:
@@ -321,6 +325,25 @@ class Corr_3PCF(TaskBase):
             func_util.safe_exit(1)
         self.rho = float(self.geometry.field_mean_density(value_unit="grid"))
         self._fields_prepared = True
+
+    def _catalogue_random_centres(self, random1, cache):
+        """Random vertex-1 centres for r_delta_dd when random_pos1 is not given (corr3pcf.py:825-833, used at
+        :1384-1386): the particles of random1's own catalogue with their projection weights in the task's
+        normalisation.  A uniform random1 needs none (r_delta_dd = rrr xi23).  A random1 field that cannot
+        give its particles ends the task like the reference (logged error, safe_exit)."""
+        if random1 in (None, "") or _is_density(random1):
+            return None
+        if isinstance(random1, str) and random1.strip().lower() == "uniform":
+            return None
+        base = self._load(random1, "random vertex 1", cache)
+        try:
+            pdata = self._normalized(base).get_particle_data()
+            return self._centre_arrays(pdata["pos"], pdata["projection_weight"], "random1 particle positions",
+                                       "random1 particle weight")
+        except Exception:
+            self.logger.error("For center='particle', random1 could not provide usable particle coordinates. "
+                              "Please provide random_pos1 and random_weight1 explicitly.")
+            func_util.safe_exit(1)
 
     # ------------------------------------------------------------ estimator
     def _delta(self, field, random):

@@ -346,6 +346,41 @@ def test_particle_centres_from_saved_particle_data(phi, tmp_path):
         (field - 1.0).get_particle_data()
 
 
+def test_random_centres_from_random_catalogue_field(phi, tmp_path):
+    """r_delta_dd with random1 a projected random catalogue and no random_pos1: the random centres are that
+    catalogue's own particles (corr3pcf.py:825-833, :1384-1386), equal to passing them as random_pos1 /
+    random_weight1, and different from the data-centre estimate d_delta_dd.  A random1 field that cannot give
+    its particles ends the task (SystemExit through safe_exit)."""
+    from pyhermes_amd.base import SFCProjection
+    from pyhermes_amd.theory import Corr_3PCF
+    print("random centres from the random catalogue J=5", flush=True)
+    box, J = 320.0, 5
+    pos, w = _catalogue(8000, box, 31)
+    rpos = np.random.default_rng(12).uniform(0.0, box, (6000, 3))
+
+    def proj(p, wt, name):
+        return SFCProjection({"SFCProjection": {"J": J, "box_size": box, "particle_pos": p, "catalog_weight": wt,
+                                                "weight_normalization": "catalog", "save_particle_data": True,
+                                                "particle_data_path": str(tmp_path / name),
+                                                "fout_path": ""}}).run(save_result=False)
+
+    field = proj(pos, w, "data.npz")
+    rfield = proj(rpos, np.ones(6000), "rand.npz")
+    rpd = rfield.get_particle_data()
+    base = {"sfc_field": field, "random": rfield, "r12": 40.0, "r13": 60.0, "angle_param": "mu",
+            "mu": {"mu_min": -0.5, "mu_max": 0.5, "n_mu": 2}, "n_rot": 3, "center": "particle",
+            "n_box_centers": 1, "base_seed": 5, "products": ["delta_ddd", "rrr"], "fout_path": ""}
+    a = Corr_3PCF({"Corr_3PCF": base}).run(save_result=False)
+    b = Corr_3PCF({"Corr_3PCF": {**base, "random_pos1": rpd["pos"].astype(np.float64),
+                                 "random_weight1": rpd["projection_weight"]}}).run(save_result=False)
+    np.testing.assert_array_equal(a.r_delta_dd, b.r_delta_dd)
+    np.testing.assert_array_equal(a.rrr, b.rrr)
+    np.testing.assert_array_equal(a.delta_ddd, b.delta_ddd)
+    assert np.all(np.abs(a.r_delta_dd - a.d_delta_dd) > 1e-6 * np.abs(a.d_delta_dd).max())
+    with pytest.raises(SystemExit):
+        Corr_3PCF({"Corr_3PCF": {**base, "random": rfield - 0.0}}).run(save_result=False)
+
+
 def test_counting_task(phi, tmp_path):
     """Counting (counting.py:185-258): the field density at default_rng(seed) box points, physical units,
     against the oracle's interpolation, with and without a window; the result file round-trips."""
