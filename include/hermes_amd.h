@@ -167,6 +167,16 @@ int hermes_spectrum_mul(hermes_ctx *ctx, int J, int half, int dbl, void *spec, c
 int hermes_pair_reduce(hermes_ctx *ctx, int J, int dbl, const void *F1, const void *F2, const double *P,
                        const hermes_window_desc *descs, int n_desc, double *out_host);
 
+/* hermes_pair_reduce without any device allocation: no counterpart in the reference.  For callers
+ * that cannot allocate device memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  Same arguments, kernels and results as hermes_pair_reduce, but its scratch (window
+ * tables and per-block partial sums) lives in a fixed 8 MiB device array of the library, one per device,
+ * resolved when the device's first context is created.  A call whose scratch would exceed 8 MiB
+ * (roughly n_desc * 2048 * 8 bytes at L >= 64) fails with status 2.  All calls on one device share the
+ * array: calls that overlap on a device (other threads, other streams) race. */
+int hermes_pair_reduce_noalloc(hermes_ctx *ctx, int J, int dbl, const void *F1, const void *F2, const double *P,
+                               const hermes_window_desc *descs, int n_desc, double *out_host);
+
 /* The same reduction with its window tables prepared once and the result left on
  * the device (capturable): a plan takes thin shells and even-l, m = 0 shell
  * multipoles (l <= 10 in FP32, l <= 4 in FP64) for level J.  out_dev receives

@@ -60,6 +60,8 @@ _SIGNATURES.update({
     "hermes_spectrum_mul": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_double],
     "hermes_pair_reduce": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(WindowDesc), c_int,
                            _dpp],
+    "hermes_pair_reduce_noalloc": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(WindowDesc),
+                                   c_int, _dpp],
     "hermes_multipole_apply_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_int, c_double,
                                    c_void_p],
     "hermes_multipole_window_ifft_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_int, c_double,

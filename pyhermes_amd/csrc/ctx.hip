@@ -93,6 +93,7 @@ int hermes_ctx_create(int device, hermes_ctx** out) {
     HERMES_REQUIRE(device >= 0 && device < n, "hermes_ctx_create: device %d out of range (%d devices)",
                    device, n);
     HERMES_CHECK_HIP(hipSetDevice(device));
+    if (int rc = hermes_scratch_setup(device)) return rc;
     hermes_ctx* c = new hermes_ctx();
     c->device = device;
     c->stream = nullptr;

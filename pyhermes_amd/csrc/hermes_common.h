@@ -50,6 +50,9 @@ void hermes_set_error(const char* fmt, ...);
 // Fill n 32-bit words with v on the stream (a kernel, so it can be captured into graphs).
 hipError_t hermes_fill_u32(hipStream_t st, void* p, size_t n, uint32_t v);
 void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes);
+// Resolves, on the current device, the device address of the allocation-free pair reduction's fixed
+// scratch array (window.hip); called once per device by hermes_ctx_create.
+int hermes_scratch_setup(int device);
 // Reads one device int after the stream's preceding work, waiting by polling the stream (a blocking
 // wait would add the host thread's wake-up latency to every eager call).  Not while capturing.
 int hermes_read_word(hermes_ctx* ctx, const int* dev, int* out);

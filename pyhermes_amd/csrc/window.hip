@@ -7,6 +7,8 @@
 #include "hermes_common.h"
 #include "special.h"
 
+#include <mutex>
+
 namespace {
 
 constexpr int RED_THREADS = 256;
@@ -947,6 +949,18 @@ static hipError_t launch_radial(hipStream_t st, int J, int dbl, int LM, int nblo
     return hipGetLastError();
 }
 
+// Scratch of the pair reduction: the context's grow-only slot, or, for the allocation-free entry point, a
+// fixed buffer of fixed_cap bytes (a request beyond it fails with status 2 and a message).
+static char* pair_workspace(hermes_ctx* ctx, char* fixed_ws, size_t fixed_cap, size_t need) {
+    if (!fixed_ws) return (char*)hermes_workspace(ctx, WS_REDUCE, need);
+    if (need > fixed_cap) {
+        hermes_set_error("hermes_pair_reduce_noalloc: this reduction needs %zu bytes of scratch, more than the "
+                         "fixed %zu (split the windows over several calls)", need, fixed_cap);
+        return nullptr;
+    }
+    return fixed_ws;
+}
+
 static int pair_blocks(int J) {
     const int Lh = (1 << J) / 2 + 1;
     return (int)hermes_blocks((int64_t)Lh * Lh * Lh, RED_THREADS, 2048);
@@ -955,7 +969,8 @@ static int pair_blocks(int J) {
 // The radial path of hermes_pair_reduce.  Returns false (rc untouched) when it does not apply;
 // true with rc = the status otherwise.
 static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
-                               const hermes_window_desc* descs, int n_desc, int nblocks, double* out_host, int& rc) {
+                               const hermes_window_desc* descs, int n_desc, int nblocks, double* out_host,
+                               char* fixed_ws, size_t fixed_cap, int& rc) {
     RadialChannels ch;
     if (!radial_channels(J, dbl, descs, n_desc, ch)) return false;
     rc = 0;
@@ -964,9 +979,9 @@ static bool pair_reduce_radial(hermes_ctx* ctx, int J, int dbl, const void* F1, 
     const size_t off_part = ((off_chan + sizeof(int) * ch.chan.size() + 255) / 256) * 256;
     const int ng = part_groups(nblocks);
     const size_t need = off_part + sizeof(double) * ((size_t)nblocks * nch + (size_t)ng * nch + nch);
-    char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);
+    char* ws = pair_workspace(ctx, fixed_ws, fixed_cap, need);
     if (!ws) {
-        rc = 1;
+        rc = fixed_ws ? 2 : 1;
         return true;
     }
     double* d_radii = (double*)ws;
@@ -1091,8 +1106,9 @@ int hermes_pair_reduce_planned(hermes_ctx* ctx, const hermes_pair_plan* plan, co
     return 0;
 }
 
-int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
-                       const hermes_window_desc* descs, int n_desc, double* out_host) {
+static int pair_reduce_impl(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
+                            const hermes_window_desc* descs, int n_desc, double* out_host, char* fixed_ws,
+                            size_t fixed_cap) {
     HERMES_REQUIRE(ctx && F1 && F2 && P && descs && out_host, "hermes_pair_reduce: null argument");
     HERMES_REQUIRE(J >= 1 && J <= 10, "J=%d unsupported", J);
     HERMES_REQUIRE(n_desc >= 1 && n_desc <= 4096, "hermes_pair_reduce: n_desc=%d out of range", n_desc);
@@ -1112,13 +1128,14 @@ int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const vo
     const int nblocks = (int)hermes_blocks(nrep, RED_THREADS, 2048);
     {
         int rc = -1;
-        if (pair_reduce_radial(ctx, J, dbl, F1, F2, P, descs, n_desc, nblocks, out_host, rc)) return rc;
+        if (pair_reduce_radial(ctx, J, dbl, F1, F2, P, descs, n_desc, nblocks, out_host, fixed_ws, fixed_cap, rc))
+            return rc;
     }
     const int ng = part_groups(nblocks);
     size_t need = sizeof(hermes_window_desc) * n_desc + sizeof(float) * n_desc + 64 +
                   sizeof(double) * ((size_t)nblocks * n_desc + (size_t)ng * n_desc + n_desc);
-    char* ws = (char*)hermes_workspace(ctx, WS_REDUCE, need);
-    HERMES_REQUIRE(ws, "%s", hermes_last_error());
+    char* ws = pair_workspace(ctx, fixed_ws, fixed_cap, need);
+    if (!ws) return fixed_ws ? 2 : 1;
     hermes_window_desc* d_desc = (hermes_window_desc*)ws;
     float* d_cn = (float*)(ws + sizeof(hermes_window_desc) * n_desc);
     double* partial = (double*)(ws + ((sizeof(hermes_window_desc) * n_desc + sizeof(float) * n_desc + 63) / 64) * 64);
@@ -1154,6 +1171,11 @@ int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const vo
     HERMES_CHECK_HIP(hipMemcpyAsync(out_host, d_out, sizeof(double) * n_desc, hipMemcpyDeviceToHost, ctx->stream));
     HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
     return 0;
+}
+
+int hermes_pair_reduce(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
+                       const hermes_window_desc* descs, int n_desc, double* out_host) {
+    return pair_reduce_impl(ctx, J, dbl, F1, F2, P, descs, n_desc, out_host, nullptr, 0);
 }
 
 static int make_apply_args(hermes_ctx* ctx, int J, const double* P, const double* radii, int n_shell, int l,
@@ -1330,3 +1352,41 @@ int hermes_reduce_sum(hermes_ctx* ctx, const void* x, int64_t n, int ncomp, int 
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------- allocation-free pair reduction
+// hermes_pair_reduce_noalloc runs the kernels of hermes_pair_reduce with their scratch (window tables,
+// per-block partial sums, group sums, the result row) in this fixed device array instead of the context's
+// grow-only workspace, so it never allocates device memory.  Each device holds its own copy until the
+// process ends; its address on a device comes from hipGetSymbolAddress, called once per device when the
+// first context on it is created (hermes_ctx_create).  Host code must use that address, never the array's
+// name.  Every call on a device shares the array, so calls that overlap on one device race; the kernels
+// write every scratch word they read, so nothing needs clearing between calls.
+constexpr size_t PAIR_SCRATCH_BYTES = size_t(8) << 20;
+static __device__ __align__(256) char scratch[PAIR_SCRATCH_BYTES];
+
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static char* g_scratch_ptr[SCRATCH_MAX_DEVICES] = {nullptr};
+static std::mutex g_scratch_mu;
+
+int hermes_scratch_setup(int device) {
+    HERMES_REQUIRE(device >= 0 && device < SCRATCH_MAX_DEVICES, "device %d beyond the scratch table", device);
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[device]) return 0;
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !ptr) {
+        hermes_set_error("hipGetSymbolAddress(scratch) on device %d failed: %s", device, hipGetErrorString(e));
+        return 1;
+    }
+    g_scratch_ptr[device] = (char*)ptr;
+    return 0;
+}
+
+int hermes_pair_reduce_noalloc(hermes_ctx* ctx, int J, int dbl, const void* F1, const void* F2, const double* P,
+                               const hermes_window_desc* descs, int n_desc, double* out_host) {
+    HERMES_REQUIRE(ctx, "hermes_pair_reduce_noalloc: null context");
+    HERMES_REQUIRE(ctx->device >= 0 && ctx->device < SCRATCH_MAX_DEVICES && g_scratch_ptr[ctx->device],
+                   "hermes_pair_reduce_noalloc: no scratch address for device %d", ctx->device);
+    return pair_reduce_impl(ctx, J, dbl, F1, F2, P, descs, n_desc, out_host, g_scratch_ptr[ctx->device],
+                            PAIR_SCRATCH_BYTES);
+}
