@@ -31,8 +31,11 @@ Parity pinning (what fixes each function independently of retyping it):
   force pair sums on tiny grids.
 * ``multipole``: brute-force 27x27 triple sum on tiny grids and the constant
   field closed form v = 4*pi*a*b*c.
-* ``mc``: numpy RandomState streams (parity of Numba's np.random.seed/rand
-  stream with numpy's MT19937 is believed, not pinned -- "parity unpinned"
-  for the rotation draws; everything downstream of the draws is pinned by
-  brute force).
+* ``mt19937`` (the rotation draws of ``mc``): Matsumoto & Nishimura's
+  init_genrand + genrand_res53, the generator Numba's np.random.seed/rand
+  implement, pinned by the C++ standard's known value (std::mt19937 seeded 5489
+  gives 4123659995 as its 10,000th output) and by libstdc++'s std::mt19937
+  for other seeds; everything downstream of the draws is pinned by brute force.
+* ``multipole.xi_multipole``: the Gauss-Legendre mu-integral of the ring-window
+  (s, mu) pair products, an independent route to the xi_l prefactor.
 """

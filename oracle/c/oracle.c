@@ -330,6 +330,45 @@ void oracle_radial_multipole_window(int L, const double *P, double R, int l, int
     }
 }
 
+/* Pair product of a real field with the real, even window Re K_{l,0} of radial_multipole_window (even l),
+ * from the field's rFFT half spectrum Fh (L, L, L/2+1) interleaved complex128: the full-grid sum
+ * sum_k Re K(k) |F(k)|^2, with |F(-k)| = |F(k)| and K(-k) = K(k) folding it onto the half grid (planes
+ * z = 0 and z = L/2 once, the others twice).  Each point's K is the 8-alias sum of the function above.
+ * corr2pcf.py:366-372 in Parseval form; the caller divides by N^2. */
+double oracle_pair_half_multipole(int L, const double *P, double R, int l, const double *Fh) {
+    const double inv_L = 1.0 / L;
+    const int Lh = L / 2 + 1;
+    double total = 0.0;
+#pragma omp parallel for schedule(static) reduction(+ : total)
+    for (int x = 0; x < L; ++x) {
+        double pi0 = P[L - x], pi1 = P[x];
+        double ki0 = (x - L) * inv_L, ki1 = x * inv_L;
+        double row = 0.0;
+        for (int y = 0; y < L; ++y) {
+            double pj0 = P[L - y], pj1 = P[y];
+            double kj0 = (y - L) * inv_L, kj1 = y * inv_L;
+            for (int z = 0; z < Lh; ++z) {
+                double pk0 = P[L - z], pk1 = P[z];
+                double kk0 = (z - L) * inv_L, kk1 = z * inv_L;
+                double t = pi0 * pj0 * pk0 * radial_multipole_eval(ki0, kj0, kk0, R, l, 0).re;
+                t += pi0 * pj0 * pk1 * radial_multipole_eval(ki0, kj0, kk1, R, l, 0).re;
+                t += pi0 * pj1 * pk0 * radial_multipole_eval(ki0, kj1, kk0, R, l, 0).re;
+                t += pi0 * pj1 * pk1 * radial_multipole_eval(ki0, kj1, kk1, R, l, 0).re;
+                t += pi1 * pj0 * pk0 * radial_multipole_eval(ki1, kj0, kk0, R, l, 0).re;
+                t += pi1 * pj0 * pk1 * radial_multipole_eval(ki1, kj0, kk1, R, l, 0).re;
+                t += pi1 * pj1 * pk0 * radial_multipole_eval(ki1, kj1, kk0, R, l, 0).re;
+                t += pi1 * pj1 * pk1 * radial_multipole_eval(ki1, kj1, kk1, R, l, 0).re;
+                size_t o = ((size_t)x * L + y) * Lh + z;
+                double p2 = Fh[2 * o] * Fh[2 * o] + Fh[2 * o + 1] * Fh[2 * o + 1];
+                double wz = (z == 0 || 2 * z == L) ? 1.0 : 2.0;
+                row += wz * t * p2;
+            }
+        }
+        total += row;
+    }
+    return total;
+}
+
 /* ---------------------------------------------------------------------------
  * corr3pcf_multipoles.py:134-170 (compute_3d_summand_cpu): returns the raw
  * complex sum (without the 4*pi/N factor).  F1, F2 interleaved complex128.

@@ -4,22 +4,19 @@ import ctypes
 
 import numpy as np
 
-from . import native
+from . import mt19937, native
 
 
 def rotation_uniforms(seed_base_rot, n_rot, mu_index):
-    """corr3pcf_kernels.py:119-125: per rotation ``np.random.seed(s); rand(3)``.
-
-    Numba's ``np.random.seed``/``rand`` is believed to reproduce numpy's legacy
-    MT19937 ``RandomState(s).rand(3)``; that equality is parity unpinned (only
-    an import of numba could confirm it).
-    """
+    """corr3pcf_kernels.py:119-125: per rotation ``np.random.seed(s); rand(3)`` inside the Numba kernel,
+    restated with Numba's generator (MT19937 init_genrand + genrand_res53, oracle/mt19937.py), seed
+    s = seed_base_rot + irot (+ mu_index * 1000003, corr3pcf.py:1462)."""
     out = np.empty((n_rot, 3))
     for irot in range(n_rot):
         s = seed_base_rot + irot
         if mu_index >= 0:
             s += mu_index * 1000003
-        out[irot] = np.random.RandomState(s).rand(3)
+        out[irot] = mt19937.uniforms(s, 3)
     return out
 
 

@@ -163,3 +163,52 @@ def los_rfft_kernel(L, P, win_type, R, H, los=(0.0, 0.0, 1.0)):
         (iz, kz) = neg[2] if sz else pos[2]
         out = out + P[ix] * P[iy] * P[iz] * los_window(win_type, kx, ky, kz, R, H, los)
     return out
+
+
+# ---------------------------------------------------------------- 2PCF multipoles xi_l(s)
+# The reference has no xi_l task; the build defines it (corr2pcf.py:366-372 pair products with the shell
+# multipole window of radial_multipole_windows.py:128-144, m = 0):
+#   xi_l(s) = (-i)^l sqrt(4 pi (2l + 1)) <(delta @ W_{s,l,0}) delta> / rr,   W = j_l(2 pi |k| s) Y_l0(k-hat).
+# Two independent routes below.  For even l they agree: the plane-wave expansion gives
+# (2l + 1) int_0^1 dmu P_l(mu) J0(q_perp) cos(q_par) = i^l sqrt(4 pi (2l + 1)) j_l(q) Y_l0(k-hat) per alias,
+# so the Gauss-Legendre mu-integral of the ring-window pair products xi(s, mu) (window_functions.py:164-192,
+# corr2pcf.py:275-284 mapping R = s sqrt(1 - mu^2), H = s mu) pins the prefactor.
+
+def xi_ell_window_route(delta, P, s_grid, ell, rr, workers=1):
+    """xi_l(s) from the complex shell-multipole window: (-i)^l sqrt(4 pi (2l+1)) Re<(delta @ W_l0) delta>/rr
+    (s in grid units; even l, where (-i)^l = (-1)^(l/2))."""
+    if ell % 2:
+        raise ValueError("xi_l of a real field is defined for even l")
+    L = delta.shape[0]
+    K = radial_multipole_window(L, P, float(s_grid), int(ell), 0)
+    pp = float(np.real(np.sum(convolve_complex(delta, K, workers) * delta))) / delta.size
+    return (-1.0) ** (ell // 2) * np.sqrt(4.0 * np.pi * (2 * ell + 1)) * pp / rr
+
+
+def xi_ell_ring_route(delta, P, s_grid, ell, rr, n_mu=64, los=(0.0, 0.0, 1.0), workers=1):
+    """xi_l(s) = (2l + 1) int_0^1 xi(s, mu) P_l(mu) dmu with xi(s, mu) from ring-window pair products at
+    n_mu Gauss-Legendre nodes on [0, 1] (exact for the even integrand up to degree 4 n_mu - 1 in mu)."""
+    from scipy.special import eval_legendre
+    L = delta.shape[0]
+    x, w = np.polynomial.legendre.leggauss(int(n_mu))
+    mu, wmu = 0.5 * (x + 1.0), 0.5 * w
+    s = float(s_grid)
+    total = 0.0
+    for m, wm in zip(mu, wmu):
+        K = los_rfft_kernel(L, P, "ring", s * np.sqrt(1.0 - m * m), s * m, los)
+        total += wm * eval_legendre(int(ell), m) * pair_product(delta, delta, K, workers) / rr
+    return (2 * ell + 1) * total
+
+
+def xi_ell_half_spectrum(delta, P, s_grid, ell, rr, workers=1):
+    """xi_window_route from the rFFT half spectrum without building the window grid (large grids):
+    (-i)^l sqrt(4 pi (2l+1)) sum_k Re K_l0(k) |F(k)|^2 / N^2 / rr for even l."""
+    if ell % 2:
+        raise ValueError("xi_l of a real field is defined for even l")
+    Fh = np.ascontiguousarray(rfftn(np.asarray(delta, dtype=np.float64), workers=workers))
+    P = native.f64(P)
+    L = delta.shape[0]
+    raw = native.lib().oracle_pair_half_multipole(L, native.ptr(P), float(s_grid), int(ell),
+                                                   native.ptr(Fh.view(np.float64)))
+    pp = raw / float(delta.size) ** 2
+    return (-1.0) ** (ell // 2) * np.sqrt(4.0 * np.pi * (2 * ell + 1)) * pp / rr

@@ -18,35 +18,41 @@ from ..utils import func_util
 from .base import HermesData
 
 
+# Normalisation vocabularies (sfc_field.py:21-47): accepted spellings -> canonical name, and the error for
+# anything else.  A projection folds 'unit' into 'field' (unit weights are the field normalisation of the
+# weights); tasks and field values keep 'unit' (rescale the field to unit integral).
+_FOUR = ("raw", "catalog", "field", "unit")
+_VOCABULARIES = {
+    "projection": ({**{m: m for m in _FOUR}, "unit": "field"},
+                   "weight_normalization must be one of 'raw', 'catalog', 'field', or 'unit'."),
+    "task": ({m: m for m in _FOUR}, "task weight_normalization must be one of 'raw', 'catalog', 'field', or 'unit'."),
+    "value": ({m: m for m in _FOUR}, "normalization must be None, 'raw', 'catalog', 'field', or 'unit'."),
+}
+
+
+def _canonical(vocabulary, value, none_ok=False):
+    names, message = _VOCABULARIES[vocabulary]
+    if value is None and none_ok:
+        return None
+    key = None if value is None else str(value).strip().lower()
+    if key not in names:
+        raise ValueError(message)
+    return names[key]
+
+
 def normalize_weight_normalization(weight_normalization):
-    """sfc_field.py:21-29."""
-    if weight_normalization is None:
-        raise ValueError("weight_normalization must be one of 'raw', 'catalog', 'field', or 'unit'.")
-    mode = str(weight_normalization).strip().lower()
-    if mode == "unit":
-        return "field"
-    if mode not in {"raw", "catalog", "field"}:
-        raise ValueError("weight_normalization must be one of 'raw', 'catalog', 'field', or 'unit'.")
-    return mode
+    """Projection weight normalisation: 'raw', 'catalog' or 'field' ('unit' means 'field')."""
+    return _canonical("projection", weight_normalization)
 
 
 def normalize_task_weight_normalization(weight_normalization):
-    """sfc_field.py:32-38."""
-    if weight_normalization is None:
-        raise ValueError("task weight_normalization must be one of 'raw', 'catalog', 'field', or 'unit'.")
-    mode = str(weight_normalization).strip().lower()
-    if mode not in {"raw", "catalog", "field", "unit"}:
-        raise ValueError("task weight_normalization must be one of 'raw', 'catalog', 'field', or 'unit'.")
-    return mode
+    """A task's weight normalisation: 'raw', 'catalog', 'field' or 'unit'."""
+    return _canonical("task", weight_normalization)
 
 
 def normalize_field_value_normalization(normalization):
-    if normalization is None:
-        return None
-    mode = str(normalization).strip().lower()
-    if mode not in {"raw", "catalog", "field", "unit"}:
-        raise ValueError("normalization must be None, 'raw', 'catalog', 'field', or 'unit'.")
-    return mode
+    """The normalisation a field's values are wanted in: None (as they are) or one of the four names."""
+    return _canonical("value", normalization, none_ok=True)
 
 
 def _scalar(x):
@@ -273,32 +279,39 @@ class SFCField(HermesData):
             raise ValueError(f"Cannot use weight_normalization='{mode}' without a finite, non-zero {label}.")
         return float(value)
 
-    def get_particle_data(self):
-        """sfc_field.py:425-453: the catalogue behind a catalogue field, from the particle file SFCProjection
-        saved (save_particle_data=True): pos, catalog_weight, normalized_catalog_weight, field_value and
-        projection_weight (in the field's weight normalization).  Re-reading the original input catalogue
-        (sfc_field.py:455-480) needs the reference's file readers and is not provided."""
+    def _particle_file(self):
+        """The particle file behind a catalogue field (sfc_field.py:425-445 conditions and messages)."""
         if getattr(self, "field_kind", None) != "catalog_field":
             raise ValueError("Only catalog fields can provide particle data. Derived fields do not retain a unique "
                              "particle catalogue.")
         if not bool(getattr(self, "particle_data_retrievable", True)):
             raise ValueError("This field no longer maps to one retrievable particle catalogue. Provide "
                              "particle_pos1 and particle_weight1 explicitly for particle-centred statistics.")
-        den = self._normalization_denominator(getattr(self, "weight_normalization", "raw"))
         path = getattr(self, "particle_data_path", "")
         if not path:
             raise ValueError("This field has no saved particle file (project with save_particle_data=True); "
                              "reading the input catalogue again is not provided by pyhermes_amd.")
-        with np.load(path, allow_pickle=False) as pd:
-            required = {"pos", "catalog_weight", "field_value"}
-            if not required.issubset(pd.files):
-                self.logger.error(f"Particle dataset '{path}' must contain {sorted(required)} arrays.")
+        return path
+
+    def get_particle_data(self):
+        """The catalogue behind a catalogue field, from the particle file SFCProjection saved
+        (save_particle_data=True): pos, catalog_weight, normalized_catalog_weight, field_value and
+        projection_weight, the last in the field's own weight normalisation (sfc_field.py:425-453).  Reading
+        the original input catalogue again (sfc_field.py:455-480) needs the reference's file readers and is
+        not provided."""
+        path = self._particle_file()
+        den = self._normalization_denominator(getattr(self, "weight_normalization", "raw"))
+        with np.load(path, allow_pickle=False) as archive:
+            missing = {"pos", "catalog_weight", "field_value"} - set(archive.files)
+            if missing:
+                self.logger.error(f"Particle dataset '{path}' must contain "
+                                  f"{sorted({'pos', 'catalog_weight', 'field_value'})} arrays.")
                 func_util.safe_exit(1)
-            cw = np.asarray(pd["catalog_weight"], dtype=np.float64)
-            fv = np.asarray(pd["field_value"], dtype=np.float64)
-            return {"pos": np.asarray(pd["pos"]), "catalog_weight": cw,
-                    "normalized_catalog_weight": cw / float(self.catalog_weight_sum), "field_value": fv,
-                    "projection_weight": cw * fv / den}
+            arrays = {k: np.asarray(archive[k]) for k in ("pos", "catalog_weight", "field_value")}
+        w = arrays["catalog_weight"].astype(np.float64)
+        q = arrays["field_value"].astype(np.float64)
+        return {"pos": arrays["pos"], "catalog_weight": w, "normalized_catalog_weight": w / float(self.catalog_weight_sum),
+                "field_value": q, "projection_weight": w * q / den}
 
     def _field_integral_for_normalization(self, weight_normalization):
         raw = getattr(self, "raw_field_weighted_sum", None)
@@ -339,32 +352,37 @@ class SFCField(HermesData):
         return new
 
     def with_normalization(self, normalization=None):
+        """This field in another normalisation (sfc_field.py:397-405): None keeps it, 'unit' rescales to unit
+        integral, 'raw' / 'catalog' / 'field' re-weight a catalogue field."""
         mode = normalize_field_value_normalization(normalization)
-        if mode is None:
-            return self
-        if mode == "unit":
-            return self.to_unit_weight()
-        if getattr(self, "field_kind", None) != "catalog_field":
-            raise ValueError("normalization='raw', 'catalog', or 'field' can only be applied to catalog fields.")
-        return self.switch_weight_normalization(mode)
+        if mode in (None, "unit"):
+            return self if mode is None else self.to_unit_weight()
+        if getattr(self, "field_kind", None) == "catalog_field":
+            return self.switch_weight_normalization(mode)
+        raise ValueError("normalization='raw', 'catalog', or 'field' can only be applied to catalog fields.")
+
+    _VALUE_UNITS = ("auto", "grid", "physical")
 
     def _resolve_value_unit(self, value_unit, sample_field=None):
-        field = self if sample_field is None else sample_field
-        value_unit = str(value_unit).strip().lower()
-        if value_unit not in {"auto", "grid", "physical"}:
+        """'grid' (per grid cell) or 'physical' (per box volume unit); 'auto' is physical for catalogue fields
+        and grid for derived ones (sfc_field.py:407-414)."""
+        unit = str(value_unit).strip().lower()
+        if unit not in self._VALUE_UNITS:
             raise ValueError("value_unit must be 'auto', 'grid', or 'physical'.")
-        if value_unit == "auto":
-            value_unit = "physical" if getattr(field, "field_kind", None) == "catalog_field" else "grid"
-        return value_unit
+        if unit != "auto":
+            return unit
+        kind = getattr(self if sample_field is None else sample_field, "field_kind", None)
+        return "physical" if kind == "catalog_field" else "grid"
 
     def field_mean_density(self, normalization=None, value_unit="auto"):
-        field = self if normalization is None else self.with_normalization(normalization)
-        value_unit = self._resolve_value_unit(value_unit, sample_field=field)
-        value = getattr(field, "field_integral", None)
-        if value is None:
+        """Mean density: the field integral over the volume, in grid cells or box units (sfc_field.py:416-423);
+        None when the integral is unknown."""
+        field = self.with_normalization(normalization)
+        unit = self._resolve_value_unit(value_unit, sample_field=field)
+        total = getattr(field, "field_integral", None)
+        if total is None:
             return None
-        volume = float(field.V) if value_unit == "grid" else float(field.box_size) ** 3
-        return value / volume
+        return total / (float(field.V) if unit == "grid" else float(field.box_size) ** 3)
 
     # ------------------------------------------------------------ sampling at positions
     def field_density_at_pos(self, pos, epsilon=None, filter=None, normalization=None, value_unit="auto"):

@@ -300,3 +300,54 @@ def test_counting_defaults_and_validation():
     assert t.weight_normalization == "catalog"
     with pytest.raises(ValueError, match="non-negative"):
         Counting({"Counting": {"sfc_field": "unused", "random_count": -1}})
+
+
+def test_window_parameter_schema_keys_defaults_and_errors():
+    """The window-parameter schema keeps the reference's keys, defaults, error types and messages
+    (window_params.py): built-in binning strings, type clean-up, unset lengths, the z line of sight of the
+    cylindrical windows, kernel-mode defaults and validation, len_args / los_args coercion."""
+    from pyhermes_amd.utils import window_params as wp
+    assert wp.normalize_binning_window_template(None) == {
+        "type": "shell", "len_args": {"R": None}, "los_args": {}, "other_args": {}, "mapping": "s_to_R",
+        "kernel_mode": "octant"}
+    ring = wp.normalize_binning_window_template(" Ring ")
+    assert ring == {"type": "ring", "len_args": {"R": None, "H": None}, "los_args": {"nx": 0.0, "ny": 0.0, "nz": 1.0},
+                    "other_args": {}, "mapping": "smu_to_RH", "kernel_mode": "auto"}
+    thick = wp.normalize_binning_window_template({"type": "THICK_SHELL", "len_args": ["R"]})
+    assert thick["len_args"] == {"R": None, "delta_R": None} and thick["mapping"] == "s_to_R"
+    # an untyped dict becomes a shell without the shell defaults; a func makes it custom
+    bare = wp.normalize_binning_window_template({"len_args": {"R": 5.0}})
+    assert bare["type"] == "shell" and "mapping" not in bare and bare["kernel_mode"] == "octant"
+    f = wp.normalize_binning_window_template({"func": lambda k, **kw: k})
+    assert f["type"] == "custom" and f["kernel_mode"] == "full_rfft" and f["los_args"] == {}
+    assert wp.normalize_binning_window_template({"type": "disk", "los_args": [1, 0, 0]})["los_args"] == {
+        "nx": 1.0, "ny": 0.0, "nz": 0.0}
+    with pytest.raises(ValueError, match="Supported built-in strings are 'shell', 'thick_shell', 'ring', 'disk', "
+                                         "'cylinder', and 'cylshell'"):
+        wp.normalize_binning_window_template("sphere")
+    with pytest.raises(TypeError, match="expected dict, string, or None"):
+        wp.normalize_binning_window_template(3)
+    with pytest.raises(TypeError, match="'func' must be callable"):
+        wp.normalize_binning_window_template({"type": "shell", "func": 1})
+    with pytest.raises(ValueError, match=r"Unsupported kernel_mode 'bogus'. Supported values are \['auto', "
+                                         r"'complex_full_fft', 'complex_rfft', 'full_rfft', 'octant'\]"):
+        wp.normalize_binning_window_template({"type": "shell", "kernel_mode": " BOGUS"})
+    assert wp.default_kernel_mode("radial_multipole") == "complex_full_fft"
+    assert wp.default_kernel_mode("directional_derivative") == "complex_rfft"
+    assert wp.default_kernel_mode("sphere") == "octant" and wp.default_kernel_mode("ring", True) == "full_rfft"
+    assert wp.normalize_len_args("R") == {"R": None} and wp.normalize_len_args(None) == {}
+    with pytest.raises(TypeError, match="len_args entries must be strings."):
+        wp.normalize_len_args(["R", 2])
+    with pytest.raises(TypeError, match="len_args must be a dict, string, list, tuple, or None."):
+        wp.normalize_len_args(4.0)
+    with pytest.raises(ValueError, match=r"exactly three values: \[nx, ny, nz\]"):
+        wp.normalize_los_args([0, 1])
+    with pytest.raises(ValueError, match="must define nx, ny, and nz together"):
+        wp.normalize_los_args({"nx": 1.0})
+    with pytest.raises(ValueError, match="vector must be non-zero"):
+        wp.normalize_los_args({"nx": 0, "ny": 0.0, "nz": -0.0})
+    with pytest.raises(TypeError, match="los_args must be a dict, length-3 array, or None."):
+        wp.normalize_los_args("z")
+    assert wp.normalize_los_args({}, "sphere") == {}
+    ser = wp.serialize_window_params({"func": np.sin, "len_args": {"R": np.float64(2.0)}, "v": np.arange(2)})
+    assert ser == {"custom_func": {"kind": "callable", "name": "numpy.sin"}, "len_args": {"R": 2.0}, "v": [0, 1]}

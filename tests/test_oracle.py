@@ -510,3 +510,63 @@ def test_particle_centre_estimator_pins(phi):
     assert a == pytest.approx(b, rel=1e-12)
     c = mc.triplet_particle_centers(R1, R2, mu, cen, 3.0 * w, 0.7, abc, e2, e3, phi)
     assert c == pytest.approx(mc.triplet_particle_centers(R1, R2, mu, cen, w, 0.7, abc, e2, e3, phi), rel=1e-12)
+
+
+def _anisotropic_field(L, seed):
+    rng = np.random.default_rng(seed)
+    F = np.fft.fftn(rng.normal(size=(L, L, L)))
+    k = np.fft.fftfreq(L)
+    F *= np.exp(-20.0 * k[:, None, None] ** 2 - 5.0 * k[None, :, None] ** 2 - 2.0 * k[None, None, :] ** 2)
+    d = np.real(np.fft.ifftn(F))
+    return d - d.mean()
+
+
+@pytest.mark.parametrize("ell", [0, 2, 4])
+def test_xi_ell_prefactor_by_gauss_legendre_ring_route(phi, ell):
+    """The build-defined xi_l(s) = (-i)^l sqrt(4 pi (2l+1)) <(delta @ j_l Y_l0) delta> / rr equals
+    (2l+1) int_0^1 xi(s, mu) P_l(mu) dmu over the ring-window (s, mu) route (window_functions.py:164-192,
+    corr2pcf.py:275-284) on an anisotropic field; a dropped (2l+1), a sign flip or the odd-l factor i^l
+    in place of (-1)^(l/2) would each miss by far more than the tolerance."""
+    L = 16
+    d = _anisotropic_field(L, 5)
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    rr = 0.37
+    for s in (2.5, 5.0, 6.75):
+        a = windows.xi_ell_window_route(d, P, s, ell, rr)
+        b = windows.xi_ell_ring_route(d, P, s, ell, rr)
+        assert abs(a) > 1e-6
+        assert a == pytest.approx(b, rel=1e-10)
+        if ell:
+            assert abs(a / (2 * ell + 1) - b) > 1e-3 * abs(b)
+            assert abs(-a - b) > 1e-3 * abs(b)
+
+
+def test_summand_pairs_regrouping_matches_summand(phi):
+    """The matrix regrouping of the contraction for many field pairs equals the per-pair restatement of
+    compute_3d_summand_cpu (corr3pcf_multipoles.py:134-170) for every pair, including swapped pairs."""
+    L = 8
+    rng = np.random.default_rng(41)
+    d = rng.normal(size=(L, L, L))
+    F1 = [rng.normal(size=(L, L, L)) + 1j * rng.normal(size=(L, L, L)) for _ in range(3)]
+    F2 = [rng.normal(size=(L, L, L)) + 1j * rng.normal(size=(L, L, L)) for _ in range(2)]
+    g = scaling.cal_gamma(phi, 3, 1024)
+    got = multipole.summand_pairs(d, F1, F2, g)
+    for i in range(3):
+        for j in range(2):
+            want = multipole.summand_c(d, F1[i], F2[j], g)
+            assert abs(got[i, j] - want) <= 1e-12 * abs(want)
+    same = multipole.summand_pairs(d, F1, F1, g)
+    assert abs(same[2, 0] - multipole.summand_c(d, F1[2], F1[0], g)) <= 1e-12 * abs(same[2, 0])
+
+
+@pytest.mark.parametrize("ell", [0, 2, 4])
+def test_xi_ell_half_spectrum_sum_matches_window_route(phi, ell):
+    """The half-spectrum Parseval sum of the multipole pair product (used at large grids) equals the
+    convolution route with the full complex window (corr2pcf.py:366-372), even and odd L/2 planes included."""
+    for L in (8, 16):
+        d = _anisotropic_field(L, 7 + L)
+        P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+        for s in (1.75, 0.3 * L):
+            a = windows.xi_ell_window_route(d, P, s, ell, 0.8)
+            b = windows.xi_ell_half_spectrum(d, P, s, ell, 0.8)
+            assert b == pytest.approx(a, rel=1e-11, abs=1e-14)
