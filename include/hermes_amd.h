@@ -137,6 +137,39 @@ typedef struct {
     double los[3]; /* unit line-of-sight vector when axis = 3 */
 } hermes_window_desc;
 
+/* ---- radial profiles of multipole windows (radial_multipole_windows.py:17-126, :186-316;
+ *      radial_profiles.py).  The window of (l, m) is U_l(|k|) Y_l^m(k-hat) summed over the 8 signed
+ * aliases with |phi_hat|^2 weights, where U_l is
+ *   SHELL          j_l(2 pi k a0)                      (any l; a0 = R)
+ *   TABLE          the cubic interpolation of a table of U_l (any l): table[j] = U_l at
+ *                  j / k_scale cycles per grid cell, j < n_table (the local cubic through the four
+ *                  nearest samples, linear in the first and last interval, clamped outside)
+ *   SPHERE, GAUSSIAN, THICK_SHELL (a0 = R, a1 = delta_R), GAUSSIAN_SHELL (a0 = R_shell, a1 = R_smooth):
+ *                  the closed-form transfer of the isotropic window of that name (l = 0 only).
+ * Lengths are in grid units, k in cycles per grid cell; table is a DEVICE pointer of n_table doubles. */
+enum {
+    HERMES_RADIAL_SHELL = 0,
+    HERMES_RADIAL_TABLE = 1,
+    HERMES_RADIAL_SPHERE = 2,
+    HERMES_RADIAL_GAUSSIAN = 3,
+    HERMES_RADIAL_THICK_SHELL = 4,
+    HERMES_RADIAL_GAUSSIAN_SHELL = 5
+};
+
+typedef struct {
+    int32_t kind, n_table;
+    double a0, a1;
+    double k_scale;
+    const double *table;
+} hermes_radial_profile;
+
+/* Complex full-grid kernel (L, L, L) of the (l, m) window of one radial profile.  Replaces
+ * calculate_radial_multipole_window_array (radial_multipole_windows.py:267-316): the direct builder of
+ * :147-183 for thin shells and closed-form monopoles, the tabulated builder of :186-264 otherwise.
+ * dbl selects FP64 or FP32 output. */
+int hermes_window_build_full_profile(hermes_ctx *ctx, int J, const double *P, const hermes_radial_profile *prof,
+                                     int l, int m, int dbl, void *out);
+
 /* Real rFFT-layout kernel (L, L, L/2+1) of one isotropic window, summed over the
  * 8 aliases with |phi_hat|^2 weights P (L+1 values).  Replaces
  * calculate_real_window_octant_array_numba + fold_octant_window_to_rfft_kernel,
@@ -214,6 +247,14 @@ int hermes_multipole_window_ifft_f32(hermes_ctx *ctx, int J, const void *H, cons
 int hermes_multipole_window_ifft_multi_f32(hermes_ctx *ctx, int J, const void *H, const double *P,
                                            const double *radii, int n_shell, int l, const int *ms_host, int n_m,
                                            double scale, void *out);
+
+/* hermes_multipole_window_ifft_multi_f32 with a radial profile per shell instead of thin-shell radii
+ * (profs_host: n_shell host structs whose tables are device pointers; analytic monopole kinds need l = 0).
+ * The fused window kernel of the SH-3PCF edges with non-thin profiles (corr3pcf_multipoles.py:267-321 with
+ * radial_multipole_windows.py:186-316).  No packed variant. */
+int hermes_multipole_window_ifft_profiles_f32(hermes_ctx *ctx, int J, const void *H, const double *P,
+                                              const hermes_radial_profile *profs_host, int n_shell, int l,
+                                              const int *ms_host, int n_m, double scale, void *out);
 
 /* The (l, 0) and (l + 1, 0) windows of every shell packed into one complex field per shell (J <= 9):
  * the window of the even degree is real and even in k and that of the odd degree real and odd, so

@@ -330,6 +330,79 @@ void oracle_radial_multipole_window(int L, const double *P, double R, int l, int
     }
 }
 
+/* radial_multipole_windows.py:17-126 closed-form radial profiles (code 0 shell j_l, 1 gaussian_shell,
+ * 2 thick_shell, 3 sphere, 4 gaussian; the last four are monopoles) and :186-210 table lookup. */
+static double profile_sinc(double q) { return q == 0.0 ? 1.0 : sin(q) / q; }
+static double profile_sphere(double k, double R) {
+    double q = 2.0 * M_PI * k * R;
+    if (q == 0.0) return 1.0;
+    return 3.0 * (sin(q) - q * cos(q)) / (q * q * q);
+}
+static double radial_profile_eval(double k, int code, double a0, double a1, int l) {
+    if (code == 0) return oracle_spherical_jn(l, 2.0 * M_PI * k * a0);
+    if (code == 1) {
+        if (k == 0.0) return 1.0;
+        double qs = 2.0 * M_PI * k * a0, qm = 2.0 * M_PI * k * a1, den = a0 * a0 + a1 * a1;
+        if (den == 0.0) return 1.0;
+        return (a1 * a1 * cos(qs) + a0 * a0 * profile_sinc(qs)) / den * exp(-0.5 * qm * qm);
+    }
+    if (code == 2) {
+        double ri = a0 - 0.5 * a1, ro = a0 + 0.5 * a1;
+        if (ri < 0.0) ri = 0.0;
+        double den = ro * ro * ro - ri * ri * ri;
+        if (den == 0.0) return 1.0;
+        return (ro * ro * ro * profile_sphere(k, ro) - ri * ri * ri * profile_sphere(k, ri)) / den;
+    }
+    if (code == 3) return profile_sphere(k, a0);
+    if (code == 4) {
+        double q = 2.0 * M_PI * k * a0;
+        return exp(-0.5 * q * q);
+    }
+    return NAN;
+}
+static double table_lookup(double k, const double *table, int n, double k_max) {
+    if (k <= 0.0) return table[0];
+    if (k >= k_max) return table[n - 1];
+    double index = k * (n - 1) / k_max;
+    int lo = (int)index;
+    double f = index - lo;
+    if (lo == 0 || lo >= n - 2) return table[lo] * (1.0 - f) + table[lo + 1] * f;
+    double p0 = table[lo - 1], p1 = table[lo], p2 = table[lo + 1], p3 = table[lo + 2];
+    double f2 = f * f, f3 = f2 * f;
+    return 0.5 * (2.0 * p1 + (-p0 + p2) * f + (2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3) * f2 +
+                  (-p0 + 3.0 * p1 - 3.0 * p2 + p3) * f3);
+}
+
+/* radial_multipole_windows.py:147-183 (closed-form profile, code as above) when table == NULL, else
+ * :214-264 (U_l from the table over physical k in [0, k_max], k_phys = k_grid * g2p) -> (L, L, L) complex128. */
+void oracle_profile_multipole_window(int L, const double *P, int code, double a0, double a1, const double *table,
+                                     int n_table, double k_max, double g2p, int l, int m, double *out) {
+    const double inv_L = 1.0 / L;
+#pragma omp parallel for schedule(static)
+    for (int x = 0; x < L; ++x) {
+        double pis[2] = {P[L - x], P[x]}, kis[2] = {(x - L) * inv_L, x * inv_L};
+        for (int y = 0; y < L; ++y) {
+            double pjs[2] = {P[L - y], P[y]}, kjs[2] = {(y - L) * inv_L, y * inv_L};
+            for (int z = 0; z < L; ++z) {
+                double pks[2] = {P[L - z], P[z]}, kks[2] = {(z - L) * inv_L, z * inv_L};
+                cplx t = c_make(0.0, 0.0);
+                for (int a = 0; a < 8; ++a) {
+                    int u = (a >> 2) & 1, v = (a >> 1) & 1, w = a & 1;
+                    double ki = kis[u], kj = kjs[v], kk = kks[w];
+                    double k = sqrt(ki * ki + kj * kj + kk * kk);
+                    double rad = table ? table_lookup(k * g2p, table, n_table, k_max)
+                                       : radial_profile_eval(k, code, a0, a1, l);
+                    cplx ang = oracle_sph_harm_c(l, m, ki, kj, kk);
+                    t = c_add(t, c_scale(pis[u] * pjs[v] * pks[w] * rad, ang));
+                }
+                size_t o = ((size_t)x * L + y) * L + z;
+                out[2 * o] = t.re;
+                out[2 * o + 1] = t.im;
+            }
+        }
+    }
+}
+
 /* Pair product of a real field with the real, even window Re K_{l,0} of radial_multipole_window (even l),
  * from the field's rFFT half spectrum Fh (L, L, L/2+1) interleaved complex128: the full-grid sum
  * sum_k Re K(k) |F(k)|^2, with |F(-k)| = |F(k)| and K(-k) = K(k) folding it onto the half grid (planes

@@ -42,6 +42,9 @@ def lib():
                                       ctypes.c_double, _dp, _dp]
         L.oracle_radial_multipole_window.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int,
                                                      ctypes.c_int, _dp]
+        L.oracle_profile_multipole_window.argtypes = [ctypes.c_int, _dp, ctypes.c_int, ctypes.c_double,
+                                                      ctypes.c_double, _dp, ctypes.c_int, ctypes.c_double,
+                                                      ctypes.c_double, ctypes.c_int, ctypes.c_int, _dp]
         L.oracle_pair_half_multipole.restype = ctypes.c_double
         L.oracle_pair_half_multipole.argtypes = [ctypes.c_int, _dp, ctypes.c_double, ctypes.c_int, _dp]
         L.oracle_summand.argtypes = [_dp, _dp, _dp, _dp, ctypes.c_int, ctypes.c_int, _dp, _dp]

@@ -47,6 +47,12 @@ _SIGNATURES = {
 }
 
 
+class RadialProfile(ctypes.Structure):
+    """hermes_radial_profile (include/hermes_amd.h)."""
+    _fields_ = [("kind", ctypes.c_int32), ("n_table", ctypes.c_int32), ("a0", ctypes.c_double),
+                ("a1", ctypes.c_double), ("k_scale", ctypes.c_double), ("table", ctypes.c_void_p)]
+
+
 class WindowDesc(ctypes.Structure):
     """hermes_window_desc (include/hermes_amd.h)."""
     _fields_ = [("type", ctypes.c_int32), ("l", ctypes.c_int32), ("m", ctypes.c_int32), ("axis", ctypes.c_int32),
@@ -60,6 +66,10 @@ _SIGNATURES.update({
     "hermes_spectrum_mul": [c_void_p, c_int, c_int, c_int, c_void_p, c_void_p, c_int, c_double],
     "hermes_pair_reduce": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(WindowDesc), c_int,
                            _dpp],
+    "hermes_window_build_full_profile": [c_void_p, c_int, c_void_p, ctypes.POINTER(RadialProfile), c_int, c_int,
+                                         c_int, c_void_p],
+    "hermes_multipole_window_ifft_profiles_f32": [c_void_p, c_int, c_void_p, c_void_p, ctypes.POINTER(RadialProfile),
+                                                  c_int, c_int, ctypes.POINTER(c_int), c_int, c_double, c_void_p],
     "hermes_pair_reduce_noalloc": [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, ctypes.POINTER(WindowDesc),
                                    c_int, _dpp],
     "hermes_multipole_apply_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_int, c_double,

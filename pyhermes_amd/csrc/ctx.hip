@@ -78,7 +78,7 @@ int hermes_release_workspaces(hermes_ctx* ctx) {
     HERMES_REQUIRE(!ctx->capturing, "hermes_release_workspaces: the stream is being captured");
     HERMES_CHECK_HIP(hipSetDevice(ctx->device));
     HERMES_CHECK_HIP(hipDeviceSynchronize());  // the slots may be in use on any stream
-    for (int i = 0; i < 8; ++i) {
+    for (int i = 0; i < HERMES_WS_SLOTS; ++i) {
         if (ctx->ws[i]) HERMES_CHECK_HIP(hipFree(ctx->ws[i]));
         ctx->ws[i] = nullptr;
         ctx->ws_size[i] = 0;
@@ -105,7 +105,7 @@ int hermes_ctx_destroy(hermes_ctx* ctx) {
     if (!ctx) return 0;
     (void)hipSetDevice(ctx->device);
     (void)hipStreamSynchronize(ctx->stream);
-    for (int i = 0; i < 8; ++i)
+    for (int i = 0; i < HERMES_WS_SLOTS; ++i)
         if (ctx->ws[i]) (void)hipFree(ctx->ws[i]);
     hermes_fft_cache_destroy(ctx->fft);
     if (ctx->twiddles) (void)hipFree(ctx->twiddles);

@@ -16,8 +16,8 @@ struct hermes_ctx {
     int device = 0;
     hipStream_t stream = nullptr;  // default stream unless the caller installs one
     // grow-only scratch buffers owned by the context
-    void* ws[8] = {nullptr};
-    size_t ws_size[8] = {0};
+    void* ws[9] = {nullptr};
+    size_t ws_size[9] = {0};
     hermes_fft_cache* fft = nullptr;
     float2* twiddles = nullptr;  // e^{+-2 pi i k / 2^j} tables for the LDS FFTs (first use)
     int fft_cols_variant[11][2] = {};  // autotuned strided-pass tile per (log2 L, inverse); 0 = not tuned
@@ -44,7 +44,9 @@ enum hermes_ws_slot {
     WS_MC = 5,
     WS_MLP = 6,
     WS_CONTRACT = 7,
+    WS_PROFILE = 8,     // radial profiles of the fused window kernel
 };
+constexpr int HERMES_WS_SLOTS = 9;
 
 void hermes_set_error(const char* fmt, ...);
 // Fill n 32-bit words with v on the stream (a kernel, so it can be captured into graphs).

@@ -267,6 +267,30 @@ __device__ inline double iso_window_f64(int type, double ki, double kj, double k
     return __builtin_nan("");
 }
 
+// U_l(k) of a radial profile at k cycles per grid cell (hermes_radial_profile; FP64).
+__device__ inline double radial_table_f64(const double* __restrict__ tab, int n, double x) {
+    // x = k * k_scale, the fractional table index; radial_multipole_windows.py:186-210 lookup
+    if (x <= 0.0) return tab[0];
+    if (x >= (double)(n - 1)) return tab[n - 1];
+    const int i = (int)x;
+    const double t = x - i;
+    if (i == 0 || i >= n - 2) return tab[i] * (1.0 - t) + tab[i + 1] * t;
+    const double p0 = tab[i - 1], p1 = tab[i], p2 = tab[i + 1], p3 = tab[i + 2];
+    return p1 + 0.5 * t * ((p2 - p0) + t * ((2.0 * p0 - 5.0 * p1 + 4.0 * p2 - p3) + t * (3.0 * (p1 - p2) + p3 - p0)));
+}
+
+__device__ inline double radial_profile_f64(const hermes_radial_profile& p, int l, double k) {
+    switch (p.kind) {
+    case HERMES_RADIAL_SHELL: return jl_f64(l, 2.0 * HERMES_PI * k * p.a0);
+    case HERMES_RADIAL_TABLE: return radial_table_f64(p.table, p.n_table, k * p.k_scale);
+    case HERMES_RADIAL_SPHERE: return iso_window_f64(HERMES_WIN_SPHERE, k, 0.0, 0.0, p.a0, p.a1);
+    case HERMES_RADIAL_GAUSSIAN: return iso_window_f64(HERMES_WIN_GAUSSIAN, k, 0.0, 0.0, p.a0, p.a1);
+    case HERMES_RADIAL_THICK_SHELL: return iso_window_f64(HERMES_WIN_THICK_SHELL, k, 0.0, 0.0, p.a0, p.a1);
+    case HERMES_RADIAL_GAUSSIAN_SHELL: return iso_window_f64(HERMES_WIN_GAUSSIAN_SHELL, k, 0.0, 0.0, p.a0, p.a1);
+    }
+    return __builtin_nan("");
+}
+
 // Complex transfer function of a window at signed k (isotropic windows are real).  general = false
 // compiles out the non-axis line-of-sight branch (for callers that never see axis-3 descriptors).
 __device__ inline double2 window_value_f64(const hermes_window_desc& d, double ki, double kj, double kk,

@@ -7,6 +7,7 @@
 #include "hermes_common.h"
 #include "special.h"
 
+#include <cmath>
 #include <mutex>
 
 namespace {
@@ -76,6 +77,34 @@ __global__ void k_build_full(int L, const double* __restrict__ P, hermes_window_
             }
         }
         if (d.type == HERMES_WIN_INVERSE_LAPLACIAN && i == 0) re = im = 0.0;
+        out[2 * i] = (T)re;
+        out[2 * i + 1] = (T)im;
+    }
+}
+
+// radial_multipole_windows.py:147-183 and :186-264: K[x,y,z] = sum over the 8 signed aliases of
+// P P P U_l(|k|) Y_lm(k-hat), U_l from a radial profile (closed form or table).
+template <typename T>
+__global__ void k_build_full_profile(int L, const double* __restrict__ P, hermes_radial_profile prof, int l, int m,
+                                     T* __restrict__ out) {
+    const int64_t n = (int64_t)L * L * L;
+    const double inv_L = 1.0 / L;
+    for (int64_t i = blockIdx.x * (int64_t)blockDim.x + threadIdx.x; i < n; i += (int64_t)gridDim.x * blockDim.x) {
+        const int z = (int)(i % L), y = (int)((i / L) % L), x = (int)(i / ((int64_t)L * L));
+        const double pi[2] = {P[L - x], P[x]}, pj[2] = {P[L - y], P[y]}, pk[2] = {P[L - z], P[z]};
+        const double ki[2] = {(x - L) * inv_L, x * inv_L}, kj[2] = {(y - L) * inv_L, y * inv_L},
+                     kk[2] = {(z - L) * inv_L, z * inv_L};
+        double re = 0.0, im = 0.0;
+#pragma unroll
+        for (int a = 0; a < 8; ++a) {
+            const int u = (a >> 2) & 1, v = (a >> 1) & 1, w = a & 1;
+            const double p = pi[u] * pj[v] * pk[w];
+            const double k = sqrt(ki[u] * ki[u] + kj[v] * kj[v] + kk[w] * kk[w]);
+            const double rad = radial_profile_f64(prof, l, k);
+            const double2 yv = ylm_f64(l, m, ki[u], kj[v], kk[w]);
+            re += p * rad * yv.x;
+            im += p * rad * yv.y;
+        }
         out[2 * i] = (T)re;
         out[2 * i + 1] = (T)im;
     }
@@ -550,11 +579,14 @@ constexpr int mpwin_threads() {
 // under k -> -k, so the field of the even degree is real and that of the odd one imaginary: their
 // windows are summed into one set of four lines (one field per shell, half the transforms), and the
 // real and imaginary products of the Gram separate them again.
-template <int LOG2L, int MP, bool PACK = false>
+// PROF: the radial factor of shell s comes from the radial profile prof[s] (thin shell, table or closed-form
+// monopole) instead of j_l(2 pi |a| R_s / L); prof is unused otherwise.
+template <int LOG2L, int MP, bool PACK = false, bool PROF = false>
 __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
     const float2* __restrict__ H, const float* __restrict__ Pf, ApplyArgs A, MultiM M,
-    const float2* __restrict__ twg, float2* __restrict__ out) {
+    const float2* __restrict__ twg, float2* __restrict__ out, const hermes_radial_profile* __restrict__ prof) {
     static_assert(!PACK || MP == 2, "packing needs two orders");
+    static_assert(!(PACK && PROF), "packed pairs take thin shells only");
     constexpr int L = 1 << LOG2L, Lh = L / 2 + 1;
     constexpr int NT = mpwin_threads<LOG2L>();
     constexpr bool NYQ = Lh > NT;  // z = L/2 by helper lanes 0..7
@@ -653,18 +685,25 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
             lds[F::addr(3, zm)] = cmulf(conjf2(hh[0]), mix(sz0 * sx0, k0, sz1 * sx1, k1));
         }
     };
-    // radial factor of alias argument t for each order (one l, or l and l + 1 when packed)
-    auto radial = [&](double t, float* j) {
+    // radial factor of an alias of norm nrm (grid points) for each order (one l, or l and l + 1 when packed)
+    hermes_radial_profile pr{};
+    auto radial = [&](double nrm_a, double rs, float* j) {
         if (PACK) {
-            jl_pair_f32(M.l[0], t, j[0], j[MP - 1]);
+            jl_pair_f32(M.l[0], nrm_a * rs, j[0], j[MP - 1]);
         } else {
-            const float jj = jl_f32(M.l[0], t);
+            float jj;
+            if (PROF)
+                jj = pr.kind == HERMES_RADIAL_SHELL ? jl_f32(M.l[0], nrm_a * (pr.a0 / L))
+                                                    : (float)radial_profile_f64(pr, M.l[0], nrm_a / L);
+            else
+                jj = jl_f32(M.l[0], nrm_a * rs);
 #pragma unroll
             for (int mi = 0; mi < MP; ++mi) j[mi] = jj;
         }
     };
     for (int s = 0; s < A.n_shell; ++s) {
         const double rs = A.radii_over_L[s];
+        if (PROF) pr = prof[s];
         if (zact) {
             float2 K[MP];
 #pragma unroll
@@ -673,7 +712,7 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
             for (int a = 0; a < 8; ++a) {
                 if (!((live >> a) & 1u)) continue;
                 float j[MP];
-                radial(nrm[a] * rs, j);
+                radial(nrm[a], rs, j);
 #pragma unroll
                 for (int mi = 0; mi < MP; ++mi) {
                     K[mi].x += j[mi] * wy[a][mi].x;
@@ -691,7 +730,7 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
             if (tid < 8) {
                 float j[MP];
                 if (live_n) {
-                    radial(nrm_n * rs, j);
+                    radial(nrm_n, rs, j);
                 } else {
 #pragma unroll
                     for (int mi = 0; mi < MP; ++mi) j[mi] = 0.0f;
@@ -851,6 +890,37 @@ int hermes_window_build_full(hermes_ctx* ctx, int J, const double* P, const herm
     int64_t n = (int64_t)L * L * L;
     if (dbl) k_build_full<double><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *desc, (double*)out);
     else k_build_full<float><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *desc, (float*)out);
+    HERMES_LAUNCH_CHECK();
+    return 0;
+}
+
+static int check_profile(const hermes_radial_profile& p, int l, const char* who) {
+    HERMES_REQUIRE(p.kind >= HERMES_RADIAL_SHELL && p.kind <= HERMES_RADIAL_GAUSSIAN_SHELL, "%s: unknown radial kind %d",
+                   who, p.kind);
+    HERMES_REQUIRE(std::isfinite(p.a0) && std::isfinite(p.a1) && p.a0 >= 0.0, "%s: invalid profile lengths", who);
+    HERMES_REQUIRE(p.kind <= HERMES_RADIAL_TABLE || l == 0,
+                   "%s: closed-form profile kinds are monopoles (l = 0), got l = %d", who, l);
+    if (p.kind == HERMES_RADIAL_TABLE)
+        HERMES_REQUIRE(p.table && p.n_table >= 4 && std::isfinite(p.k_scale) && p.k_scale > 0.0,
+                       "%s: a table profile needs a device table of >= 4 values and k_scale > 0", who);
+    return 0;
+}
+
+int hermes_window_build_full_profile(hermes_ctx* ctx, int J, const double* P, const hermes_radial_profile* prof, int l,
+                                     int m, int dbl, void* out) {
+    HERMES_REQUIRE(ctx && P && prof && out, "hermes_window_build_full_profile: null argument");
+    HERMES_REQUIRE(J >= 1 && J <= 10, "J=%d unsupported", J);
+    HERMES_REQUIRE(l >= 0 && l <= 40 && m >= -l && m <= l, "invalid multipole (l=%d, m=%d)", l, m);
+    if (int rc = check_profile(*prof, l, "hermes_window_build_full_profile")) return rc;
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const int L = 1 << J;
+    const int64_t n = (int64_t)L * L * L;
+    if (dbl)
+        k_build_full_profile<double><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *prof, l, m,
+                                                                                              (double*)out);
+    else
+        k_build_full_profile<float><<<hermes_blocks(n, 256, 65536), 256, 0, ctx->stream>>>(L, P, *prof, l, m,
+                                                                                             (float*)out);
     HERMES_LAUNCH_CHECK();
     return 0;
 }
@@ -1224,16 +1294,50 @@ int hermes_multipole_window_ifft_f32(hermes_ctx* ctx, int J, const void* H, cons
     return hermes_multipole_window_ifft_multi_f32(ctx, J, H, P, radii, n_shell, l, &m, 1, scale, out);
 }
 
+// The window + z-pass launch of n_m orders of one l: thin shells at radii (prof_dev = nullptr), or per shell
+// the device radial profiles prof_dev (radii unused).
+static int mpwin_zpass(hermes_ctx* ctx, int J, const void* H, const double* P, const double* radii,
+                       const hermes_radial_profile* prof_dev, int n_shell, int l, const int* ms_host, int n_m,
+                       double scale, void* out);
+
 int hermes_multipole_window_zpass_multi_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
                                             const double* radii, int n_shell, int l, const int* ms_host, int n_m,
                                             double scale, void* out) {
-    HERMES_REQUIRE(ctx && H && P && radii && ms_host && out, "hermes_multipole_window_zpass_multi_f32: null argument");
+    HERMES_REQUIRE(radii, "hermes_multipole_window_zpass_multi_f32: null argument");
+    return mpwin_zpass(ctx, J, H, P, radii, nullptr, n_shell, l, ms_host, n_m, scale, out);
+}
+
+int hermes_multipole_window_ifft_profiles_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
+                                              const hermes_radial_profile* profs_host, int n_shell, int l,
+                                              const int* ms_host, int n_m, double scale, void* out) {
+    HERMES_REQUIRE(ctx && profs_host, "hermes_multipole_window_ifft_profiles_f32: null argument");
+    HERMES_REQUIRE(!ctx->capturing, "hermes_multipole_window_ifft_profiles_f32: not capturable");
+    HERMES_REQUIRE(n_shell >= 1 && n_shell <= MAX_SHELLS, "n_shell=%d outside 1..%d", n_shell, MAX_SHELLS);
+    for (int s = 0; s < n_shell; ++s)
+        if (int rc = check_profile(profs_host[s], l, "hermes_multipole_window_ifft_profiles_f32")) return rc;
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    auto* dev = (hermes_radial_profile*)hermes_workspace(ctx, WS_PROFILE, sizeof(hermes_radial_profile) * n_shell);
+    HERMES_REQUIRE(dev, "%s", hermes_last_error());
+    HERMES_CHECK_HIP(hipMemcpyAsync(dev, profs_host, sizeof(hermes_radial_profile) * n_shell, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    const int rc = mpwin_zpass(ctx, J, H, P, nullptr, dev, n_shell, l, ms_host, n_m, scale, out);
+    if (rc) return rc;
+    // the pageable source of the copy must outlive it: wait before returning to the caller
+    HERMES_CHECK_HIP(hipStreamSynchronize(ctx->stream));
+    return hermes_fft3_inverse_yx(ctx, J, n_m * n_shell, (float2*)out);
+}
+
+static int mpwin_zpass(hermes_ctx* ctx, int J, const void* H, const double* P, const double* radii,
+                       const hermes_radial_profile* prof_dev, int n_shell, int l, const int* ms_host, int n_m,
+                       double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && ms_host && out, "hermes_multipole_window_zpass_multi_f32: null argument");
     HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_multipole_window_zpass_multi_f32: J=%d outside 3..10", J);
     HERMES_REQUIRE(n_m >= 1 && n_m <= MAX_MP, "hermes_multipole_window_zpass_multi_f32: n_m=%d outside 1..%d", n_m,
                    MAX_MP);
     ApplyArgs A;
     float* Pf = nullptr;
-    int rc = make_apply_args(ctx, J, P, radii, n_shell, l, ms_host[0], scale, &A, &Pf);
+    const double zeros[MAX_SHELLS] = {0.0};
+    int rc = make_apply_args(ctx, J, P, radii ? radii : zeros, n_shell, l, ms_host[0], scale, &A, &Pf);
     if (rc) return rc;
     MultiM M{};
     M.n = n_m;
@@ -1251,11 +1355,17 @@ int hermes_multipole_window_zpass_multi_f32(hermes_ctx* ctx, int J, const void* 
     const unsigned nblk = (unsigned)(((1 << J) / 2 + 1) * ((1 << J) / 2 + 1));
     const float2* h = (const float2*)H;
     float2* o = (float2*)out;
-#define HERMES_MPW(LG)                                                                                   \
-    if (n_m == 1)                                                                                        \
-        k_mpwin_zfft_multi<LG, 1><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o);  \
-    else                                                                                                 \
-        k_mpwin_zfft_multi<LG, 2><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o);
+#define HERMES_MPW(LG)                                                                                         \
+    if (prof_dev && n_m == 1)                                                                                  \
+        k_mpwin_zfft_multi<LG, 1, false, true><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o, \
+                                                                                              prof_dev);       \
+    else if (prof_dev)                                                                                         \
+        k_mpwin_zfft_multi<LG, 2, false, true><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o, \
+                                                                                              prof_dev);       \
+    else if (n_m == 1)                                                                                         \
+        k_mpwin_zfft_multi<LG, 1><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o, nullptr);   \
+    else                                                                                                       \
+        k_mpwin_zfft_multi<LG, 2><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o, nullptr);
     switch (J) {
         case 3: HERMES_MPW(3) break;
         case 4: HERMES_MPW(4) break;
@@ -1274,8 +1384,12 @@ int hermes_multipole_window_zpass_multi_f32(hermes_ctx* ctx, int J, const void* 
                 Mi.cnorm[0] = M.cnorm[i];
                 Mi.sx[0] = M.sx[i];
                 Mi.sz[0] = M.sz[i];
-                k_mpwin_zfft_multi<10, 1><<<nblk, mpwin_threads<10>(), 0, ctx->stream>>>(
-                    h, Pf, A, Mi, twg, o + (size_t)i * n_shell * ((size_t)1 << 30));
+                if (prof_dev)
+                    k_mpwin_zfft_multi<10, 1, false, true><<<nblk, mpwin_threads<10>(), 0, ctx->stream>>>(
+                        h, Pf, A, Mi, twg, o + (size_t)i * n_shell * ((size_t)1 << 30), prof_dev);
+                else
+                    k_mpwin_zfft_multi<10, 1><<<nblk, mpwin_threads<10>(), 0, ctx->stream>>>(
+                        h, Pf, A, Mi, twg, o + (size_t)i * n_shell * ((size_t)1 << 30), nullptr);
                 HERMES_LAUNCH_CHECK();
             }
             break;
@@ -1308,7 +1422,8 @@ int hermes_multipole_window_ifft_pack0_f32(hermes_ctx* ctx, int J, const void* H
     const unsigned nblk = (unsigned)(((1 << J) / 2 + 1) * ((1 << J) / 2 + 1));
     const float2* h = (const float2*)H;
     float2* o = (float2*)out;
-#define HERMES_MPWP(LG) k_mpwin_zfft_multi<LG, 2, true><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o);
+#define HERMES_MPWP(LG) \
+    k_mpwin_zfft_multi<LG, 2, true><<<nblk, mpwin_threads<LG>(), 0, ctx->stream>>>(h, Pf, A, M, twg, o, nullptr);
     switch (J) {
         case 3: HERMES_MPWP(3) break;
         case 4: HERMES_MPWP(4) break;

@@ -5,10 +5,14 @@ inverse_laplacian) and the line-of-sight windows (ring, disk, cylshell, cylinder
 window_functions.py:164-319) are built by hermes_window_build_rfft in the rFFT
 layout (window.py:274-332); thin-shell ``radial_multipole`` and
 ``legendre_multipole`` windows by hermes_window_build_full in the complex
-full-FFT layout (window.py:197-264).  A user ``func`` is evaluated on the host
-over the same alias sums and then used on the device.  Line-of-sight windows take
-a line of sight along a coordinate axis (any sign); other directions, and the
-derivative and tabulated-profile windows of the reference, raise NotImplementedError.
+full-FFT layout (window.py:197-264); ``radial_multipole`` windows of the other radial
+profiles (sphere, thick_shell, gaussian, gaussian_shell, custom real/k-space profiles)
+by hermes_window_build_full_profile from a closed-form monopole or a host-built Hankel
+table (radial_multipole_windows.py:267-316).  A user ``func`` is evaluated on the host
+over the same alias sums and then used on the device.  Line-of-sight windows take any
+line of sight; a non-axis one needs kernel_mode 'auto' or 'full_rfft' (the octant
+fold assumes a window even in each k component).  The derivative windows of the
+reference raise NotImplementedError.
 """
 
 import copy
@@ -42,16 +46,20 @@ def _iso_desc(wtype, rescaled):
 
 
 def los_axis(los_args):
-    """(axis, unit vector) of a line of sight {nx, ny, nz}: axis 0 x, 1 y, 2 z when it lies along one
-    coordinate axis (either sign; the windows are even in k), else 3 with the normalised direction, as
-    the reference divides k . n by |n| (convolution.py:73-157)."""
+    """(axis, unit vector) of a line of sight {nx, ny, nz}: axis 0 x, 1 y, 2 z when |n| / norm is within
+    numpy.isclose's default tolerances of a coordinate axis (either sign; the windows are even in k) -- the
+    reference's rule for an axis-aligned line of sight (window.py:150-170) --, else 3 with the normalised
+    direction, as the reference divides k . n by |n| (convolution.py:73-157)."""
     vec = np.array([los_args["nx"], los_args["ny"], los_args["nz"]], dtype=np.float64)
     norm = float(np.linalg.norm(vec))
     if not np.isfinite(norm) or norm == 0.0:
         raise ValueError(f"line-of-sight vector must be finite and non-zero, got {tuple(vec)}")
-    nz = np.nonzero(vec)[0]
-    if nz.size == 1:
-        return int(nz[0]), None
+    unit = np.abs(vec / norm)
+    for axis in range(3):
+        target = np.zeros(3)
+        target[axis] = 1.0
+        if np.all(np.isclose(unit, target)):
+            return axis, None
     return 3, vec / norm
 
 
@@ -157,12 +165,26 @@ class WindowFunc(SFCField):
             raise ValueError("thick_shell requires a finite positive delta_R.")
 
     # ------------------------------------------------------------ kernel construction
+    def _radial_multipole_kernel(self, P, dbl):
+        """window.py:212-238: radial_multipole windows of any radial profile -- closed form for thin shells and
+        built-in monopoles, else the profile's Hankel table (profile_config = other_args['profile_config'])."""
+        from ..edges import EdgeProfile
+        try:
+            radial_type = str(self.other_args["radial_type"])
+            l, m = int(self.other_args["l"]), int(self.other_args["m"])
+        except KeyError as exc:
+            self.logger.error(f"radial_multipole missing required other_args key: {exc}")
+            func_util.safe_exit(1)
+        try:
+            edge = EdgeProfile(radial_type, self.len_args, self.other_args.get("profile_config", {}), self.L,
+                               self.box_size)
+            return edge.window_full(self.J, P, l, m, dbl=dbl)
+        except (TypeError, ValueError) as exc:
+            self.logger.error(f"Failed to build radial_multipole window: {exc}")
+            func_util.safe_exit(1)
+
     def _multipole_desc(self):
         oa = self.other_args
-        if self.type == "radial_multipole":
-            radial_type = str(oa.get("radial_type", "")).strip().lower()
-            if radial_type != "shell":
-                raise NotImplementedError("pyhermes_amd builds radial_multipole windows for thin shells only.")
         if "R" not in self.rescale_len_args:
             self.logger.error(f"{self.type} requires len_args={{'R': ...}}.")
             func_util.safe_exit(1)
@@ -227,7 +249,10 @@ class WindowFunc(SFCField):
                     self.logger.error(f"Window type '{self.type}' requires kernel_mode='complex_full_fft' or "
                                       "kernel_mode='auto'.")
                     func_util.safe_exit(1)
-                t = kernels.window_build_full(self.J, P, self._multipole_desc(), dbl=dbl)
+                if self.type == "radial_multipole":
+                    t = self._radial_multipole_kernel(P, dbl)
+                else:
+                    t = kernels.window_build_full(self.J, P, self._multipole_desc(), dbl=dbl)
             elif self.type in LOS_TYPES:
                 # even in every component of k for an axis line of sight: the rFFT kernel of the reference's
                 # "auto" (full_rfft) mode and the octant fold sum the same alias values.  Along any other

@@ -177,6 +177,41 @@ def window_build_full(J, P, desc, dbl=True):
     return out
 
 
+RADIAL_KINDS = {"shell": 0, "table": 1, "sphere": 2, "gaussian": 3, "thick_shell": 4, "gaussian_shell": 5}
+
+
+def radial_profile(kind, a0=0.0, a1=0.0, table=None, k_scale=0.0):
+    """A hermes_radial_profile: lengths a0, a1 in grid units; for kind 'table' a device float64 tensor of U_l
+    samples with k_scale samples per cycle per grid cell (the tensor must outlive every use)."""
+    if kind not in RADIAL_KINDS:
+        raise ValueError(f"unknown radial profile kind {kind!r}")
+    if kind == "table" and (table is None or table.dtype != torch.float64 or not table.is_contiguous()):
+        raise ValueError("a table profile needs a contiguous float64 device tensor")
+    return _lib.RadialProfile(RADIAL_KINDS[kind], 0 if table is None else int(table.numel()), float(a0), float(a1),
+                              float(k_scale), None if table is None else int(table.data_ptr()))
+
+
+def window_build_full_profile(J, P, prof, l, m, dbl=True):
+    """(L, L, L) complex kernel U_l(|k|) Y_lm(k-hat) of one radial profile (alias sums as the shell builder)."""
+    L = 1 << J
+    out = torch.empty((L, L, L), dtype=torch.complex128 if dbl else torch.complex64, device=P.device)
+    _lib.call("hermes_window_build_full_profile", _lib.context(), int(J), ptr(P), ctypes_byref(prof), int(l), int(m),
+              1 if dbl else 0, ptr(out))
+    return out
+
+
+def multipole_window_ifft_profiles_f32(H, P, profiles, l, ms, scale, out):
+    """multipole_window_ifft_multi_f32 with a radial profile per shell (``profiles``: RadialProfile list)."""
+    S = len(profiles)
+    ms = [int(m) for m in ms]
+    L = _check_multipole_inputs(H, P, len(ms) * S, out)
+    arr = (ctypes.c_int * len(ms))(*ms)
+    profs = (_lib.RadialProfile * S)(*profiles)
+    _lib.call("hermes_multipole_window_ifft_profiles_f32", _lib.context(), _J_of(L), ptr(H), ptr(P), profs, S,
+              int(l), arr, len(ms), float(scale), ptr(out))
+    return out
+
+
 def spectrum_mul(spec, kernel, scale=1.0):
     """spec *= kernel * scale in place (half spectrum with a real kernel, or full with a complex one)."""
     L = spec.shape[0]

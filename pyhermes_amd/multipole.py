@@ -21,6 +21,7 @@ import numpy as np
 import torch
 
 from . import kernels
+from .edges import EdgeProfile
 from .utils.wavelet_grid import cal_gamma, fourier_power_spectrum
 
 
@@ -48,9 +49,13 @@ def combine_m_terms(v_re_by_m, l):
 
 
 # ---------------------------------------------------------------- plain FP64
-def _multipole_field_f64(geo, delta_c, R_grid, l, m):
-    """field @ radial_multipole window (complex_full_fft), FP64: fftn -> x K -> ifftn."""
-    K = kernels.window_build_full(geo.J, geo.P, kernels.window_desc("shell_multipole", R_grid, l=l, m=m), dbl=True)
+def _multipole_field_f64(geo, delta_c, edge, l, m):
+    """field @ radial_multipole window (complex_full_fft), FP64: fftn -> x K -> ifftn.  ``edge``: a thin-shell
+    radius in grid units or an EdgeProfile."""
+    if isinstance(edge, EdgeProfile):
+        K = edge.window_full(geo.J, geo.P, l, m, dbl=True)
+    else:
+        K = kernels.window_build_full(geo.J, geo.P, kernels.window_desc("shell_multipole", edge, l=l, m=m), dbl=True)
     spec = kernels.fftn_c2c(delta_c)
     kernels.spectrum_mul(spec, K, 1.0 / geo.L ** 3)
     del K
@@ -117,38 +122,68 @@ def _shell_blocks(n, size=16):
     return [(i, min(n, i + size)) for i in range(0, n, size)]
 
 
+def _edges(values):
+    """Thin-shell radii (grid units, float64 array) -- also when every EdgeProfile is a thin shell -- or the
+    list of EdgeProfile for any other radial profile."""
+    if isinstance(values, np.ndarray) or not any(isinstance(v, EdgeProfile) for v in values):
+        return np.asarray(values, dtype=np.float64)
+    if not all(isinstance(v, EdgeProfile) for v in values):
+        raise TypeError("edges must be all radii or all EdgeProfile")
+    radii = [v.thin_radius for v in values]
+    return np.asarray(radii, dtype=np.float64) if all(r is not None for r in radii) else list(values)
+
+
+def _window_runner(edges):
+    """win(H, P, l, ms, scale, out): the fused window + inverse FFT of every edge for orders ms of one l."""
+    if isinstance(edges, np.ndarray):
+        return lambda H, P, l, ms, scale, out: kernels.multipole_window_ifft_multi_f32(H, P, edges, l, ms, scale, out)
+    return lambda H, P, l, ms, scale, out: kernels.multipole_window_ifft_profiles_f32(
+        H, P, [e.profile(l) for e in edges], l, ms, scale, out)
+
+
+def _same_edges(a, b):
+    if isinstance(a, np.ndarray) and isinstance(b, np.ndarray):
+        return a.shape == b.shape and np.array_equal(a, b)
+    return not isinstance(a, np.ndarray) and not isinstance(b, np.ndarray) and list(a) == list(b)
+
+
 def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min, l_max, timer=None,
-                              schedule="serial"):
+                              schedule="serial", m_values=None):
     """All (r12, r13) shell pairs: returns (n12, n13, n_l) float64.  d*: (L,L,L) float32 device tensors.
     ``schedule`` picks the stream schedule of the auto case with at most 16 shells (see
-    _ddd_auto_pipelined): "serial", "split" or "overlap"."""
+    _ddd_auto_pipelined): "serial", "split" or "overlap".  A dict passed as ``m_values`` receives
+    Re v_{l,m} (n12 x n13, m >= 0) per (l, m), the terms combine_multipole_m_terms sums."""
     L, J = geo.L, geo.J
     tm = timer if timer is not None else _NoTimer()
     N = L ** 3
-    r12 = np.asarray(radii12_grid, dtype=np.float64)
-    r13 = np.asarray(radii13_grid, dtype=np.float64)
-    auto = (d3 is d2) and r12.shape == r13.shape and np.array_equal(r12, r13)
+    r12 = _edges(radii12_grid)
+    r13 = _edges(radii13_grid)
+    win12, win13 = _window_runner(r12), _window_runner(r13)
+    n12, n13 = len(r12), len(r13)
+    profiled = not isinstance(r12, np.ndarray) or not isinstance(r13, np.ndarray)
+    auto = (d3 is d2) and _same_edges(r12, r13)
     t = tm.start("fft_r2c")
     H2 = kernels.rfftn(d2)
     H3 = H2 if d3 is d2 else kernels.rfftn(d3)
     tm.stop(t)
     n_l = l_max - l_min + 1
-    result = np.zeros((r12.size, r13.size, n_l), dtype=np.float64)
-    if auto and r12.size <= 16:
-        if packs_m0(geo.L, r12.size, schedule):
-            return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm)
-        return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm, schedule=schedule)
-    blocks12 = _shell_blocks(r12.size)
-    blocks13 = _shell_blocks(r13.size)
-    U = torch.empty((r12.size, L, L, L), dtype=torch.complex64, device=d1.device)
-    V = None if auto else torch.empty((r13.size, L, L, L), dtype=torch.complex64, device=d1.device)
+    result = np.zeros((n12, n13, n_l), dtype=np.float64)
+    if auto and n12 <= 16:
+        if not profiled and packs_m0(geo.L, n12, schedule):
+            return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm, m_values)
+        return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm, schedule="serial" if profiled else schedule,
+                                   m_values=m_values, win=win12)
+    blocks12 = _shell_blocks(n12)
+    blocks13 = _shell_blocks(n13)
+    U = torch.empty((n12, L, L, L), dtype=torch.complex64, device=d1.device)
+    V = None if auto else torch.empty((n13, L, L, L), dtype=torch.complex64, device=d1.device)
     for li, l in enumerate(range(l_min, l_max + 1)):
         v_re = []
         for m in range(l + 1):
             t = tm.start("window_ifft")
-            kernels.multipole_window_ifft_f32(H2, geo.P, r12, l, m, 1.0 / N, out=U)
+            win12(H2, geo.P, l, [m], 1.0 / N, U)
             tm.stop(t)
-            G = np.zeros((r12.size, r13.size))
+            G = np.zeros((n12, n13))
             t = tm.start("gram")
             if auto:
                 sign = (-1.0) ** (l + m)
@@ -162,7 +197,7 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
             else:
                 tm.stop(t)
                 t = tm.start("window_ifft")
-                kernels.multipole_window_ifft_f32(H3, geo.P, r13, l, -m, 1.0 / N, out=V)
+                win13(H3, geo.P, l, [-m], 1.0 / N, V)
                 tm.stop(t)
                 t = tm.start("gram")
                 for a0, a1 in blocks12:
@@ -170,6 +205,8 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
                         G[a0:a1, b0:b1] = kernels.multipole_gram_f32(d1, U[a0:a1], geo.Q, geo.lam, V=V[b0:b1])
             tm.stop(t)
             v_re.append(4.0 * np.pi * G / N)
+            if m_values is not None:
+                m_values[(l, m)] = v_re[-1]
         result[:, :, li] = combine_m_terms(v_re, l)
     return result
 
@@ -207,7 +244,7 @@ def _packed_units(l_min, l_max):
     return units
 
 
-def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm):
+def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm, m_values=None):
     """Auto case with 9..16 shells on the serial schedule, the m = 0 terms packed in pairs: the windows of
     (l, 0) and (l + 1, 0) are real with opposite parity under k -> -k, so one transform yields the even
     degree's fields as real parts and the odd degree's as imaginary parts, and one Gram with the real and
@@ -246,6 +283,8 @@ def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm):
             v_re[(odd, 0)] = -4.0 * np.pi * kernels.gram_block_to_matrix(host[b, 256:], S) / N
         else:
             v_re[(l, m)] = 4.0 * np.pi * ((-1.0) ** (l + m)) * kernels.gram_block_to_matrix(host[b, :256], S) / N
+    if m_values is not None:
+        m_values.update(v_re)
     result = np.zeros((S, S, l_max - l_min + 1), dtype=np.float64)
     for li, l in enumerate(range(l_min, l_max + 1)):
         result[:, :, li] = combine_m_terms([v_re[(l, m)] for m in range(l + 1)], l)
@@ -271,7 +310,7 @@ def _window_runs(group):
     return runs
 
 
-def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, schedule="serial"):
+def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, schedule="serial", m_values=None, win=None):
     """Auto case with S <= 16 shells.  Each iteration takes G = max(16 // S, 2) consecutive (l, m) terms:
     window + inverse-FFT launches over runs of one l, two orders at a time (they share the radial factors),
     fill a buffer of G * S shell fields, and Grams over chunks of 16 // S terms follow (the Gram's MFMA tile is
@@ -291,7 +330,11 @@ def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, schedule="serial"):
     split = schedule == "split"
     L = geo.L
     N = L ** 3
-    S = r.size
+    S = len(r)
+    if win is None:
+        win = _window_runner(r)
+    if split and not isinstance(r, np.ndarray):
+        raise ValueError("the split schedule takes thin-shell radii only")
     pack = 16 // S
     G = max(pack, 2)
     lm = [(l, m) for l in range(l_min, l_max + 1) for m in range(l + 1)]
@@ -313,8 +356,10 @@ def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, schedule="serial"):
         t = tm.start("window_z" if split else "window_ifft")
         pos = 0
         for l, ms in _window_runs(group):
-            win = kernels.multipole_window_zpass_multi_f32 if split else kernels.multipole_window_ifft_multi_f32
-            win(H, geo.P, r, l, ms, 1.0 / N, U[pos * S:(pos + len(ms)) * S])
+            if split:
+                kernels.multipole_window_zpass_multi_f32(H, geo.P, r, l, ms, 1.0 / N, U[pos * S:(pos + len(ms)) * S])
+            else:
+                win(H, geo.P, l, ms, 1.0 / N, U[pos * S:(pos + len(ms)) * S])
             pos += len(ms)
         tm.stop(t)
         n_fields = pos * S
@@ -362,6 +407,8 @@ def _ddd_auto_pipelined(geo, d1, H, r, l_min, l_max, tm, schedule="serial"):
         full = kernels.gram_block_to_matrix(host[b], len(chunk) * S)
         for j, (l, m) in enumerate(chunk):
             v_re[(l, m)] = 4.0 * np.pi * ((-1.0) ** (l + m)) * full[j * S:(j + 1) * S, j * S:(j + 1) * S] / N
+    if m_values is not None:
+        m_values.update(v_re)
     result = np.zeros((S, S, l_max - l_min + 1), dtype=np.float64)
     for li, l in enumerate(range(l_min, l_max + 1)):
         result[:, :, li] = combine_m_terms([v_re[(l, m)] for m in range(l + 1)], l)

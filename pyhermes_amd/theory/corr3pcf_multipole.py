@@ -15,7 +15,10 @@ calc_DDD_multipole is replaced by:
 * summation_backend 'cpu' is accepted for compatibility and runs 'gpu_fp64'
   (this package has no CPU path).
 
-Edge binning windows must be thin shells (radial type 'shell').  The reference's
+Edge binning windows take the reference's radial profiles: thin shells (closed
+form), the built-in sphere / thick_shell / gaussian / gaussian_shell profiles and
+custom real- or k-space profiles (Hankel tables, utils/radial_profiles.py, with the
+reference's table diagnostics recorded in corr3pcf_multipole_info).  The reference's
 pair_mpi / sample_mpi modes distribute samples over MPI ranks; one process per
 GPU runs them as 'serial'.
 """
@@ -31,6 +34,8 @@ from .. import kernels, multipole
 from ..io import Corr3PCFMultipoleData, SFCField, WindowFunc, normalize_task_weight_normalization
 from ..pipeline import TaskBase
 from ..utils import func_util
+from ..edges import EdgeProfile
+from ..utils.radial_profiles import diagnose_radial_multipole_table, is_thin_shell, validate_radial_profile_request
 from ..utils.special_functions import solve_multipoles_from_ratio
 
 ALLOWED = {"ddd_l", "rrr_l", "delta_ddd_l", "zeta_l"}
@@ -87,12 +92,10 @@ class Corr_3PCF_Multipole(TaskBase):
             raise ValueError("Corr_3PCF_Multipole requires 0 <= l_min <= l_max.")
         for name, wins in (("binning_window12", self.binning_window12), ("binning_window13", self.binning_window13)):
             for k, w in enumerate(wins):
-                if w["type"] != "shell":
-                    raise NotImplementedError(f"Invalid {name} for sample {k}: pyhermes_amd supports thin 'shell' "
-                                              "edge windows.")
-                if "R" not in w["len_args"]:
-                    raise ValueError(f"Invalid {name} for sample {k}: radial_multipole profile 'shell' requires "
-                                     "len_args['R'].")
+                try:
+                    validate_radial_profile_request(w["type"], w["len_args"], w.get("other_args", {}))
+                except (TypeError, ValueError, KeyError) as exc:
+                    raise ValueError(f"Invalid {name} for sample {k}: {exc}") from exc
         backend = str(tp.get("summation_backend", "gpu")).strip().lower()
         if backend not in ("gpu", "gpu_fp64", "cpu"):
             raise ValueError("summation_backend must be either 'gpu', 'gpu_fp64' or 'cpu'.")
@@ -104,6 +107,14 @@ class Corr_3PCF_Multipole(TaskBase):
         if self.execution_mode not in {"serial", "pair_mpi", "sample_mpi"}:
             raise ValueError("Corr_3PCF_Multipole execution_mode must be 'serial', 'pair_mpi', or 'sample_mpi'.")
         self.zeta_condition_warning = float(tp.get("zeta_condition_warning", 1e12))
+        self.radial_profile_diagnostics_enabled = bool(tp.get("radial_profile_diagnostics", True))
+        self.radial_profile_diagnostic_tolerance = float(tp.get("radial_profile_diagnostic_tolerance", 1.0e-5))
+        if not np.isfinite(self.radial_profile_diagnostic_tolerance) or self.radial_profile_diagnostic_tolerance <= 0:
+            raise ValueError("radial_profile_diagnostic_tolerance must be finite and positive.")
+        self.radial_profile_diagnostic_probes = int(tp.get("radial_profile_diagnostic_probes", 33))
+        if self.radial_profile_diagnostic_probes < 3:
+            raise ValueError("radial_profile_diagnostic_probes must be at least 3.")
+        self.radial_profile_diagnostics = []
         self.threads = int(tp["threads"])
         self.products = self._normalize_products(tp.get("products", "zeta_l"))
         self.fout_path = tp["fout_path"]
@@ -319,18 +330,48 @@ class Corr_3PCF_Multipole(TaskBase):
             tens.append(seen[id(s)])
         return tens
 
+    def _edge_list(self, windows, L, box):
+        """Per sample: the thin-shell radius in grid units, or an EdgeProfile (corr3pcf_multipoles.py:231-252
+        turns each edge window into a radial_multipole window of its profile)."""
+        out, cache = [], {}
+        for w in windows:
+            if w["type"] == "shell":
+                out.append(float(w["len_args"]["R"]) * L / box)
+                continue
+            edge = EdgeProfile(w["type"], w["len_args"], w.get("other_args", {}), L, box)
+            out.append(cache.setdefault(edge.key, edge))
+        return out
+
+    @staticmethod
+    def _unique_edges(edges):
+        """Distinct edges in a stable order and each sample's index into them."""
+        keys, index, uniq = {}, [], []
+        for e in edges:
+            k = e.key if isinstance(e, EdgeProfile) else ("shell", float(e))
+            if k not in keys:
+                keys[k] = len(uniq)
+                uniq.append(e)
+            index.append(keys[k])
+        if all(not isinstance(e, EdgeProfile) for e in uniq):
+            order = np.argsort(np.asarray(uniq, dtype=np.float64), kind="stable")
+            rank = np.empty_like(order)
+            rank[order] = np.arange(order.size)
+            return np.asarray(uniq, dtype=np.float64)[order], rank[np.asarray(index)]
+        return uniq, np.asarray(index)
+
     def _compute(self, product):
         d1, d2, d3 = self._product_tensors(product)
         L = int(self.reference_sfc.L)
         box = float(self.reference_sfc.box_size)
         geo = self._geometry(L)
-        r12 = np.array([float(w["len_args"]["R"]) for w in self.binning_window12]) * L / box
-        r13 = np.array([float(w["len_args"]["R"]) for w in self.binning_window13]) * L / box
+        e12 = self._edge_list(self.binning_window12, L, box)
+        e13 = self._edge_list(self.binning_window13, L, box)
         n_l = self.l_max - self.l_min + 1
         rows = np.empty((len(self.samples), n_l))
         if self.summation_backend == "gpu":
-            u12, i12 = np.unique(r12, return_inverse=True)
-            u13, i13 = np.unique(r13, return_inverse=True)
+            u12, i12 = self._unique_edges(e12)
+            u13, i13 = self._unique_edges(e13)
+            u12, u13 = self._profiles_or_radii(u12, L, box), self._profiles_or_radii(u13, L, box)
             f1 = d1.float()
             f2 = d2.float()
             f3 = f2 if d3 is d2 else d3.float()
@@ -339,8 +380,43 @@ class Corr_3PCF_Multipole(TaskBase):
                 rows[k] = res[i12[k], i13[k]]
         else:
             for k in range(len(self.samples)):
-                rows[k] = multipole.ddd_multipole_plain_f64(geo, d1, d2, d3, r12[k], r13[k], self.l_min, self.l_max)
+                rows[k] = multipole.ddd_multipole_plain_f64(geo, d1, d2, d3, e12[k], e13[k], self.l_min, self.l_max)
         return rows
+
+    @staticmethod
+    def _profiles_or_radii(uniq, L, box):
+        """Radii stay a float64 array; a mixed list becomes all EdgeProfile (thin shells as 'shell')."""
+        if isinstance(uniq, np.ndarray):
+            return uniq
+        return [e if isinstance(e, EdgeProfile) else EdgeProfile("shell", {"R": float(e) * box / L}, {}, L, box)
+                for e in uniq]
+
+    def _record_radial_profile_diagnostics(self):
+        """corr3pcf_multipole.py:668-727: each tabulated edge table checked against a refined quadrature (and
+        its k = 0 value); failures are logged as warnings and kept in the result's info."""
+        self.radial_profile_diagnostics = []
+        if not self.radial_profile_diagnostics_enabled:
+            return
+        L, box = int(self.reference_sfc.L), float(self.reference_sfc.box_size)
+        k_max = float(np.sqrt(3.0) * L / box)
+        for edge_name, windows in (("binning_window12", self.binning_window12),
+                                   ("binning_window13", self.binning_window13)):
+            for k, w in enumerate(windows):
+                t = str(w["type"]).strip().lower()
+                if is_thin_shell(t, w["len_args"]):
+                    continue
+                first = self.l_min if t.startswith("custom_") else max(1, self.l_min)
+                for l in range(first, self.l_max + 1):
+                    d = diagnose_radial_multipole_table(t, w["len_args"], l, k_max,
+                                                        profile_config=w.get("other_args", {}),
+                                                        tolerance=self.radial_profile_diagnostic_tolerance,
+                                                        probe_count=self.radial_profile_diagnostic_probes)
+                    d.update({"edge": edge_name, "sample_idx": int(k)})
+                    self.radial_profile_diagnostics.append(d)
+                    if not d["passed"]:
+                        self.logger.warning(f"Radial profile diagnostic exceeded tolerance | {edge_name}[{k}], "
+                                            f"type={t}, l={l}, tolerance={self.radial_profile_diagnostic_tolerance:.3e}."
+                                            " Increase profile quadrature/table points or verify the profile support.")
 
     def _geometry(self, L):
         if getattr(self, "_geo", None) is None or self._geo.L != L:
@@ -369,6 +445,7 @@ class Corr_3PCF_Multipole(TaskBase):
         if not self._fields_prepared:
             self.prepare_input_fields()
         exp = self._expanded_products()
+        self._record_radial_profile_diagnostics()
         for product in exp:
             if product == "zeta_l":
                 continue
@@ -394,7 +471,9 @@ class Corr_3PCF_Multipole(TaskBase):
                 self.logger.warning(f"zeta_l mixing matrices exceed zeta_condition_warning for samples {bad.tolist()}.")
         data.corr3pcf_multipole_info = {"l_min": self.l_min, "l_max": self.l_max, "n_samples": len(self.samples),
                                         "summation_backend": self.summation_backend,
-                                        "products": list(self.products), "rho_vertices": self.rho_vertices}
+                                        "products": list(self.products), "rho_vertices": self.rho_vertices,
+                                        "radial_profile_diagnostics_result": copy.deepcopy(
+                                            self.radial_profile_diagnostics)}
         if save_result and self.fout_path:
             data.save_corr3pcf_multipole(self.fout_path, overwrite=overwrite)
         self.logger.info(f"The time for task: {time.perf_counter() - t0:.4f} sec")

@@ -31,6 +31,13 @@ def _project(pos, w, J, box, norm="catalog"):
     return task.run(save_result=False)
 
 
+def _project_prec(pos, w, J, box, precision):
+    from pyhermes_amd.base import SFCProjection
+    return SFCProjection({"SFCProjection": {"J": J, "box_size": box, "particle_pos": pos, "catalog_weight": w,
+                                            "weight_normalization": "catalog", "precision": precision,
+                                            "fout_path": ""}}).run(save_result=False)
+
+
 def _oracle_eps(pos, w, J, box, phi, norm="catalog"):
     pw, sums = oproj.projection_weights(w, np.ones_like(w), norm)
     return oproj.project_c(pos, pw, phi, J, box)[0], sums
@@ -99,12 +106,37 @@ def test_corr2pcf_task(phi):
     np.testing.assert_allclose(res.xi, ddd_ref / rr_ref, rtol=1e-9, atol=1e-12)
     # xi_0 from the j_0 Y_00 window equals the isotropic shell xi
     np.testing.assert_allclose(res.xi_l[0], res.xi, rtol=1e-9, atol=1e-12)
-    # xi_2 against the full complex multipole window on the CPU
+    # xi_2 against the oracle's multipole-window route (its prefactor pinned by the ring route on the CPU)
     for j, sv in enumerate(s):
-        K2 = ow.radial_multipole_window(L, P, sv * L / box, 2, 0)
-        pp = np.real(np.sum(ow.convolve_complex(delta, K2) * delta)) / L ** 3 * scale
-        ref2 = -np.sqrt(4 * np.pi * 5) * pp / rr_ref
+        ref2 = ow.xi_ell_window_route(delta, P, sv * L / box, 2, rho * rho)
         assert res.xi_l[1][j] == pytest.approx(ref2, rel=1e-8, abs=1e-12)
+
+
+def test_xi_ell_against_gauss_legendre_ring_route(phi):
+    """xi_0, xi_2, xi_4 of Corr_2PCF (multipoles, the fused pair pass) against the oracle's independent route
+    (2l+1) int_0^1 xi(s, mu) P_l(mu) dmu over ring windows (window_functions.py:164-192, corr2pcf.py:275-284),
+    on a redshift-space mock with lognormal weights, FP64 and FP32 paths."""
+    from pyhermes_amd.mocks import lognormal_mock
+    from pyhermes_amd.theory import Corr_2PCF
+    print("Corr_2PCF xi_0,2,4 J=5 vs Gauss-Legendre ring route", flush=True)
+    box, J = 400.0, 5
+    L = 1 << J
+    pos, w = lognormal_mock(40000, 32, box, 61, rsd_sigma_kms=1500.0, weight_sigma=0.3)
+    ells = [0, 2, 4]
+    s = np.array([35.0, 60.0, 90.0])
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    for precision, tol in (("fp64", 1e-8), ("fp32", 1e-4)):
+        field = _project_prec(pos, w, J, box, precision)
+        res = Corr_2PCF({"Corr_2PCF": {"sfc_field": field, "random": "uniform", "sampling": {"s": s},
+                                       "products": ["xi"], "multipoles": ells, "fout_path": ""}}).run(
+            save_result=False)
+        eps = field.epsilon.astype(np.float64)
+        rho = field.field_integral / L ** 3
+        delta = eps - rho
+        for j, ell in enumerate(ells):
+            ref = np.array([ow.xi_ell_ring_route(delta, P, sv * L / box, ell, rho * rho) for sv in s])
+            scale = np.abs(ref).max()
+            assert np.abs(res.xi_l[j] - ref).max() <= tol * scale, (precision, ell)
 
 
 def test_corr2pcf_smu_and_rppi_binning(phi):

@@ -31,9 +31,11 @@ def test_fft3_matches_numpy(J, inverse):
     assert err <= 3e-6, err
 
 
-@pytest.mark.parametrize("J", [3, 4, 5, 6])
+@pytest.mark.parametrize("J", [3, 4, 5, 6, 7])
 @pytest.mark.parametrize("l,m", [(0, 0), (1, 1), (3, -2), (6, 6), (10, 3)])
 def test_window_ifft_matches_unfused_path(phi, J, l, m):
+    """The fused window + inverse FFT against the separate apply and rocFFT; J = 7 (L = 128) runs the
+    branch where the Nyquist plane z = L/2 is evaluated by helper lanes."""
     import torch
     from pyhermes_amd import kernels
     L = 1 << J
@@ -53,18 +55,18 @@ def test_window_ifft_matches_unfused_path(phi, J, l, m):
         assert err <= 3e-6, (s, err)
 
 
-@pytest.mark.parametrize("l,m", [(0, 0), (2, 1), (4, -3)])
-def test_window_ifft_matches_oracle(phi, l, m):
+@pytest.mark.parametrize("J,l,m", [(4, 0, 0), (4, 2, 1), (4, 4, -3), (7, 0, 0), (7, 5, 2), (7, 10, -7)])
+def test_window_ifft_matches_oracle(phi, J, l, m):
+    """Against the oracle's window builder and scipy convolution; J = 7 covers the Nyquist helper lanes."""
     import torch
     from pyhermes_amd import kernels
-    J = 4
     L = 1 << J
     g = np.random.default_rng(77 + l)
     field = g.normal(size=(L, L, L))
     H = kernels.rfftn(kernels.as_device(field, torch.float32))
     Pn = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
     P = kernels.as_device(Pn, torch.float64)
-    radii = np.array([2.0, 3.3])
+    radii = np.array([2.0, 3.3, 0.3 * L])
     got = kernels.multipole_window_ifft_f32(H, P, radii, l, m, 1.0 / L ** 3).cpu().numpy()
     for s, R in enumerate(radii):
         ref = ow.convolve_complex(field, ow.radial_multipole_window(L, Pn, R, l, m))

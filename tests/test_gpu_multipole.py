@@ -145,12 +145,13 @@ def test_schedules_and_split_window_agree(phi):
     assert np.array_equal(runs["serial"], runs["overlap"])
 
 
-def test_packed_m0_window_and_split_gram(phi):
+@pytest.mark.parametrize("L", [32, 128])
+def test_packed_m0_window_and_split_gram(phi, L):
     """The packed (l, 0) + (l + 1, 0) window equals the sum of the two single windows; the split Gram's
-    real and imaginary blocks are the Grams of the real and imaginary parts."""
+    real and imaginary blocks are the Grams of the real and imaginary parts.  L = 128 runs the window
+    kernel's Nyquist helper lanes."""
     import torch
     from pyhermes_amd import kernels, multipole
-    L = 32
     rng = np.random.default_rng(23)
     delta = rng.normal(size=(L, L, L))
     d32 = kernels.as_device(delta - delta.mean(), torch.float32)
@@ -159,7 +160,7 @@ def test_packed_m0_window_and_split_gram(phi):
     radii = np.array([1.5, 3.0, 4.5, 6.0, 8.0])
     S = radii.size
     for l in (0, 3, 6):
-        print(f"packed window l={l}", flush=True)
+        print(f"packed window L={L} l={l}", flush=True)
         packed = torch.empty((S, L, L, L), dtype=torch.complex64, device=d32.device)
         kernels.multipole_window_ifft_pack0_f32(H, geo.P, radii, l, 1.0 / L ** 3, packed)
         a = kernels.multipole_window_ifft_f32(H, geo.P, radii, l, 0, 1.0 / L ** 3)
@@ -201,3 +202,54 @@ def test_packed_pipeline_matches_unpacked_and_fp64(phi):
     assert np.linalg.norm(packed - unpacked) <= 1e-5 * np.linalg.norm(unpacked)
     plain = np.array(multipole.ddd_multipole_plain_f64(geo, d64, d64, d64, radii[2], radii[7], 0, lmax))
     assert np.linalg.norm(packed[2, 7] - plain) <= 1e-4 * np.linalg.norm(plain)
+
+
+def test_headline_path_every_set_matches_oracle(phi):
+    """The headline's exact code path -- auto case, 15 shells, m = 0 terms packed in pairs, l <= 10, FP32
+    batched -- on every (r12, r13) set at L = 32 against the oracle's restatement of calc_DDD_multipole
+    (corr3pcf_multipoles.py:460-579): relative L2 error over l <= 1e-4 per set.  The oracle's shell fields
+    are built once per (shell, l, m) and its contraction regrouped over all shell pairs (summand_pairs,
+    pinned against summand_c on the CPU); one set is also run through calc_ddd_multipole verbatim."""
+    import torch
+    from oracle import native
+    from oracle import projection as oproj
+    from pyhermes_amd import kernels, multipole
+    from pyhermes_amd.mocks import lognormal_mock
+    L, lmax, S = 32, 10, 15
+    J = 5
+    box = 250.0
+    print(f"headline path L={L} S={S} l<={lmax}: product", flush=True)
+    pos, _ = lognormal_mock(60000, 32, box, 4100)
+    w = np.full(pos.shape[0], 1.0 / pos.shape[0])
+    eps, _, _ = oproj.project_c(pos, w, phi, J, box)
+    delta = eps / eps.mean() - 1.0
+    radii = np.linspace(1.5, 12.0, S)
+    geo = multipole.MultipoleGeometry(phi, L)
+    assert multipole.packs_m0(L, S)
+    d32 = kernels.as_device(delta, torch.float32)
+    got = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, lmax)
+    print("headline path: oracle fields", flush=True)
+    threads = native.num_threads()
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    g = scaling.cal_gamma(phi, 3, 1024)
+    fields = {}
+    for l in range(lmax + 1):
+        for m in range(-l, l + 1):
+            fields[(l, m)] = [om.multipole_fields(delta, P, R, l, m, threads) for R in radii]
+    print("headline path: oracle contraction", flush=True)
+    ref = np.zeros((S, S, lmax + 1))
+    for l in range(lmax + 1):
+        v = [4.0 * np.pi * om.summand_pairs(delta, fields[(l, m)], fields[(l, -m)], g) / L ** 3
+             for m in range(l + 1)]
+        for i in range(S):
+            for j in range(S):
+                ref[i, j, l] = om.combine_multipole_m_terms([vm[i, j] for vm in v], l)
+    one = om.calc_ddd_multipole(delta, delta, delta, phi, radii[0], radii[9], 0, lmax, workers=threads)
+    assert np.linalg.norm(ref[0, 9] - one) <= 1e-10 * np.linalg.norm(one)
+    worst = 0.0
+    for i in range(S):
+        for j in range(S):
+            rel = np.linalg.norm(got[i, j] - ref[i, j]) / np.linalg.norm(ref[i, j])
+            worst = max(worst, rel)
+            assert rel <= 1e-4, (i, j, rel)
+    print(f"headline path: worst relative L2 error over 225 ordered sets {worst:.2e}", flush=True)

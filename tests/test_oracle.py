@@ -570,3 +570,119 @@ def test_xi_ell_half_spectrum_sum_matches_window_route(phi, ell):
             a = windows.xi_ell_window_route(d, P, s, ell, 0.8)
             b = windows.xi_ell_half_spectrum(d, P, s, ell, 0.8)
             assert b == pytest.approx(a, rel=1e-11, abs=1e-14)
+
+
+# ---------------------------------------------------------------- radial-profile tables (non-thin edges)
+RADIAL_CASES = [("sphere", {"R": 20.0}), ("thick_shell", {"R": 30.0, "delta_R": 10.0}), ("gaussian", {"R": 8.0}),
+                ("gaussian_shell", {"R_shell": 30.0, "R_smooth": 5.0})]
+
+
+@pytest.mark.parametrize("rtype,args", RADIAL_CASES)
+def test_hankel_table_monopole_is_the_closed_form(rtype, args):
+    """U_0 from the quadrature equals the closed-form isotropic window (window_functions.py:29-129) of the
+    same profile; for the k-space Gaussian shell this also pins the inverse transform."""
+    from oracle import radial
+    k_max = np.sqrt(3.0) * 64 / 500.0
+    tab = radial.hankel_table(rtype, args, 0, k_max)
+    k = np.linspace(0.0, k_max, tab.size)
+    if rtype == "gaussian_shell":
+        want = radial.gaussian_shell_T(k, args["R_shell"], args["R_smooth"])
+    elif rtype == "gaussian":
+        want = np.exp(-0.5 * (2.0 * np.pi * k * args["R"]) ** 2)
+    elif rtype == "sphere":
+        want = radial._sphere_T(k, args["R"])
+    else:
+        a, b = args["R"] - 0.5 * args["delta_R"], args["R"] + 0.5 * args["delta_R"]
+        want = (b ** 3 * radial._sphere_T(k, b) - a ** 3 * radial._sphere_T(k, a)) / (b ** 3 - a ** 3)
+    assert np.abs(tab - want).max() <= 1e-8
+
+
+def test_hankel_table_higher_orders_by_adaptive_quadrature():
+    """U_2 and U_5 at sampled k against scipy.integrate.quad of 4 pi r^2 w(r) j_l(2 pi k r)."""
+    from scipy.integrate import quad
+    from oracle import radial
+    k_max = np.sqrt(3.0) * 32 / 300.0
+    for rtype, args, w, (lo, hi) in (
+            ("sphere", {"R": 12.0}, lambda r: 3.0 / (4.0 * np.pi * 12.0 ** 3), (0.0, 12.0)),
+            ("thick_shell", {"R": 20.0, "delta_R": 6.0}, lambda r: 3.0 / (4.0 * np.pi * (23.0 ** 3 - 17.0 ** 3)),
+             (17.0, 23.0))):
+        for l in (2, 5):
+            tab = radial.hankel_table(rtype, args, l, k_max)
+            k = np.linspace(0.0, k_max, tab.size)
+            for j in (0, 37, tab.size // 3, tab.size - 1):
+                ref = quad(lambda r: 4.0 * np.pi * r * r * w(r) * sps.spherical_jn(l, 2.0 * np.pi * k[j] * r), lo, hi,
+                           limit=400, epsabs=1e-13, epsrel=1e-12)[0]
+                assert abs(tab[j] - ref) <= 1e-9, (rtype, l, j)
+
+
+def test_thin_thick_shell_tends_to_the_shell_transfer():
+    """A thick shell of width 1e-4 R has U_l(k) = j_l(2 pi k R) up to O((2 pi k delta_R)^2)."""
+    from oracle import radial
+    k_max = np.sqrt(3.0) * 32 / 300.0
+    R = 40.0
+    tab = radial.hankel_table("thick_shell", {"R": R, "delta_R": 1e-4 * R}, 3, k_max)
+    k = np.linspace(0.0, k_max, tab.size)
+    assert np.abs(tab - sps.spherical_jn(3, 2.0 * np.pi * k * R)).max() <= 1e-6
+
+
+@pytest.mark.parametrize("l", [0, 1, 4])
+@pytest.mark.parametrize("rtype,args", RADIAL_CASES)
+def test_product_hankel_tables_match_oracle(rtype, args, l):
+    """The product's host tables (utils/radial_profiles.py) equal the oracle's restatement: node counts,
+    supports, normalisation and the transform."""
+    from oracle import radial
+    from pyhermes_amd.utils import radial_profiles as rp
+    k_max = np.sqrt(3.0) * 64 / 500.0
+    k, tab = rp.build_radial_multipole_table(rtype, args, l, k_max)
+    ref = radial.hankel_table(rtype, args, l, k_max)
+    assert k == k_max and tab.shape == ref.shape
+    assert np.abs(tab - ref).max() <= 1e-12 * max(1.0, np.abs(ref).max())
+    d = rp.diagnose_radial_multipole_table(rtype, args, max(l, 1), k_max)
+    assert d["passed"] and d["table_points"] == ref.size
+
+
+def test_profile_window_table_path_matches_closed_form_monopole(phi):
+    """The C oracle's tabulated builder (radial_multipole_windows.py:186-264) at l = 0 agrees with its
+    closed-form builder (:147-183) to the table's interpolation error."""
+    from oracle import radial
+    L, box = 16, 200.0
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    g2p = L / box
+    k_max = np.sqrt(3.0) * g2p
+    for rtype, args in RADIAL_CASES:
+        a = radial.profile_multipole_window(L, P, rtype, args, 0, 0, box)
+        tab = native_f64(radial.hankel_table(rtype, args, 0, k_max))
+        out = np.empty((L, L, L), dtype=np.complex128)
+        from oracle import native
+        native.lib().oracle_profile_multipole_window(L, native.ptr(native.f64(P)), 0, 0.0, 0.0, native.ptr(tab),
+                                                     tab.size, k_max, g2p, 0, 0, native.ptr(out.view(np.float64)))
+        assert np.abs(out - a).max() <= 1e-7 * np.abs(a).max(), rtype
+
+
+def native_f64(a):
+    return np.ascontiguousarray(a, dtype=np.float64)
+
+
+def test_radial_profile_validation_messages():
+    """The reference's request checks and messages (radial_profiles.py:115-186)."""
+    from pyhermes_amd.utils import radial_profiles as rp
+    with pytest.raises(ValueError, match="Unsupported radial profile 'cube'. Supported profiles: custom_kspace, "
+                                         "custom_real, gaussian, gaussian_shell, shell, sphere, thick_shell."):
+        rp.validate_radial_profile_request("cube", {})
+    with pytest.raises(ValueError, match=r"radial profile 'sphere' requires len_args\['R'\]."):
+        rp.validate_radial_profile_request("sphere", {})
+    with pytest.raises(ValueError, match="requires delta_R > 0"):
+        rp.validate_radial_profile_request("thick_shell", {"R": 1.0, "delta_R": 0.0})
+    with pytest.raises(ValueError, match="non-negative R_shell and R_smooth"):
+        rp.validate_radial_profile_request("gaussian_shell", {"R_shell": 1.0, "R_smooth": -1.0})
+    with pytest.raises(ValueError, match=r"require other_args\['profile'\]"):
+        rp.validate_radial_profile_request("custom_real", {}, None)
+    with pytest.raises(ValueError, match="requires a real-space profile specification"):
+        rp.validate_radial_profile_request("custom_real", {}, {"profile": rp.kspace_profile(np.cos, 5.0, 1.0)})
+    prof = rp.real_space_profile(lambda r, R: (r <= R).astype(float), 10.0)
+    rp.validate_radial_profile_request("custom_real", {"R": 10.0}, {"profile": prof})
+    # a custom top-hat equals the built-in sphere's table
+    k_max = np.sqrt(3.0) * 32 / 300.0
+    _, a = rp.build_radial_multipole_table("custom_real", {"R": 10.0}, 2, k_max, {"profile": prof})
+    _, b = rp.build_radial_multipole_table("sphere", {"R": 10.0}, 2, k_max)
+    assert a.shape == b.shape and np.abs(a - b).max() <= 1e-10
