@@ -90,7 +90,11 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, cons
     float2* tw = smem;
     float2* lds = smem + L;
     const int tid = threadIdx.x;
-    int64_t b = blockIdx.x;
+    // Workgroups are dealt to the 8 XCDs round-robin; the remap hands each XCD a contiguous range of
+    // tiles, so the ntile_z tiles of one line position (which touch the same pages: in the x pass one
+    // page per element, 2 MB apart) are translated by one XCD's TLBs instead of eight.
+    const int64_t nb = (int64_t)gridDim.x;
+    int64_t b = (nb % 8 == 0) ? (int64_t)(blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : (int64_t)blockIdx.x;
     const int zt = (int)(b % ntile_z);
     b /= ntile_z;
     const int mid = (int)(b % n_mid);

@@ -269,6 +269,16 @@ __device__ __forceinline__ long long round_to_ll(double x) {
     return __double_as_longlong(x + MAGIC) - __double_as_longlong(MAGIC);
 }
 
+// LDS slot of tile cell (row, lz), row = lx * ty + ly.  With FIXED tiles (64 z cells per row) the z
+// index is XOR-ed with an even mask that depends on the row, 2 (row & 31): a row is 512 bytes, so without
+// it the cells at one z of every row share an LDS bank, and the three x taps of a record (three rows,
+// one z) as well as records at nearby z would conflict.  Pairs (lz, lz + 1) stay adjacent and aligned,
+// so the 16-byte accesses of the clearing and output phases keep their layout within the row.
+template <bool FIXED>
+__device__ __forceinline__ int acc_slot(int row, int lz, int lgtz) {
+    return FIXED ? (row << 6) | (lz ^ ((row & 31) << 1)) : (row << lgtz) + lz;
+}
+
 // OUTS: bit 0 = FP32 field, bit 1 = FP64 field.  FIXED: the full 4 x 8 x 64 tile (grids of 64^3 and
 // larger) with compile-time extents.
 //
@@ -403,13 +413,13 @@ __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
                     const int ly = ((cy - sy) & m) - oy;
                     if (ly < 0 || ly >= ty) continue;
                     const double phxy = phx * sphi[fy + res * sy];
-                    const int rowxy = (rowx + ly) << g.lgtz;
+                    const int row = rowx + ly;
 #pragma unroll
                     for (int sz = 0; sz < 3; ++sz) {
                         const int lz = ((cz - sz) & m) - oz;
                         if (lz < 0 || lz >= tz) continue;
                         const long long q = round_to_ll(phxy * sphi[fz + res * sz]);
-                        atomicAdd((unsigned long long*)&acc[rowxy + lz], (unsigned long long)q);
+                        atomicAdd((unsigned long long*)&acc[acc_slot<FIXED>(row, lz, g.lgtz)], (unsigned long long)q);
                     }
                 }
             }
@@ -431,8 +441,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
             const int i2 = tid * 2 + j * 2 * ACC_THREADS;
             if (!FIXED && i2 >= ncell) break;
             const int64_t gi = tbase + loff[j];
-            const longlong2 a = *(const longlong2*)&acc[i2];
-            *(longlong2*)&acc[i2] = make_longlong2(0, 0);
+            const int ai = acc_slot<FIXED>(i2 >> g.lgtz, i2 & (tz - 1), g.lgtz);
+            const longlong2 a = *(const longlong2*)&acc[ai];
+            *(longlong2*)&acc[ai] = make_longlong2(0, 0);
             if (OUTS == 1) {
                 *(float2*)(out32 + gi) = make_float2(fixed_to_f32(a.x, S), fixed_to_f32(a.y, S));
             } else if (OUTS != 0) {
