@@ -315,8 +315,8 @@ static_assert(2 * (AUTO_TILE + AUTO_TBUF) * 8 >= AUTO_WAVES * 256 * 8, "reductio
 
 // NNEG = number of negative eigenvalues of Gamma; with the columns of Qs sorted by ascending lam and
 // scaled by sqrt|lam|, the filtered slot (b, c) carries sqrt|lam_b lam_c| and its weight's sign is
-// -1 exactly when one of b, c is below NNEG: those slots accumulate into a second pair of MFMA
-// accumulators that is subtracted at each flush.
+// -1 exactly when one of b, c is below NNEG: those slots enter the MFMAs with negated combination
+// coefficients (a source modifier of the FMAs, no extra instruction).
 //
 // SPLIT: the products of the real parts and those of the imaginary parts are summed apart (rows of 512
 // partials: real block, then imaginary block; the imaginary sums accumulate in LDS, keeping the
@@ -460,8 +460,9 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     float dC = 0.0f;
     float dnext = d1[(int64_t)((x0 + 2) & m) * LL + col];
 
+    // MFMA accumulators: real and imaginary products together, or apart (SPLIT); slots of negative
+    // weight enter with negated combination coefficients
     float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
-    float4_t nacr = {0.f, 0.f, 0.f, 0.f}, naci = {0.f, 0.f, 0.f, 0.f};  // negative-weight slots
     // Half of the waves (two per SIMD whichever way waves are dealt to SIMDs) run one plane behind:
     // within a barrier interval they issue the MFMAs of plane p-1 before the filter work of the step,
     // while the others filter first and then issue the MFMAs of plane p, so every SIMD has VALU work
@@ -475,16 +476,15 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         const float c1 = g10 * DB1 + g21 * DB2;                                                \
         const float c2 = g20 * DB2;                                                            \
         _Pragma("unroll") for (int bc = 0; bc < 9; ++bc) {                                     \
+            const float sg = (((bc / 3) < NNEG) != ((bc % 3) < NNEG)) ? -1.0f : 1.0f;           \
             const float2 ya = YA[bc];                                                          \
-            const float bx = c0 * ya.x + c1 * YB1[bc].x + c2 * YB2[bc].x;                      \
-            const float by = c0 * ya.y + c1 * YB1[bc].y + c2 * YB2[bc].y;                      \
-            if (((bc / 3) < NNEG) != ((bc % 3) < NNEG)) {                                      \
-                nacr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.x, bx, nacr, 0, 0, 0);          \
-                naci = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.y, by, naci, 0, 0, 0);          \
-            } else {                                                                           \
-                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.x, bx, accr, 0, 0, 0);          \
+            const float bx = (sg * c0) * ya.x + (sg * c1) * YB1[bc].x + (sg * c2) * YB2[bc].x; \
+            const float by = (sg * c0) * ya.y + (sg * c1) * YB1[bc].y + (sg * c2) * YB2[bc].y; \
+            accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.x, bx, accr, 0, 0, 0);              \
+            if (SPLIT)                                                                         \
                 acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.y, by, acci, 0, 0, 0);          \
-            }                                                                                  \
+            else                                                                               \
+                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.y, by, accr, 0, 0, 0);          \
         }                                                                                      \
     }
     // step p (p = x0 .. xend): (barrier) [lagging waves: MFMAs of plane p-1, whose values sit in the
@@ -510,18 +510,15 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         AUTO_STEP(Y0, Y1, Y2, dA, dB, dC)
         AUTO_STEP(Y1, Y2, Y0, dB, dC, dA)
         AUTO_STEP(Y2, Y0, Y1, dC, dA, dB)
+        // FP32 sums of at most six planes (432 products per entry) move into the FP64 slots
+        if ((p - x0) % 6 == 0 || p > xend) {
 #pragma unroll
-        for (int j = 0; j < 4; ++j) {
-            if (SPLIT) {
-                acc_slot[j] += (double)accr[j] - (double)nacr[j];
-                islot[j * 16] += (double)acci[j] - (double)naci[j];
-            } else {
-                acc_slot[j] += ((double)accr[j] + (double)acci[j]) - ((double)nacr[j] + (double)naci[j]);
+            for (int j = 0; j < 4; ++j) {
+                acc_slot[j] += (double)accr[j];
+                if (SPLIT) islot[j * 16] += (double)acci[j];
+                accr[j] = 0.f;
+                acci[j] = 0.f;
             }
-            accr[j] = 0.f;
-            acci[j] = 0.f;
-            nacr[j] = 0.f;
-            naci[j] = 0.f;
         }
     }
 #undef AUTO_STEP
@@ -615,7 +612,10 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
     g.cross = V != nullptr;
     g.S2 = g.cross ? n_v : n_u;
     g.vsign = v_conj ? 1.0f : -1.0f;  // Re(U V) needs -V.im, Re(U conj V) needs +V.im
-    g.xseg = L < 64 ? L : 64;
+    // planes per workgroup: the auto kernel's warm-up (two planes filtered ahead) and its final reduction
+    // are paid once per segment, so it takes 128 planes where the grid stays large (16384 workgroups at L = 512)
+    const int seg = g.cross ? 64 : 128;
+    g.xseg = L < seg ? L : seg;
     for (int i = 0; i < 9; ++i) g.Q[i] = (float)Q3x3_host[i];
     for (int a = 0; a < 3; ++a)
         for (int b = 0; b < 3; ++b)

@@ -300,7 +300,9 @@ __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
     }
     __shared__ __align__(16) long long acc[TILE_CELLS];
     __shared__ __align__(16) int cnt[COUNTS ? TILE_CELLS : 4];
-    __shared__ double sphi[3 * 1024];            // the scaling function, resident for the workgroup's life
+    // the scaling function, resident for the workgroup's life, tap-interleaved: sphi[3 f + s] = phi[f + 1024 s],
+    // so the three x taps of one record (three lanes of a wave) read adjacent words instead of one bank
+    __shared__ double sphi[3 * 1024];
     __shared__ __align__(16) ProjRec srec[ACC_RCHUNK];  // one chunk of the tile's records
     __shared__ int2 soff[OFFCH];
     const int tid = threadIdx.x;
@@ -315,7 +317,10 @@ __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
     for (int j = 0; j < 3 * 1024 / ACC_THREADS; ++j) ph[j] = phi[tid + j * ACC_THREADS];
     const double unit = __longlong_as_double((long long)*wmax_bits) * phimax3;
 #pragma unroll
-    for (int j = 0; j < 3 * 1024 / ACC_THREADS; ++j) sphi[tid + j * ACC_THREADS] = ph[j];
+    for (int j = 0; j < 3 * 1024 / ACC_THREADS; ++j) {
+        const int i = tid + j * ACC_THREADS;  // phi[i] = phi[f + 1024 s]
+        sphi[3 * (i & 1023) + (i >> 10)] = ph[j];
+    }
 
     // Software pipeline over this workgroup's tiles t, t+G, t+2G, ...: while tile t is processed, the
     // first record chunk of tile t+G is in flight into registers.
@@ -406,19 +411,19 @@ __global__ __launch_bounds__(ACC_THREADS) void k_proj_accumulate(
                 if (lx < 0 || lx >= tx) continue;
                 const int cy = (R.cell >> 10) & 1023, cz = (R.cell >> 20) & 1023;
                 const int fx = R.frac & 1023, fy = (R.frac >> 10) & 1023, fz = (R.frac >> 20) & 1023;
-                const double phx = (R.w * mult) * sphi[fx + res * sx];
+                const double phx = (R.w * mult) * sphi[3 * fx + sx];
                 const int rowx = lx << g.lgty;
 #pragma unroll
                 for (int sy = 0; sy < 3; ++sy) {
                     const int ly = ((cy - sy) & m) - oy;
                     if (ly < 0 || ly >= ty) continue;
-                    const double phxy = phx * sphi[fy + res * sy];
+                    const double phxy = phx * sphi[3 * fy + sy];
                     const int row = rowx + ly;
 #pragma unroll
                     for (int sz = 0; sz < 3; ++sz) {
                         const int lz = ((cz - sz) & m) - oz;
                         if (lz < 0 || lz >= tz) continue;
-                        const long long q = round_to_ll(phxy * sphi[fz + res * sz]);
+                        const long long q = round_to_ll(phxy * sphi[3 * fz + sz]);
                         atomicAdd((unsigned long long*)&acc[acc_slot<FIXED>(row, lz, g.lgtz)], (unsigned long long)q);
                     }
                 }
