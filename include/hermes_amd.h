@@ -101,12 +101,13 @@ int hermes_fft_c2c(hermes_ctx *ctx, int J, int batch, int dbl, int inverse, cons
  * The FP32 counterpart of scipy.fft.fftn / ifftn in convolution.py:291-295. */
 int hermes_fft3_c2c_f32(hermes_ctx *ctx, int J, int batch, int inverse, void *data);
 
-/* Tile shape the autotuner chose for the strided passes of size 2^J and direction (tile width in
- * z lines, threads per workgroup); both 0 while that size has not been transformed yet.
- * The first transform of each size times every candidate tile once on a scratch grid.
+/* Tile shape the autotuner chose for the strided y-line pass of size 2^J and direction (tile width in
+ * z lines, threads per workgroup); both 0 while that size has not been transformed yet.  The first
+ * transform of each size times every candidate tile once per axis on a scratch grid and keeps the
+ * fastest for the y lines and for the x lines apart.
  * No reference counterpart (tuning of this build's own kernels). */
 int hermes_fft3_tile_choice(hermes_ctx *ctx, int J, int inverse, int *tw_out, int *threads_out);
-/* Pins the strided-pass tile (must be one of the candidates) instead of autotuning. */
+/* Pins the strided-pass tile (must be one of the candidates) for both axes instead of autotuning. */
 int hermes_fft3_set_tile(hermes_ctx *ctx, int J, int inverse, int tw, int threads);
 
 /* ---- window descriptors (window_functions.py:29-129, :164-319, :426-438;

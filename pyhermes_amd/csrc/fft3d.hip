@@ -77,14 +77,15 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(float2* __restrict__ d
         if (r0 + i / L < nrows) base[i] = lds[F::addr(i / L, i % L)];
 }
 
-// Pass over strided lines.  Tile t = (outer, mid, zt): lines z = zt*TW .. zt*TW+TW-1, element e at
-// base + e * axis_stride (e * axis_stride < 2^31 for L <= 1024).  LDS (dynamic) holds the L twiddles
-// and TW * L complex values, element-major.
+// Pass over strided lines.  Tile t = (outer, mid, zt): the lines at z = zt*TW .. zt*TW+TW-1 of one
+// position `mid` on the other strided axis; element e of a line at e * axis_stride (< 2^31 for
+// L <= 1024).  LDS (dynamic) holds the L twiddles and TW * L complex values, element-major.
 template <int LOG2L, int TW, int NT, int DIR>
 __global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, const float2* __restrict__ twg,
                                                  int axis_stride, int64_t mid_stride, int n_mid,
                                                  int64_t outer_stride, int ntile_z) {
     constexpr int L = 1 << LOG2L;
+    static_assert(NT % TW == 0, "a thread keeps one z line of the tile");
     using F = LdsFft<LOG2L, TW, NT, DIR, true>;
     extern __shared__ float2 smem[];
     float2* tw = smem;
@@ -99,30 +100,27 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, cons
     b /= ntile_z;
     const int mid = (int)(b % n_mid);
     const int64_t outer = b / n_mid;
-    float2* base = data + outer * outer_stride + (int64_t)mid * mid_stride + (int64_t)zt * TW;
+    // this thread's z line is fixed (NT is a multiple of TW); its elements are NT / TW apart
+    float2* base = data + outer * outer_stride + (int64_t)mid * mid_stride + (zt * TW + tid % TW) +
+                   (int64_t)(tid / TW) * axis_stride;
+    const int64_t estep = (int64_t)(NT / TW) * axis_stride;
     load_twiddles<LOG2L>(tw, twg, tid, NT);
     // every load of the thread is issued before the first LDS store (TW * L / NT of them), so a
     // workgroup keeps its whole tile in flight: the pass is bound by HBM latency otherwise
     constexpr int PER = (TW * L + NT - 1) / NT;
-    constexpr bool EVEN = TW * L % NT == 0;
+    constexpr bool EVEN = TW * L % NT == 0;  // otherwise (L = 8) threads beyond the tile idle
     float2 v[PER];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * NT;
-        if (EVEN || i < TW * L) v[k] = base[(i / TW) * axis_stride + (i % TW)];
-    }
+    for (int k = 0; k < PER; ++k)
+        if (EVEN || tid + k * NT < TW * L) v[k] = base[k * estep];
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * NT;
-        if (EVEN || i < TW * L) lds[i] = v[k];
-    }
+    for (int k = 0; k < PER; ++k)
+        if (EVEN || tid + k * NT < TW * L) lds[tid + k * NT] = v[k];
     __syncthreads();
     F::run(lds, tw, tid);
 #pragma unroll
-    for (int k = 0; k < PER; ++k) {
-        const int i = tid + k * NT;
-        if (EVEN || i < TW * L) base[(i / TW) * axis_stride + (i % TW)] = lds[i];
-    }
+    for (int k = 0; k < PER; ++k)
+        if (EVEN || tid + k * NT < TW * L) base[k * estep] = lds[tid + k * NT];
 }
 
 template <int LOG2L>
@@ -145,8 +143,9 @@ struct ColVariant {
     static constexpr int NT = col_nt_of(V);
 };
 
+// One strided pass (axis 0: y lines, axis 1: x lines) of a batch of grids with tile variant V.
 template <int LOG2L, int V, int DIR>
-int launch_yx_variant(hermes_ctx* ctx, int batch, float2* data, const float2* twg) {
+int launch_cols_variant(hermes_ctx* ctx, int batch, float2* data, const float2* twg, int axis) {
     constexpr int L = 1 << LOG2L;
     constexpr int TW = ColVariant<LOG2L, V>::TW, NT = ColVariant<LOG2L, V>::NT;
     const size_t shmem = sizeof(float2) * (size_t)(L + TW * L);
@@ -154,73 +153,88 @@ int launch_yx_variant(hermes_ctx* ctx, int batch, float2* data, const float2* tw
     HERMES_CHECK_HIP(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     const int ntz = L / TW;
     const unsigned nblk = (unsigned)((int64_t)batch * L * ntz);
-    // y lines: tiles (b * L + x, zt)
-    kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L, 0, 1, (int64_t)L * L, ntz);
-    HERMES_LAUNCH_CHECK();
-    // x lines: tiles (b, y, zt)
-    kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L * L, L, L, (int64_t)L * L * L, ntz);
+    if (axis == 0)  // y lines: tiles (b * L + x, zt)
+        kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L, (int64_t)L * L, L, (int64_t)L * L * L, ntz);
+    else  // x lines: tiles (b, y, zt)
+        kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L * L, L, L, (int64_t)L * L * L, ntz);
     HERMES_LAUNCH_CHECK();
     return 0;
 }
 
 template <int LOG2L, int DIR>
-int launch_yx(hermes_ctx* ctx, int variant, int batch, float2* data, const float2* twg) {
-    if (LOG2L < 7) return launch_yx_variant<LOG2L, 0, DIR>(ctx, batch, data, twg);
+int launch_cols(hermes_ctx* ctx, int variant, int batch, float2* data, const float2* twg, int axis) {
+    if (LOG2L < 7) return launch_cols_variant<LOG2L, 0, DIR>(ctx, batch, data, twg, axis);
     switch (variant) {
-        case 1: return launch_yx_variant<LOG2L, 1, DIR>(ctx, batch, data, twg);
-        case 2: return launch_yx_variant<LOG2L, 2, DIR>(ctx, batch, data, twg);
-        case 3: return launch_yx_variant<LOG2L, 3, DIR>(ctx, batch, data, twg);
-        case 4: return launch_yx_variant<LOG2L, 4, DIR>(ctx, batch, data, twg);
-        case 5: return launch_yx_variant<LOG2L, 5, DIR>(ctx, batch, data, twg);
-        default: return launch_yx_variant<LOG2L, 0, DIR>(ctx, batch, data, twg);
+        case 1: return launch_cols_variant<LOG2L, 1, DIR>(ctx, batch, data, twg, axis);
+        case 2: return launch_cols_variant<LOG2L, 2, DIR>(ctx, batch, data, twg, axis);
+        case 3: return launch_cols_variant<LOG2L, 3, DIR>(ctx, batch, data, twg, axis);
+        case 4: return launch_cols_variant<LOG2L, 4, DIR>(ctx, batch, data, twg, axis);
+        case 5: return launch_cols_variant<LOG2L, 5, DIR>(ctx, batch, data, twg, axis);
+        default: return launch_cols_variant<LOG2L, 0, DIR>(ctx, batch, data, twg, axis);
     }
 }
 
-// Autotuner: on the first strided pass of a grid size and direction, time every variant on a
-// scratch grid (median of three y+x passes, HIP events on the context stream) and keep the
-// fastest in the context.  The scratch grid is freed afterwards.  Runs eagerly, so the first
+static float median3(const float t[3]) {
+    return t[0] < t[1] ? (t[1] < t[2] ? t[1] : (t[0] < t[2] ? t[2] : t[0]))
+                       : (t[0] < t[2] ? t[0] : (t[1] < t[2] ? t[2] : t[1]));
+}
+
+// Autotuner: on the first strided pass of a grid size and direction, time every variant on each axis
+// of a scratch batch (median of three passes, HIP events on the context stream) and keep the
+// fastest per axis in the context.  The scratch batch is freed afterwards.  Runs eagerly, so the first
 // call of a size must not be inside stream capture.
 template <int LOG2L, int DIR>
-int tuned_variant(hermes_ctx* ctx, const float2* twg, int* out) {
+int tuned_variants(hermes_ctx* ctx, const float2* twg, int out[2]) {
     const int d = DIR > 0 ? 1 : 0;
-    if (ctx->fft_cols_variant[LOG2L][d] > 0) {
-        *out = ctx->fft_cols_variant[LOG2L][d] - 1;
+    int* slot = ctx->fft_cols_variant[LOG2L][d];
+    if (slot[0] > 0 && slot[1] > 0) {
+        out[0] = slot[0] - 1;
+        out[1] = slot[1] - 1;
         return 0;
     }
     if (LOG2L < 7) {
-        ctx->fft_cols_variant[LOG2L][d] = 1;
-        *out = 0;
+        slot[0] = slot[1] = 1;
+        out[0] = out[1] = 0;
         return 0;
     }
     HERMES_REQUIRE(!ctx->capturing, "the FFT tile for L=%d is tuned on first use; run it once before capturing",
                    1 << LOG2L);
     constexpr int L = 1 << LOG2L;
-    const size_t bytes = sizeof(float2) * (size_t)L * L * L;
+    // a batch of about 4 GB (at most 16 grids), as the batched passes run: with one grid of 512^3 the
+    // candidates' times differ by less than their tail effects; one grid if that much is not free
+    const size_t grid_bytes = sizeof(float2) * (size_t)L * L * L;
+    int nf = (int)(((size_t)1 << 32) / grid_bytes);
+    nf = nf < 1 ? 1 : (nf > 16 ? 16 : nf);
     float2* scratch = nullptr;
-    HERMES_CHECK_HIP(hipMalloc(&scratch, bytes));
-    HERMES_CHECK_HIP(hipMemsetAsync(scratch, 0, bytes, ctx->stream));
+    if (hipMalloc(&scratch, grid_bytes * nf) != hipSuccess) {
+        (void)hipGetLastError();
+        nf = 1;
+        HERMES_CHECK_HIP(hipMalloc(&scratch, grid_bytes));
+    }
+    HERMES_CHECK_HIP(hipMemsetAsync(scratch, 0, grid_bytes * nf, ctx->stream));
     hipEvent_t e0, e1;
     HERMES_CHECK_HIP(hipEventCreate(&e0));
     HERMES_CHECK_HIP(hipEventCreate(&e1));
-    int best = 0;
-    float best_ms = 1e30f;
+    int best[2] = {0, 0};
+    float best_ms[2] = {1e30f, 1e30f};
     int rc = 0;
     for (int v = 0; v < N_COL_VARIANTS && rc == 0; ++v) {
-        float t[3];
-        rc = launch_yx<LOG2L, DIR>(ctx, v, 1, scratch, twg);  // warm-up
-        for (int r = 0; r < 3 && rc == 0; ++r) {
-            if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = 1;
-            if (rc == 0) rc = launch_yx<LOG2L, DIR>(ctx, v, 1, scratch, twg);
-            if (rc == 0 && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = 1;
-            if (rc == 0 && hipEventSynchronize(e1) != hipSuccess) rc = 1;
-            if (rc == 0 && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = 1;
-        }
-        if (rc) break;
-        const float med = t[0] < t[1] ? (t[1] < t[2] ? t[1] : (t[0] < t[2] ? t[2] : t[0]))
-                                      : (t[0] < t[2] ? t[0] : (t[1] < t[2] ? t[2] : t[1]));
-        if (med < best_ms) {
-            best_ms = med;
-            best = v;
+        for (int axis = 0; axis < 2 && rc == 0; ++axis) {
+            float t[3];
+            rc = launch_cols<LOG2L, DIR>(ctx, v, nf, scratch, twg, axis);  // warm-up
+            for (int r = 0; r < 3 && rc == 0; ++r) {
+                if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = 1;
+                if (rc == 0) rc = launch_cols<LOG2L, DIR>(ctx, v, nf, scratch, twg, axis);
+                if (rc == 0 && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = 1;
+                if (rc == 0 && hipEventSynchronize(e1) != hipSuccess) rc = 1;
+                if (rc == 0 && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = 1;
+            }
+            if (rc) break;
+            const float med = median3(t);
+            if (med < best_ms[axis]) {
+                best_ms[axis] = med;
+                best[axis] = v;
+            }
         }
     }
     (void)hipEventDestroy(e0);
@@ -231,18 +245,23 @@ int tuned_variant(hermes_ctx* ctx, const float2* twg, int* out) {
         if (rc == 1) hermes_set_error("FFT autotuning failed for L=%d", L);
         return rc;
     }
-    ctx->fft_cols_variant[LOG2L][d] = best + 1;
-    *out = best;
+    for (int axis = 0; axis < 2; ++axis) {
+        slot[axis] = best[axis] + 1;
+        out[axis] = best[axis];
+    }
     return 0;
 }
 
-// Strided passes (y lines, then x lines) of a batch of grids with the tuned tile shape.
+// Strided passes (y lines, then x lines) of a batch of grids with the tuned tile shape of each axis
+// (the x lines, 2 MB apart, gain from wider z tiles than the y lines).
 template <int LOG2L, int DIR>
 int run_yx(hermes_ctx* ctx, int batch, float2* data, const float2* twg) {
-    int v = 0;
-    int rc = tuned_variant<LOG2L, DIR>(ctx, twg, &v);
+    int v[2] = {0, 0};
+    int rc = tuned_variants<LOG2L, DIR>(ctx, twg, v);
     if (rc) return rc;
-    return launch_yx<LOG2L, DIR>(ctx, v, batch, data, twg);
+    rc = launch_cols<LOG2L, DIR>(ctx, v[0], batch, data, twg, 0);
+    if (rc) return rc;
+    return launch_cols<LOG2L, DIR>(ctx, v[1], batch, data, twg, 1);
 }
 
 template <int LOG2L, int DIR>
@@ -317,7 +336,7 @@ int hermes_fft3_inverse_yx_f32(hermes_ctx* ctx, int J, int batch, void* data) {
 int hermes_fft3_tile_choice(hermes_ctx* ctx, int J, int inverse, int* tw_out, int* threads_out) {
     HERMES_REQUIRE(ctx && tw_out && threads_out, "hermes_fft3_tile_choice: null argument");
     HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_fft3_tile_choice: J=%d outside 3..10", J);
-    const int v = ctx->fft_cols_variant[J][inverse ? 1 : 0] - 1;
+    const int v = ctx->fft_cols_variant[J][inverse ? 1 : 0][0] - 1;
     if (v < 0) {
         *tw_out = 0;
         *threads_out = 0;
@@ -342,7 +361,7 @@ int hermes_fft3_set_tile(hermes_ctx* ctx, int J, int inverse, int tw, int thread
     }
     HERMES_REQUIRE(v >= 0, "hermes_fft3_set_tile: (tw=%d, threads=%d) is not a candidate tile for J=%d", tw, threads,
                    J);
-    ctx->fft_cols_variant[J][inverse ? 1 : 0] = v + 1;
+    for (int axis = 0; axis < 2; ++axis) ctx->fft_cols_variant[J][inverse ? 1 : 0][axis] = v + 1;
     return 0;
 }
 

@@ -20,7 +20,8 @@ struct hermes_ctx {
     size_t ws_size[9] = {0};
     hermes_fft_cache* fft = nullptr;
     float2* twiddles = nullptr;  // e^{+-2 pi i k / 2^j} tables for the LDS FFTs (first use)
-    int fft_cols_variant[11][2] = {};  // autotuned strided-pass tile per (log2 L, inverse); 0 = not tuned
+    // autotuned strided-pass tile per (log2 L, inverse, axis: 0 = y lines, 1 = x lines); 0 = not tuned
+    int fft_cols_variant[11][2][2] = {};
     // optional HIP-event timing of the Gram contraction kernel (hermes_profile_gram)
     bool time_gram = false;
     std::vector<hipEvent_t> gram_ev;  // (start, end) per timed launch, summed when queried
