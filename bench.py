@@ -732,8 +732,11 @@ def main():
                          f"x{n_mterms} m-terms"}
         log(f"CPU: {per_term:.2f} s per m-term at J={Js} -> {1.0 / s_per_set:.3e} sets/s at J={J}")
 
-    # roofline of the dominant kernel, the auto Gram contraction k_gram_auto: per launch it forms the
-    # S x S Gram of 18 real products per cell (9 yz slots x re/im against the combined right operand)
+    # roofline of the dominant kernel, the auto Gram contraction k_gram_auto.  Flop basis: the contraction
+    # with the x stencil on d1 and the y and z stencils on the fields, an S x S Gram of 18 real rank-1
+    # products per cell (9 yz eigen slots x re/im against a 3-term right operand).  The kernel reaches the
+    # same sums with the z stencil on d1 as well: 6 MFMA products per cell against 13-term right operands
+    # formed on the VALU, so the figure is the rate of the contraction, not of the instructions executed.
     S = args.nshell
     gram_flops = 2.0 * S * S * 18 * N
     achieved = gram_flops / (gram_ms * 1e-3) / 1e12
@@ -781,6 +784,7 @@ def main():
         "roofline": {"bound": "mfma", "kernel": "k_gram_auto", "achieved": achieved,
                      "peak": MFMA_F32_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": achieved / MFMA_F32_PEAK_TFLOPS,
                      "traffic": traffic, "avg_launch_ms": gram_ms,
+                     "flops_basis": "2 S^2 18 L^3 per launch (x-delta-form contraction)",
                      "algorithmic_bytes": S * N * 8 + N * 4},
         "cpu_baseline": cpu,
         "stages_ms_per_step": stages,

@@ -280,57 +280,56 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
     }
 }
 
-// Auto Gram (V = U): the x-direction Gamma stencil moves from the fields onto d1.  For a cell
-// column (y, z) and yz-filtered values Y_p[bc] of plane p,
-//   sum_x d(x) sum_{l1,l2} Gamma[l1][l2] Re(Y_{x-l1}(a) conj Y_{x-l2}(b))
-//     = A[a][b] + A[b][a],  A = sum_p [c0/2 Re(Y_p Y_p*) + c1 Re(Y_p Y_{p+1}*) + c2 Re(Y_p Y_{p+2}*)],
-// with c0 = sum_l Gamma[l][l] d(p+l), c1 = Gamma[1][0] d(p+1) + Gamma[2][1] d(p+2), c2 = Gamma[2][0] d(p+2)
-// (periodic sums over all planes; the transpose is added by the host after the reduction).  The
-// three terms share the left operand Y_p(a), so per bc the right operand is the combination
-// lam_bc (c0/2 Y_p + c1 Y_{p+1} + c2 Y_{p+2})(b) and one MFMA pair does the update: 18 MFMAs per
-// plane (the eigen form needs 54) and no x filter.
+// Auto Gram with the z-direction Gamma stencil also moved onto d1 (the y stencil stays on the fields as
+// the eigen-filter T_b(y) = sum_ly Qs[ly][b] F(y - ly), column b scaled by sqrt|lam_b|, sign s_b).  For the
+// cells (p, r) of a column plane and delta = (dx, dz),
+//   sum_{x, z} d(x, z) sum_{l1, l2} G[l1x][l2x] G[l1z][l2z] T^a(x - l1) T^b*(x - l2) = A[a][b] + A[b][a],
+//   A = sum_{p, r} T^a(p, r) B^b(p, r),  B(p, r) = sum_{delta in H} w_delta C_delta(p, r) T*(p + dx, r + dz),
+//   C_delta(p, r) = sum_{l1x} G[l1x][l1x - dx] E_dz(p + l1x, r),  E_dz(x', r) = sum_{l1z} G[l1z][l1z - dz] d(x', r + l1z),
+// each sum over the taps that keep l1 - delta in 0 .. 2; H = {(0, 0) with weight 1/2, (0, 1), (0, 2),
+// (1, -2 .. 2), (2, -2 .. 2)} (the pair delta, -delta gives the transposed term).  Per channel b one right
+// operand B of 13 terms and one MFMA pair: 6 MFMAs per plane instead of 18, and no z filter.
 //
-// Memory: the MFMA lane layout puts 16 shells (arrays N apart) in lanes 0..15, so direct loads touch
-// 16 cache lines per instruction.  Instead a workgroup of 16 waves owns a 16 (z) x 4 (y) cell tile and
-// stages each plane's halo'd tile (6 rows x 18 z x 16 shells) into LDS with loads that run along z.
+// Each y-filtered plane q is read from LDS once (at z - 2 .. z + 2 of the lane's cell, step q - 2) and
+// scattered into the right operands of the three planes it serves: B(q - 2) (dx = 2), B(q - 1) (dx = 1) and
+// B(q) (dx = 0); B(q - 2) is then complete and meets its left operand T(q - 2, z), kept since it was read.
+// C and E depend on the cell only; E of a plane is computed once and kept for three steps.
 //
-// Filter: Y[b][c] = sum_{ly, lz} Qs[ly][b] Qs[lz][c] F(y - ly, z - lz), the y taps first.  The y-filtered
-// rows T[b](y, z') = sum_ly Qs[ly][b] F(y - ly, z') of a plane are shared by the three cells z' .. z' + 2,
-// so the workgroup computes them once per tile position (4 rows x 18 z x 16 shells, one or two per
-// thread) into LDS, one plane ahead, and each lane applies only the z taps to its three T values per b.
-// Pipeline at step p (after the barrier): z taps of plane p+2 from T buffer (p+2)&1, y taps of plane p+3
-// from raw buffer (p+3)&1 into T buffer (p+3)&1, staged plane p+4 stored to raw buffer (p+4)&1, plane
-// p+5 loaded into registers; each buffer is written and read on opposite sides of a barrier.
-constexpr int AUTO_WAVES = 16;
-constexpr int AUTO_TY = 4, AUTO_TZ = 16;                       // cells per workgroup tile
-constexpr int AUTO_RY = AUTO_TY + 2, AUTO_RZ = AUTO_TZ + 2;   // halo'd tile
-constexpr int AUTO_PITCH = 17;                                // 16 shells + 1 pad per tile position
-constexpr int AUTO_TILE = AUTO_RY * AUTO_RZ * AUTO_PITCH;     // float2 per raw buffer
-constexpr int AUTO_TBUF = 3 * AUTO_TY * AUTO_RZ * AUTO_PITCH; // float2 per T buffer: [b][row][z'][shell]
-constexpr int AUTO_STAGE = AUTO_RY * AUTO_RZ * 16;            // elements staged per plane
-constexpr int AUTO_SPT = (AUTO_STAGE + 64 * AUTO_WAVES - 1) / (64 * AUTO_WAVES);
-constexpr int AUTO_TPOS = AUTO_TY * AUTO_RZ * 16;             // T positions per plane (1152)
-constexpr int AUTO_TPT = (AUTO_TPOS + 64 * AUTO_WAVES - 1) / (64 * AUTO_WAVES);
-static_assert(2 * (AUTO_TILE + AUTO_TBUF) * 8 >= AUTO_WAVES * 256 * 8, "reduction reuses the tile buffers");
+// LDS: y-filtered planes in 2 buffers (step p reads plane p+2 while plane p+3 is written), raw halo'd
+// tiles in 2; the z halo is 2 on both sides.  Step p, after its one barrier: plane p+2 into the operands
+// (and the MFMAs of plane p), y taps of plane p+3 from raw buffer (p+3)&1, store of staged plane p+4 into
+// raw buffer (p+4)&1, load of plane p+5 into registers.  A workgroup of 8 waves owns 2 (y) x 16 (z) cells
+// and 50 KB of LDS.
+// Tile geometry with TY rows of 16 cells: 4 TY waves (4 along z with 4 cells each, times TY rows).  The raw
+// tile keeps 16 shells per (row, z) position, the shell slot XOR-swizzled by z & 15 so that both the
+// z-fastest staging stores and the shell-fastest y-tap reads spread over the banks.
+template <int TY>
+struct D2Geom {
+    static constexpr int WAVES = 4 * TY, NROW = TY, TZ = 16;
+    static constexpr int RY = TY + 2, RZ = TZ + 4;   // raw rows y0-2 .. y0+TY-1, z z0-2 .. z0+17
+    static constexpr int RP = 16;
+    static constexpr int TILE = RY * RZ * RP;        // float2 per raw buffer
+    static constexpr int TB = TY * RZ * 16;          // float2 per channel of a T buffer
+    static constexpr int TBUF = 3 * TB;              // float2 per T buffer: [b][row][z][shell]
+    static constexpr int STAGE = RY * RZ * 16;       // elements staged per plane
+    static constexpr int SPT = (STAGE + 64 * WAVES - 1) / (64 * WAVES);
+    static constexpr int TPT = (TB + 64 * WAVES - 1) / (64 * WAVES);
+    static_assert((2 * TILE + 2 * TBUF) * 8 >= WAVES * 256 * 8, "reduction reuses the tile buffers");
+    // tiles, and the split variant's per-wave imaginary sums, in at most 80 KB: two workgroups per CU
+    static_assert((2 * TILE + 2 * TBUF) * 8 + WAVES * 256 * 8 <= 80 * 1024, "LDS of two workgroups per CU");
+};
+// 2 rows per workgroup: 8 waves and at most 66 KB of LDS, so two workgroups and four waves per SIMD
+using D2 = D2Geom<2>;
 
-// NNEG = number of negative eigenvalues of Gamma; with the columns of Qs sorted by ascending lam and
-// scaled by sqrt|lam|, the filtered slot (b, c) carries sqrt|lam_b lam_c| and its weight's sign is
-// -1 exactly when one of b, c is below NNEG: those slots enter the MFMAs with negated combination
-// coefficients (a source modifier of the FMAs, no extra instruction).
-//
-// SPLIT: the products of the real parts and those of the imaginary parts are summed apart (rows of 512
-// partials: real block, then imaginary block; the imaginary sums accumulate in LDS, keeping the
-// register count within four waves per SIMD), for fields that pack two real fields as real and
-// imaginary part.
 template <int NNEG, bool SPLIT = false>
-__global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __restrict__ d1,
-                                                               const float2* __restrict__ U, GramArgs g,
-                                                               double* __restrict__ partial) {
-    // raw buffers, T buffers; the final reduction reuses the same memory
-    __shared__ __align__(16) float2 smem[2 * AUTO_TILE + 2 * AUTO_TBUF];
-    __shared__ double redi[SPLIT ? AUTO_WAVES * 256 : 1];  // imaginary sums per wave (SPLIT)
-    auto tile = [&](int buf) -> float2* { return smem + buf * AUTO_TILE; };
-    auto tb = [&](int buf) -> float2* { return smem + 2 * AUTO_TILE + buf * AUTO_TBUF; };
+__global__ __launch_bounds__(64 * D2::WAVES, 4) void k_gram_auto(const float* __restrict__ d1,
+                                                           const float2* __restrict__ U, GramArgs g,
+                                                           double* __restrict__ partial) {
+    using GM = D2;
+    __shared__ __align__(16) float2 smem[2 * GM::TILE + 2 * GM::TBUF];
+    __shared__ double redi[SPLIT ? GM::WAVES * 256 : 1];  // imaginary sums per wave (SPLIT: registers are full)
+    auto raw = [&](int q) -> float2* { return smem + (q & 1) * GM::TILE; };
+    auto tb = [&](int q) -> float2* { return smem + 2 * GM::TILE + (q & 1) * GM::TBUF; };
     const int L = g.L, m = L - 1;
     const int tid = threadIdx.x;
     const int lane = tid % 64, wave = tid / 64;
@@ -339,16 +338,15 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     // Workgroups are dealt to the 8 XCDs round-robin; the remap gives each XCD a contiguous range of
     // logical blocks (x segment slowest, then columns 4 tiles wide walked along y), so co-resident
     // workgroups share halo rows in the XCD's L2.
-    const int nzb = L / AUTO_TZ, nyb = L / AUTO_TY;
+    const int nzb = L / GM::TZ, nyb = L / GM::NROW;
     const int nb = nzb * nyb * (L / g.xseg);
     const int q = (nb % 8 == 0) ? (int)(blockIdx.x % 8) * (nb / 8) + (int)(blockIdx.x / 8) : (int)blockIdx.x;
     const int xs = q / (nzb * nyb);
-    const int r = q % (nzb * nyb);
+    const int rr = q % (nzb * nyb);
     const int PZ = nzb < 4 ? nzb : 4;
-    const int zb = (r / (PZ * nyb)) * PZ + (r % (PZ * nyb)) % PZ;
-    const int yb = (r % (PZ * nyb)) / PZ;
-    const int z0 = zb * AUTO_TZ, y0 = yb * AUTO_TY;
-    const int y = y0 + yw;
+    const int zb = (rr / (PZ * nyb)) * PZ + (rr % (PZ * nyb)) % PZ;
+    const int yb = (rr % (PZ * nyb)) / PZ;
+    const int z0 = zb * GM::TZ, y0 = yb * GM::NROW;
     const int x0 = xs * g.xseg, xend = x0 + g.xseg;
     const int64_t N = (int64_t)L * L * L;
     const int64_t LL = (int64_t)L * L;
@@ -356,77 +354,80 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
     float Q[9];
 #pragma unroll
     for (int i = 0; i < 9; ++i) Q[i] = g.Qs[i];
-    const float g00 = 0.5f * g.gx[0], g11 = 0.5f * g.gx[4], g22 = 0.5f * g.gx[8];
-    const float g10 = g.gx[3], g21 = g.gx[7], g20 = g.gx[6];
-    const int64_t col = (int64_t)y * L + (z0 + 4 * zw + k);
+    const float G00 = g.gx[0], G11 = g.gx[4], G22 = g.gx[8], G10 = g.gx[3], G21 = g.gx[7], G20 = g.gx[6];
+    const float H00 = 0.5f * G00, H11 = 0.5f * G11, H22 = 0.5f * G22;  // the weight 1/2 of delta = (0, 0)
+    // this lane's cell (y0 + yw, z0 + cz); d1 offsets of z, z + 1, z + 2 within a plane
+    const int cz = 4 * zw + k;
+    const int yrow = (y0 + yw) * L, zc = z0 + cz;
 
-    // staging: element e = (sh * AUTO_RY + ry) * AUTO_RZ + rz (z fastest: loads run along z);
-    // padding shells (sh >= S1) read shell 0 (their result rows are never read)
-    const float2* src[AUTO_SPT];
-    int dst[AUTO_SPT];
-    bool act[AUTO_SPT];
+    constexpr int SPT = GM::SPT;
+    const float2* src[SPT];
+    int dst[SPT];
+    bool act[SPT];
 #pragma unroll
-    for (int i = 0; i < AUTO_SPT; ++i) {
-        const int e = tid + i * 64 * AUTO_WAVES;
-        act[i] = e < AUTO_STAGE;
+    for (int i = 0; i < SPT; ++i) {
+        const int e = tid + i * 64 * GM::WAVES;
+        act[i] = e < GM::STAGE;
         const int ee = act[i] ? e : 0;
-        const int sh = ee / (AUTO_RY * AUTO_RZ), ry = (ee / AUTO_RZ) % AUTO_RY, rz = ee % AUTO_RZ;
+        const int sh = ee / (GM::RY * GM::RZ), ry = (ee / GM::RZ) % GM::RY, rz = ee % GM::RZ;
         src[i] = U + (int64_t)(sh < g.S1 ? sh : 0) * N + (int64_t)((y0 - 2 + ry) & m) * L + ((z0 - 2 + rz) & m);
-        dst[i] = (ry * AUTO_RZ + rz) * AUTO_PITCH + sh;
+        dst[i] = (ry * GM::RZ + rz) * GM::RP + (sh ^ (rz & 15));
     }
-    float2 stg[AUTO_SPT];
+    float2 stg[SPT];
     auto stage_load = [&](int plane) {
 #pragma unroll
-        for (int i = 0; i < AUTO_SPT; ++i)
+        for (int i = 0; i < SPT; ++i)
             if (act[i]) stg[i] = src[i][(int64_t)(plane & m) * LL];
     };
-    auto stage_store = [&](int buf) {
+    auto stage_store = [&](int plane) {
 #pragma unroll
-        for (int i = 0; i < AUTO_SPT; ++i)
-            if (act[i]) tile(buf)[dst[i]] = stg[i];
+        for (int i = 0; i < SPT; ++i)
+            if (act[i]) raw(plane)[dst[i]] = stg[i];
     };
-    // y taps: position u = (row * AUTO_RZ + z') * 16 + sh reads raw rows row + 2 - ly (tile row r holds
+    // y taps: position u = (row * GM::RZ + z') * 16 + sh reads raw rows row + 2 - ly (tile row r holds
     // y0 - 2 + r) and writes T[b][row][z'][sh]
-    int traw[AUTO_TPT], tout[AUTO_TPT];
-    bool tact[AUTO_TPT];
+    int traw[GM::TPT];
+    bool tact[GM::TPT];
 #pragma unroll
-    for (int i = 0; i < AUTO_TPT; ++i) {
-        const int u = tid + i * 64 * AUTO_WAVES;
-        tact[i] = u < AUTO_TPOS;
+    for (int i = 0; i < GM::TPT; ++i) {
+        const int u = tid + i * 64 * GM::WAVES;
+        tact[i] = u < GM::TB;
         const int uu = tact[i] ? u : 0;
-        const int sh = uu % 16, zp = (uu / 16) % AUTO_RZ, row = uu / (16 * AUTO_RZ);
-        traw[i] = ((row + 2) * AUTO_RZ + zp) * AUTO_PITCH + sh;
-        tout[i] = (row * AUTO_RZ + zp) * AUTO_PITCH + sh;
+        const int sh = uu % 16, zp = (uu / 16) % GM::RZ, row = uu / (16 * GM::RZ);
+        traw[i] = ((row + 2) * GM::RZ + zp) * GM::RP + (sh ^ (zp & 15));
     }
-    auto tphase = [&](int buf) {
+    auto tphase = [&](int plane) {
+        const float2* rw = raw(plane);
+        float2* t = tb(plane);
 #pragma unroll
-        for (int i = 0; i < AUTO_TPT; ++i) {
+        for (int i = 0; i < GM::TPT; ++i) {
             if (!tact[i]) continue;
-            const float2 r0 = tile(buf)[traw[i]];
-            const float2 r1 = tile(buf)[traw[i] - AUTO_RZ * AUTO_PITCH];
-            const float2 r2 = tile(buf)[traw[i] - 2 * AUTO_RZ * AUTO_PITCH];
+            const float2 r0 = rw[traw[i]];
+            const float2 r1 = rw[traw[i] - GM::RZ * GM::RP];
+            const float2 r2 = rw[traw[i] - 2 * GM::RZ * GM::RP];
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
                 const float q0 = Q[0 * 3 + b], q1 = Q[1 * 3 + b], q2 = Q[2 * 3 + b];
-                tb(buf)[b * (AUTO_TY * AUTO_RZ * AUTO_PITCH) + tout[i]] =
+                t[b * GM::TB + tid + i * 64 * GM::WAVES] =
                     make_float2(q0 * r0.x + q1 * r1.x + q2 * r2.x, q0 * r0.y + q1 * r1.y + q2 * r2.y);
             }
         }
     };
-    // z taps of this lane's cell: T[b] at z' = (4 zw + k + 2) - lz of row yw
-    const int tme = (yw * AUTO_RZ + (4 * zw + k + 2)) * AUTO_PITCH + s;
-    auto zfilter = [&](int buf, float2 Y[9]) {
-#pragma unroll
-        for (int b = 0; b < 3; ++b) {
-            const float2* t = tb(buf) + b * (AUTO_TY * AUTO_RZ * AUTO_PITCH) + tme;
-            const float2 u0 = t[0], u1 = t[-AUTO_PITCH], u2 = t[-2 * AUTO_PITCH];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const float q0 = Q[0 * 3 + c], q1 = Q[1 * 3 + c], q2 = Q[2 * 3 + c];
-                Y[b * 3 + c] = make_float2(q0 * u0.x + q1 * u1.x + q2 * u2.x, q0 * u0.y + q1 * u1.y + q2 * u2.y);
-            }
-        }
+    // E_dz of a plane from its d1 values at z, z + 1, z + 2 (index dz + 2); G symmetric
+    auto efun = [&](const float dd[3], float E[5]) {
+        E[0] = G20 * dd[0];
+        E[1] = G10 * dd[0] + G21 * dd[1];
+        E[2] = G00 * dd[0] + G11 * dd[1] + G22 * dd[2];
+        E[3] = G10 * dd[1] + G21 * dd[2];
+        E[4] = G20 * dd[2];
     };
+    auto dload = [&](int plane, float dd[3]) {
+        const float* dp = d1 + (int64_t)(plane & m) * LL;
+        dd[0] = dp[yrow + (zc & m)];
+        dd[1] = dp[yrow + ((zc + 1) & m)];
+        dd[2] = dp[yrow + ((zc + 2) & m)];
+    };
+    const int tme = (yw * GM::RZ + cz + 2) * 16 + s;  // this lane's T index at dz = 0 (channel 0)
 
     double acc_slot[4] = {0.0, 0.0, 0.0, 0.0};  // this lane's G entries 4 (lane / 16) + j, column lane % 16
     double* islot = redi + (SPLIT ? wave * 256 + (4 * (lane / 16)) * 16 + (lane % 16) : 0);
@@ -435,111 +436,131 @@ __global__ __launch_bounds__(64 * AUTO_WAVES) void k_gram_auto(const float* __re
         for (int j = 0; j < 4; ++j) islot[j * 16] = 0.0;
     }
 
-    float2 Y0[9], Y1[9], Y2[9];
-    // warm-up (the steps x0-3 .. x0-1 without MFMAs): Y0 = plane x0, Y1 = plane x0+1, T buffer
-    // (x0+2)&1 = plane x0+2, raw buffer (x0+3)&1 = plane x0+3, stg = plane x0+4
+    // warm-up: T of plane x0, raw plane x0+1, staged plane x0+2; E of planes x0, x0+1; d of plane x0+2.  The
+    // loop starts at p = x0 - 2, whose first two steps only fill the operands of planes x0 and x0+1.
+    float E0[5], E1[5], E2[5], dn[3];
+    float2 B0[3], B1[3], B2[3], A0[3], A1[3], A2[3];
     stage_load(x0);
-    stage_store(x0 & 1);
+    stage_store(x0);
     stage_load(x0 + 1);
+    dload(x0, dn);
+    efun(dn, E0);
+    dload(x0 + 1, dn);
+    efun(dn, E1);
+    dload(x0 + 2, dn);
     __syncthreads();
-    tphase(x0 & 1);
-    stage_store((x0 + 1) & 1);
+    tphase(x0);
+    stage_store(x0 + 1);
     stage_load(x0 + 2);
-    __syncthreads();
-    zfilter(x0 & 1, Y0);
-    tphase((x0 + 1) & 1);
-    stage_store(x0 & 1);
-    stage_load(x0 + 3);
-    __syncthreads();
-    zfilter((x0 + 1) & 1, Y1);
-    tphase(x0 & 1);
-    stage_store((x0 + 1) & 1);
-    stage_load(x0 + 4);
-    float dA = d1[(int64_t)(x0 & m) * LL + col];
-    float dB = d1[(int64_t)((x0 + 1) & m) * LL + col];
-    float dC = 0.0f;
-    float dnext = d1[(int64_t)((x0 + 2) & m) * LL + col];
 
-    // MFMA accumulators: real and imaginary products together, or apart (SPLIT); slots of negative
-    // weight enter with negated combination coefficients
     float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};
-    // Half of the waves (two per SIMD whichever way waves are dealt to SIMDs) run one plane behind:
-    // within a barrier interval they issue the MFMAs of plane p-1 before the filter work of the step,
-    // while the others filter first and then issue the MFMAs of plane p, so every SIMD has VALU work
-    // from one group beside the matrix work of the other.
+    // step p: slots C* hold plane p (complete now), N* plane p+1, F* plane p+2 (started now); EA = E(p+2),
+    // EB = E(p+3), EF = E(p+4) (computed now)
+#define D2_STEP(FIN, MID, BC, BN, BF, AC, AN, AF, EA, EB, EF)                                         \
+    if (p < xend) {                                                                                 \
+        __syncthreads();                                                                            \
+        efun(dn, EF);                                                                               \
+        dload(p + 5, dn);                                                                           \
+        float c2[5], c1[5];                                                                         \
+        _Pragma("unroll") for (int j = 0; j < 5; ++j) {                                             \
+            c2[j] = G20 * EA[j];                                                                    \
+            c1[j] = G10 * EA[j] + G21 * EB[j];                                                      \
+        }                                                                                           \
+        const float c00 = H00 * EA[2] + H11 * EB[2] + H22 * EF[2];                                  \
+        const float c01 = G00 * EA[3] + G11 * EB[3] + G22 * EF[3];                                  \
+        const float c02 = G00 * EA[4] + G11 * EB[4] + G22 * EF[4];                                  \
+        if (lead) {                                                                                 \
+            tphase(p + 3);                                                                          \
+            stage_store(p + 4);                                                                     \
+            stage_load(p + 5);                                                                      \
+        }                                                                                           \
+        const float2* t = tb(p + 2) + tme;                                                          \
+        _Pragma("unroll") for (int b = 0; b < 3; ++b) {                                             \
+            const float sg = b < NNEG ? -1.0f : 1.0f;                                               \
+            float2 u[5];                                                                            \
+            _Pragma("unroll") for (int j = 0; j < 5; ++j) u[j] = t[b * GM::TB + (j - 2) * 16];       \
+            /* plane p + 2 as dx = 2 of plane p: its right operand is complete */                   \
+            if (FIN) {                                                                              \
+                float bx = BC[b].x, by = BC[b].y;                                                   \
+                _Pragma("unroll") for (int j = 0; j < 5; ++j) {                                     \
+                    bx = (sg * c2[j]) * u[j].x + bx;                                                \
+                    by = (sg * c2[j]) * u[j].y + by;                                                \
+                }                                                                                   \
+                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(AC[b].x, bx, accr, 0, 0, 0);            \
+                if (SPLIT)                                                                          \
+                    acci = __builtin_amdgcn_mfma_f32_16x16x4f32(AC[b].y, by, acci, 0, 0, 0);        \
+                else                                                                                \
+                    accr = __builtin_amdgcn_mfma_f32_16x16x4f32(AC[b].y, by, accr, 0, 0, 0);        \
+            }                                                                                       \
+            /* dx = 1 of plane p + 1 */                                                             \
+            if (MID) {                                                                              \
+                _Pragma("unroll") for (int j = 0; j < 5; ++j) {                                     \
+                    BN[b].x = (sg * c1[j]) * u[j].x + BN[b].x;                                      \
+                    BN[b].y = (sg * c1[j]) * u[j].y + BN[b].y;                                      \
+                }                                                                                   \
+            }                                                                                       \
+            /* dx = 0 of plane p + 2 (its first terms) and its left operand */                      \
+            BF[b].x = (sg * c00) * u[2].x;                                                          \
+            BF[b].y = (sg * c00) * u[2].y;                                                          \
+            BF[b].x = (sg * c01) * u[3].x + BF[b].x;                                                \
+            BF[b].y = (sg * c01) * u[3].y + BF[b].y;                                                \
+            BF[b].x = (sg * c02) * u[4].x + BF[b].x;                                                \
+            BF[b].y = (sg * c02) * u[4].y + BF[b].y;                                                \
+            AF[b] = u[2];                                                                           \
+        }                                                                                           \
+        if (SPLIT && FIN) { /* the imaginary sums go to LDS every step (no registers to keep them) */ \
+            _Pragma("unroll") for (int j = 0; j < 4; ++j) {                                         \
+                islot[j * 16] += (double)acci[j];                                                   \
+                acci[j] = 0.f;                                                                      \
+            }                                                                                       \
+        }                                                                                           \
+        if (!lead) {                                                                                \
+            tphase(p + 3);                                                                          \
+            stage_store(p + 4);                                                                     \
+            stage_load(p + 5);                                                                      \
+        }                                                                                           \
+        ++p;                                                                                        \
+    }
+    // Half of the waves (two per SIMD whichever way waves are dealt to SIMDs) do the y taps and staging of
+    // a step before its operand work, the others after, so that every SIMD has LDS and arithmetic work from
+    // different phases within a barrier interval (the y taps write a buffer that no wave reads this step).
     const int wv = __builtin_amdgcn_readfirstlane(wave);
-    const bool lag = ((wv >> 2) ^ wv) & 1;
-    // MFMAs of one plane: YA its filtered values, YB1 / YB2 the next two planes, D* their d1 values
-#define AUTO_MFMA(YA, YB1, YB2, DA, DB1, DB2)                                                    \
-    {                                                                                          \
-        const float c0 = g00 * DA + g11 * DB1 + g22 * DB2;                                     \
-        const float c1 = g10 * DB1 + g21 * DB2;                                                \
-        const float c2 = g20 * DB2;                                                            \
-        _Pragma("unroll") for (int bc = 0; bc < 9; ++bc) {                                     \
-            const float sg = (((bc / 3) < NNEG) != ((bc % 3) < NNEG)) ? -1.0f : 1.0f;           \
-            const float2 ya = YA[bc];                                                          \
-            const float bx = (sg * c0) * ya.x + (sg * c1) * YB1[bc].x + (sg * c2) * YB2[bc].x; \
-            const float by = (sg * c0) * ya.y + (sg * c1) * YB1[bc].y + (sg * c2) * YB2[bc].y; \
-            accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.x, bx, accr, 0, 0, 0);              \
-            if (SPLIT)                                                                         \
-                acci = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.y, by, acci, 0, 0, 0);          \
-            else                                                                               \
-                accr = __builtin_amdgcn_mfma_f32_16x16x4f32(ya.y, by, accr, 0, 0, 0);          \
-        }                                                                                      \
-    }
-    // step p (p = x0 .. xend): (barrier) [lagging waves: MFMAs of plane p-1, whose values sit in the
-    // YN / DN slots until the filter overwrites them], z taps of plane p+2, y taps of plane p+3, store
-    // of staged plane p+4, load of plane p+5, [leading waves: MFMAs of plane p].  Step xend only
-    // finishes plane xend-1.
-#define AUTO_STEP(YP, YP1, YN, DP, DP1, DN)                                                      \
-    if (p <= xend) {                                                                           \
-        __syncthreads();                                                                       \
-        if (lag && p > x0) AUTO_MFMA(YN, YP, YP1, DN, DP, DP1)                                 \
-        if (p < xend) {                                                                        \
-            zfilter((p + 2) & 1, YN);                                                          \
-            tphase((p + 3) & 1);                                                               \
-            stage_store((p + 4) & 1);                                                          \
-            stage_load(p + 5);                                                                 \
-            DN = dnext;                                                                        \
-            dnext = d1[(int64_t)((p + 3) & m) * LL + col];                                     \
-            if (!lag) AUTO_MFMA(YP, YP1, YN, DP, DP1, DN)                                      \
-        }                                                                                      \
-        ++p;                                                                                   \
-    }
-    for (int p = x0; p <= xend;) {
-        AUTO_STEP(Y0, Y1, Y2, dA, dB, dC)
-        AUTO_STEP(Y1, Y2, Y0, dB, dC, dA)
-        AUTO_STEP(Y2, Y0, Y1, dC, dA, dB)
-        // FP32 sums of at most six planes (432 products per entry) move into the FP64 slots
-        if ((p - x0) % 6 == 0 || p > xend) {
+    const bool lead = ((wv >> 2) ^ wv) & 1;
+    int p = x0 - 2;
+    // steps x0 - 2 and x0 - 1 only start the operands of planes x0 and x0 + 1
+    D2_STEP(false, false, B0, B1, B2, A0, A1, A2, E0, E1, E2)
+    D2_STEP(false, true, B1, B2, B0, A1, A2, A0, E1, E2, E0)
+    while (p < xend) {
+        D2_STEP(true, true, B2, B0, B1, A2, A0, A1, E2, E0, E1)
+        D2_STEP(true, true, B0, B1, B2, A0, A1, A2, E0, E1, E2)
+        D2_STEP(true, true, B1, B2, B0, A1, A2, A0, E1, E2, E0)
+        // FP32 sums of at most six planes (144 products per entry) move into the FP64 slots
+        if ((p - x0) % 6 == 0 || p >= xend) {
 #pragma unroll
             for (int j = 0; j < 4; ++j) {
                 acc_slot[j] += (double)accr[j];
-                if (SPLIT) islot[j * 16] += (double)acci[j];
                 accr[j] = 0.f;
-                acci[j] = 0.f;
             }
         }
     }
-#undef AUTO_STEP
-#undef AUTO_MFMA
+#undef D2_STEP
     // per-wave G blocks through LDS (the tile buffers are free after this barrier), summed in a fixed order
+    constexpr int ROW = SPLIT ? 512 : 256;
+    double* red = (double*)smem;  // [GM::WAVES][256]
     __syncthreads();
-    double* red = (double*)smem;  // [AUTO_WAVES][256]
 #pragma unroll
     for (int j = 0; j < 4; ++j) red[wave * 256 + (4 * (lane / 16) + j) * 16 + (lane % 16)] = acc_slot[j];
     __syncthreads();
-    constexpr int ROW = SPLIT ? 512 : 256;
-    for (int i = tid; i < 256; i += 64 * AUTO_WAVES) {
-        double t = 0.0;
+    for (int i = tid; i < 256; i += 64 * GM::WAVES) {
+        double tsum = 0.0;
 #pragma unroll
-        for (int w = 0; w < AUTO_WAVES; ++w) t += red[w * 256 + i];
-        partial[(int64_t)blockIdx.x * ROW + i] = t;
+        for (int w = 0; w < GM::WAVES; ++w) tsum += red[w * 256 + i];
+        partial[(int64_t)blockIdx.x * ROW + i] = tsum;
         if (SPLIT) {
-            double u = 0.0;
+            double usum = 0.0;
 #pragma unroll
-            for (int w = 0; w < AUTO_WAVES; ++w) u += redi[w * 256 + i];
-            partial[(int64_t)blockIdx.x * ROW + 256 + i] = u;
+            for (int w = 0; w < GM::WAVES; ++w) usum += redi[w * 256 + i];
+            partial[(int64_t)blockIdx.x * ROW + 256 + i] = usum;
         }
     }
 }
@@ -643,7 +664,7 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
     // auto kernel: 16 z x 4 y cells per workgroup (needs L >= 16); cross kernel: 4 z x 4 y
     HERMES_REQUIRE(g.cross || J >= 4, "hermes_multipole_gram_f32: J=%d, the auto Gram needs J >= 4", J);
     const int nblocks = g.cross ? (L / 4) * (L / GRAM_WAVES) * (L / g.xseg)
-                                : (L / AUTO_TZ) * (L / AUTO_TY) * (L / g.xseg);
+                                : (L / D2::TZ) * (L / D2::NROW) * (L / g.xseg);
     HERMES_REQUIRE(!split || !g.cross, "hermes_multipole_gram_f32: split sums are for the auto Gram");
     const int row = split ? 512 : 256;
     double* ws = (double*)hermes_workspace(
@@ -665,14 +686,14 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
         k_gram_f32<true><<<nblocks, 64 * GRAM_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, (const float2*)V, g, ws);
     else
         switch (nneg + (split ? 4 : 0)) {
-            case 0: k_gram_auto<0><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            case 1: k_gram_auto<1><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            case 2: k_gram_auto<2><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            case 3: k_gram_auto<3><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            case 4: k_gram_auto<0, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            case 5: k_gram_auto<1, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            case 6: k_gram_auto<2, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
-            default: k_gram_auto<3, true><<<nblocks, 64 * AUTO_WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 0: k_gram_auto<0><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 1: k_gram_auto<1><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 2: k_gram_auto<2><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 3: k_gram_auto<3><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 4: k_gram_auto<0, true><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 5: k_gram_auto<1, true><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            case 6: k_gram_auto<2, true><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
+            default: k_gram_auto<3, true><<<nblocks, 64 * D2::WAVES, 0, ctx->stream>>>(d1, (const float2*)U, g, ws); break;
         }
     HERMES_LAUNCH_CHECK();
     if (timed) {

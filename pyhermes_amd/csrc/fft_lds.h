@@ -19,6 +19,16 @@
 
 namespace hermes_fft {
 
+// Workgroup barrier ordering LDS accesses only.  __syncthreads() also waits until every global store of
+// the wave has completed, which in a kernel that alternates LDS work with output stores (one field after
+// another) serialises each output phase with the next compute phase; the LDS-scope fences let the stores
+// drain while the waves go on.  Only for barriers that protect LDS data.
+__device__ __forceinline__ void lds_barrier() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+    __builtin_amdgcn_s_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
@@ -124,7 +134,7 @@ struct LdsFft {
                 dft<R, DIR>(v[b]);
             }
         }
-        __syncthreads();
+        lds_barrier();
 #pragma unroll
         for (int b = 0; b < BPT; ++b) {
             const int u = tid + b * NT;
@@ -136,7 +146,7 @@ struct LdsFft {
                 for (int r = 0; r < R; ++r) lds[addr(line, d + r * NS)] = v[b][r];
             }
         }
-        __syncthreads();
+        lds_barrier();
     }
 
     // Full transform; the caller has synchronised after filling lds and tw.

@@ -738,7 +738,7 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
 #pragma unroll
                 for (int mi = 0; mi < MP; ++mi) nyq[tid][mi] = make_float2(j[mi] * wn[mi].x, j[mi] * wn[mi].y);
             }
-            __syncthreads();
+            hermes_fft::lds_barrier();
             if (tid < (PACK ? 1 : MP)) {
                 float2 K[MP];
 #pragma unroll
@@ -760,7 +760,7 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
                 }
             }
         }
-        __syncthreads();
+        hermes_fft::lds_barrier();
         F::run(lds, tw, tid);
         // lines known at compile time: the skip of a mirror line that coincides with line 0 / 2 and the
         // line's base address are uniform, and a line's LDS reads are all issued before its stores
@@ -780,7 +780,7 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
             for (int q = 0; q < EPT; ++q)
                 if (L % NT == 0 || tid + q * NT < L) o[tid + q * NT] = v[q];
         }
-        __syncthreads();
+        hermes_fft::lds_barrier();
     }
 }
 

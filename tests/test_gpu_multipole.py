@@ -178,8 +178,12 @@ def test_packed_m0_window_and_split_gram(phi, L):
         im_only = torch.complex(packed.imag.contiguous(), torch.zeros_like(packed.imag))
         g_re = kernels.multipole_gram_f32(d32, re_only, geo.Q, geo.lam)
         g_im = kernels.multipole_gram_f32(d32, im_only, geo.Q, geo.lam)
-        assert np.array_equal(kernels.gram_block_to_matrix(h[:256], S), g_re)
-        assert np.allclose(kernels.gram_block_to_matrix(h[256:], S), g_im, rtol=1e-6, atol=1e-9 * np.abs(g_im).max())
+        # (the split sums run on one-row tiles, the plain ones on two-row tiles: the per-wave FP32 sums
+        # are the same, the FP64 order across workgroups differs)
+        assert np.allclose(kernels.gram_block_to_matrix(h[:256], S), g_re, rtol=1e-12, atol=1e-14 * np.abs(g_re).max())
+        # (the split kernel moves its imaginary FP32 sums into FP64 every plane, the plain one every six: the
+        # two differ by FP32 rounding of the partial sums)
+        assert np.allclose(kernels.gram_block_to_matrix(h[256:], S), g_im, rtol=1e-6, atol=1e-6 * np.abs(g_im).max())
 
 
 def test_packed_pipeline_matches_unpacked_and_fp64(phi):
