@@ -170,7 +170,7 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     result = np.zeros((n12, n13, n_l), dtype=np.float64)
     if auto and n12 <= 16:
         if not profiled and packs_m0(geo.L, n12, schedule):
-            return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm, m_values)
+            return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm, m_values, overlap=schedule == "overlap")
         return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm, schedule="serial" if profiled else schedule,
                                    m_values=m_values, win=win12)
     blocks12 = _shell_blocks(n12)
@@ -215,9 +215,9 @@ _SIDE = {}
 
 
 def packs_m0(L, n_shell, schedule="serial"):
-    """Whether the batched auto path packs the m = 0 terms of consecutive degrees in pairs (serial
-    schedule, 9..16 shells so that each term has its own Gram, L <= 512)."""
-    return schedule == "serial" and 8 < n_shell <= 16 and L <= 512
+    """Whether the batched auto path packs the m = 0 terms of consecutive degrees in pairs (serial or
+    overlap schedule, 9..16 shells so that each term has its own Gram, L <= 512)."""
+    return schedule in ("serial", "overlap") and 8 < n_shell <= 16 and L <= 512
 
 
 def fields_per_set(l_min, l_max, L, n_shell, schedule="serial"):
@@ -244,36 +244,59 @@ def _packed_units(l_min, l_max):
     return units
 
 
-def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm, m_values=None):
-    """Auto case with 9..16 shells on the serial schedule, the m = 0 terms packed in pairs: the windows of
-    (l, 0) and (l + 1, 0) are real with opposite parity under k -> -k, so one transform yields the even
-    degree's fields as real parts and the odd degree's as imaginary parts, and one Gram with the real and
-    imaginary products summed apart separates them (for l <= 10: 61 field groups per set instead of 66).
-    Every other term is computed as in _ddd_auto_pipelined, one Gram per term."""
+def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm, m_values=None, overlap=False):
+    """Auto case with 9..16 shells, the m = 0 terms packed in pairs: the windows of (l, 0) and (l + 1, 0)
+    are real with opposite parity under k -> -k, so one transform yields the even degree's fields as real
+    parts and the odd degree's as imaginary parts, and one Gram with the real and imaginary products summed
+    apart separates them (for l <= 10: 61 field groups per set instead of 66).  Every other term is computed
+    as in _ddd_auto_pipelined, one Gram per term.
+    overlap: two field buffers; the windows and transforms of unit k + 1 run on the caller's stream while
+    the Grams of unit k run on a second stream (the memory-bound strided FFT passes and the ALU-bound Gram
+    share the CUs: a Gram workgroup takes at most 66 KB of LDS and half of a SIMD's registers)."""
     L = geo.L
     N = L ** 3
     S = r.size
     units = _packed_units(l_min, l_max)
-    U = torch.empty((2 * S, L, L, L), dtype=torch.complex64, device=d1.device)
+    bufs = [torch.empty((2 * S, L, L, L), dtype=torch.complex64, device=d1.device)
+            for _ in range(2 if overlap and len(units) > 1 else 1)]
     n_blocks = sum(1 if kind == "pack0" else len(ms) for kind, _, ms in units)
     blocks = torch.zeros((n_blocks, 512), dtype=torch.float64, device=d1.device)
+    main = torch.cuda.current_stream(d1.device)
+    side = _side_stream(d1.device) if overlap else main
+    side.wait_stream(main)  # d1, H and the buffers are ready on the caller's stream
+    gram_done = []
     owners = []
-    for kind, l, ms in units:
+    for k, (kind, l, ms) in enumerate(units):
+        U = bufs[k % len(bufs)]
+        if overlap and k >= len(bufs):
+            main.wait_event(gram_done[k - len(bufs)])  # the buffer's previous Grams are done
         t = tm.start("window_ifft")
         if kind == "pack0":
             kernels.multipole_window_ifft_pack0_f32(H, geo.P, r, l, 1.0 / N, U[:S])
         else:
             kernels.multipole_window_ifft_multi_f32(H, geo.P, r, l, ms, 1.0 / N, U[:len(ms) * S])
         tm.stop(t)
-        t = tm.start("gram")
-        if kind == "pack0":
-            kernels.multipole_gram_f32_dev_ri(d1, U[:S], geo.Q, geo.lam, blocks[len(owners)])
-            owners.append(("pack0", l, None))
-        else:
-            for i, m in enumerate(ms):
-                kernels.multipole_gram_f32_dev(d1, U[i * S:(i + 1) * S], geo.Q, geo.lam, blocks[len(owners), :256])
-                owners.append(("term", l, m))
-        tm.stop(t)
+        if overlap:
+            win_done = torch.cuda.Event()
+            win_done.record(main)
+        with torch.cuda.stream(side):
+            if overlap:
+                side.wait_event(win_done)
+            t = tm.start("gram")
+            if kind == "pack0":
+                kernels.multipole_gram_f32_dev_ri(d1, U[:S], geo.Q, geo.lam, blocks[len(owners)])
+                owners.append(("pack0", l, None))
+            else:
+                for i, m in enumerate(ms):
+                    kernels.multipole_gram_f32_dev(d1, U[i * S:(i + 1) * S], geo.Q, geo.lam,
+                                                   blocks[len(owners), :256])
+                    owners.append(("term", l, m))
+            tm.stop(t)
+            if overlap:
+                ev = torch.cuda.Event()
+                ev.record(side)
+                gram_done.append(ev)
+    main.wait_stream(side)
     host = blocks.cpu().numpy()
     v_re = {}
     for b, (kind, l, m) in enumerate(owners):
