@@ -180,6 +180,44 @@ def test_corr2pcf_smu_and_rppi_binning(phi):
         assert rc.dd[i, j] == pytest.approx(ref, rel=1e-10)
 
 
+@pytest.mark.parametrize("precision,rtol", [("float64", 1e-10), ("float32", 2e-5)])
+def test_corr2pcf_non_axis_line_of_sight(phi, precision, rtol):
+    """Corr_2PCF with ring (s, mu) and cylinder-shell (rp, pi) binning along the non-axis line of sight
+    los_args = [1, 1, 0] (window_functions.py:164-319, corr2pcf.py:267-322): the mapping and line-of-sight
+    wiring into the fused pair pass end to end, per bin against the oracle's signed 8-alias kernel
+    (convolution.py:73-157), on an FP64 and an FP32 field."""
+    from pyhermes_amd.theory import Corr_2PCF
+    box, J = 300.0, 5
+    L = 1 << J
+    pos, w = _catalogue(30000, box, 25)
+    field = _project_prec(pos, w, J, box, precision)
+    eps = np.asarray(field.epsilon, dtype=np.float64)
+    P = scaling.fourier_power_spectrum(phi, 0, 1, L, 1024)
+    sc = (L / box) ** 6
+    los = [1.0, 1.0, 0.0]
+    s, mu = np.array([40.0, 75.0]), np.array([0.25, 0.8])
+    r = Corr_2PCF({"Corr_2PCF": {"sfc_field": field, "random": "uniform",
+                                 "binning_window": {"type": "ring", "mapping": "smu_to_RH", "los_args": los},
+                                 "sampling": {"s": s, "mu": mu}, "products": ["dd"], "fout_path": ""}}).run(
+        save_result=False)
+    assert r.dd.shape == (2, 2)
+    for i in range(2):
+        for j in range(2):
+            R, H = s[i] * np.sqrt(1 - mu[j] ** 2) * L / box, s[i] * mu[j] * L / box
+            ref = ow.pair_product(eps, eps, ow.los_rfft_kernel(L, P, "ring", R, H, los)) * sc
+            assert r.dd[i, j] == pytest.approx(ref, rel=rtol), (i, j)
+    rp, pi = np.array([30.0, 60.0]), np.array([10.0, 45.0])
+    rc = Corr_2PCF({"Corr_2PCF": {"sfc_field": field, "random": "uniform",
+                                  "binning_window": {"type": "cylshell", "mapping": "rppi_to_RH", "los_args": los},
+                                  "sampling": {"rp": rp, "pi": pi}, "products": ["dd"], "fout_path": ""}}).run(
+        save_result=False)
+    for i in range(2):
+        for j in range(2):
+            ref = ow.pair_product(eps, eps, ow.los_rfft_kernel(L, P, "cylshell", rp[i] * L / box, pi[j] * L / box,
+                                                                los)) * sc
+            assert rc.dd[i, j] == pytest.approx(ref, rel=rtol), (i, j)
+
+
 def _multipole_case(phi, backend, products):
     from pyhermes_amd.theory import Corr_3PCF_Multipole
     box, J = 160.0, 4
