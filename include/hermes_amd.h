@@ -273,6 +273,22 @@ int hermes_multipole_window_zpass_multi_f32(hermes_ctx *ctx, int J, const void *
                                             const double *radii, int n_shell, int l, const int *ms_host, int n_m,
                                             double scale, void *out);
 int hermes_fft3_inverse_yx_f32(hermes_ctx *ctx, int J, int batch, void *data);
+/* The z pass of hermes_multipole_window_ifft_pack0_f32 alone (out: n_shell fields transformed along z). */
+int hermes_multipole_window_zpass_pack0_f32(hermes_ctx *ctx, int J, const void *H, const double *P,
+                                            const double *radii, int n_shell, int l, double scale, void *out);
+/* The y and x inverse passes of a batch of x-major grids `in` already transformed along z, written to the
+ * separate grids `out` in y-major order: element (x, y, z) at (y L + x) L + z (in is left transformed along
+ * y only).  The y lines are read 4 KB apart at L = 512 and written transposed, and the x lines of the result
+ * are 4 KB apart too: strided writes cost the HBM less than the 2 MB-strided reads of the x-major x pass.
+ * For the Gram contractions with hermes_set_field_layout(ctx, 1).  No counterpart. */
+int hermes_fft3_inverse_yx_ymajor_f32(hermes_ctx *ctx, int J, int batch, const void *in, void *out);
+
+/* Layout of the fields U, V that hermes_multipole_gram_f32* read: 0 (default) x-major (L, L, L), element
+ * (x, y, z) at (x L + y) L + z; 1 y-major, at (y L + x) L + z (the output of
+ * hermes_fft3_inverse_yx_ymajor_f32).  d1 is always x-major.  The fields are internal to the SH-3PCF
+ * contraction, so their order is free.  No counterpart. */
+int hermes_set_field_layout(hermes_ctx *ctx, int layout);
+int hermes_get_field_layout(hermes_ctx *ctx, int *layout_out);
 
 /* Deterministic FP64 sum of n real (ncomp = 1) or complex (ncomp = 2) values,
  * stored as float (dbl = 0) or double (dbl = 1); result to host.  Replaces the

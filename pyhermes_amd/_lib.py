@@ -34,6 +34,9 @@ _SIGNATURES = {
     "hermes_fft3_c2c_f32": [c_void_p, c_int, c_int, c_int, c_void_p],
     "hermes_fft3_tile_choice": [c_void_p, c_int, c_int, ctypes.POINTER(c_int), ctypes.POINTER(c_int)],
     "hermes_fft3_set_tile": [c_void_p, c_int, c_int, c_int, c_int],
+    "hermes_fft3_inverse_yx_ymajor_f32": [c_void_p, c_int, c_int, c_void_p, c_void_p],
+    "hermes_set_field_layout": [c_void_p, c_int],
+    "hermes_get_field_layout": [c_void_p, ctypes.POINTER(c_int)],
     "hermes_profile_gram": [c_void_p, c_int, ctypes.POINTER(c_double), ctypes.POINTER(c_i64)],
     "hermes_capture_begin": [c_void_p],
     "hermes_capture_end": [c_void_p, ctypes.POINTER(c_void_p)],
@@ -90,6 +93,8 @@ _SIGNATURES.update({
     "hermes_multipole_gram_f32_dev_ri": [c_void_p, c_int, c_void_p, c_void_p, c_int, _dpp, _dpp, c_void_p],
     "hermes_multipole_window_ifft_pack0_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_double,
                                                c_void_p],
+    "hermes_multipole_window_zpass_pack0_f32": [c_void_p, c_int, c_void_p, c_void_p, _dpp, c_int, c_int, c_double,
+                                                c_void_p],
     "hermes_pair_plan_create": [c_void_p, c_int, c_int, ctypes.POINTER(WindowDesc), c_int, ctypes.POINTER(c_void_p)],
     "hermes_pair_plan_destroy": [c_void_p],
     "hermes_pair_reduce_planned": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p],

@@ -130,6 +130,7 @@ static void sum_rows(hipStream_t st, const double* in, int64_t nb, int width, do
 // ---------------------------------------------------------------- batched FP32 MFMA Gram
 struct GramArgs {
     int L, S1, S2, cross, xseg;
+    int xs, ys;       // element strides of x and y in the fields U, V (x-major: L^2, L; y-major: L, L^2)
     float vsign;      // sign applied to V's imaginary part in the cross case (-1: Re(U V))
     float Q[9];       // Q[tap * 3 + col]
     float lam[27];    // lam_a lam_b lam_c, index (a * 3 + b) * 3 + c
@@ -143,11 +144,12 @@ constexpr int GRAM_WAVES = 4;
 typedef float float4_t __attribute__((ext_vector_type(4)));
 
 // Raw values of field f at plane x around cell (y, z): raw[ly * 3 + lz] = f[x, y-ly, z-lz].
-__device__ __forceinline__ void load_plane(const float2* __restrict__ f, int L, int x, int y, int z, float2 raw[9]) {
-    const int m = L - 1;
+__device__ __forceinline__ void load_plane(const float2* __restrict__ f, const GramArgs& g, int x, int y, int z,
+                                           float2 raw[9]) {
+    const int m = g.L - 1;
 #pragma unroll
     for (int ly = 0; ly < 3; ++ly) {
-        const float2* row = f + ((int64_t)x * L + ((y - ly) & m)) * L;
+        const float2* row = f + (int64_t)x * g.xs + (int64_t)((y - ly) & m) * g.ys;
 #pragma unroll
         for (int lz = 0; lz < 3; ++lz) raw[ly * 3 + lz] = row[(z - lz) & m];
     }
@@ -233,16 +235,16 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
     // warm-up: P1 = Y(x0-2), P2 = Y(x0-1)
 #pragma unroll
     for (int i = 0; i < 9; ++i) { rawU[i] = zero; rawV[i] = zero; }
-    if (ua) load_plane(fu, L, (x0 - 2) & (L - 1), y, z, rawU);
-    if (CROSS && vb) load_plane(fv, L, (x0 - 2) & (L - 1), y, z, rawV);
+    if (ua) load_plane(fu, g, (x0 - 2) & (L - 1), y, z, rawU);
+    if (CROSS && vb) load_plane(fv, g, (x0 - 2) & (L - 1), y, z, rawV);
     filter_plane(rawU, Q, P1);
     if (CROSS) filter_plane(rawV, Q, W1);
-    if (ua) load_plane(fu, L, (x0 - 1) & (L - 1), y, z, rawU);
-    if (CROSS && vb) load_plane(fv, L, (x0 - 1) & (L - 1), y, z, rawV);
+    if (ua) load_plane(fu, g, (x0 - 1) & (L - 1), y, z, rawU);
+    if (CROSS && vb) load_plane(fv, g, (x0 - 1) & (L - 1), y, z, rawV);
     filter_plane(rawU, Q, P2);
     if (CROSS) filter_plane(rawV, Q, W2);
-    if (ua) load_plane(fu, L, x0, y, z, rawU);
-    if (CROSS && vb) load_plane(fv, L, x0, y, z, rawV);
+    if (ua) load_plane(fu, g, x0, y, z, rawU);
+    if (CROSS && vb) load_plane(fv, g, x0, y, z, rawV);
     float dnext = d1[((int64_t)x0 * L + y) * L + z];  // d at plane x, loaded one step ahead
 
     double acc64[4] = {0.0, 0.0, 0.0, 0.0};
@@ -253,8 +255,8 @@ __global__ __launch_bounds__(64 * GRAM_WAVES) void k_gram_f32(const float* __res
         if (CROSS) filter_plane(rawV, Q, WCUR);                                                      \
         const float dval = dnext;                                                                    \
         if (x + 1 < xend) {                                                                          \
-            if (ua) load_plane(fu, L, x + 1, y, z, rawU);                                            \
-            if (CROSS && vb) load_plane(fv, L, x + 1, y, z, rawV);                                   \
+            if (ua) load_plane(fu, g, x + 1, y, z, rawU);                                            \
+            if (CROSS && vb) load_plane(fv, g, x + 1, y, z, rawV);                                   \
             dnext = d1[((int64_t)(x + 1) * L + y) * L + z];                                          \
         }                                                                                            \
         float4_t accr = {0.f, 0.f, 0.f, 0.f}, acci = {0.f, 0.f, 0.f, 0.f};                           \
@@ -370,14 +372,14 @@ __global__ __launch_bounds__(64 * D2::WAVES, 4) void k_gram_auto(const float* __
         act[i] = e < GM::STAGE;
         const int ee = act[i] ? e : 0;
         const int sh = ee / (GM::RY * GM::RZ), ry = (ee / GM::RZ) % GM::RY, rz = ee % GM::RZ;
-        src[i] = U + (int64_t)(sh < g.S1 ? sh : 0) * N + (int64_t)((y0 - 2 + ry) & m) * L + ((z0 - 2 + rz) & m);
+        src[i] = U + (int64_t)(sh < g.S1 ? sh : 0) * N + (int64_t)((y0 - 2 + ry) & m) * g.ys + ((z0 - 2 + rz) & m);
         dst[i] = (ry * GM::RZ + rz) * GM::RP + (sh ^ (rz & 15));
     }
     float2 stg[SPT];
     auto stage_load = [&](int plane) {
 #pragma unroll
         for (int i = 0; i < SPT; ++i)
-            if (act[i]) stg[i] = src[i][(int64_t)(plane & m) * LL];
+            if (act[i]) stg[i] = src[i][(int64_t)(plane & m) * g.xs];
     };
     auto stage_store = [&](int plane) {
 #pragma unroll
@@ -629,6 +631,8 @@ static int gram_launch(hermes_ctx* ctx, int J, const float* d1, const void* U, i
     const int L = 1 << J;
     GramArgs g;
     g.L = L;
+    g.xs = ctx->field_layout ? L : L * L;  // x-major fields: (x L + y) L + z; y-major: (y L + x) L + z
+    g.ys = ctx->field_layout ? L * L : L;
     g.S1 = n_u;
     g.cross = V != nullptr;
     g.S2 = g.cross ? n_v : n_u;

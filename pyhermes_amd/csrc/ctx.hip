@@ -124,6 +124,19 @@ int hermes_ctx_set_stream(hermes_ctx* ctx, void* stream) {
     return 0;
 }
 
+int hermes_set_field_layout(hermes_ctx* ctx, int layout) {
+    HERMES_REQUIRE(ctx != nullptr, "null context");
+    HERMES_REQUIRE(layout == 0 || layout == 1, "hermes_set_field_layout: layout %d is neither 0 nor 1", layout);
+    ctx->field_layout = layout;
+    return 0;
+}
+
+int hermes_get_field_layout(hermes_ctx* ctx, int* layout_out) {
+    HERMES_REQUIRE(ctx != nullptr && layout_out != nullptr, "hermes_get_field_layout: null argument");
+    *layout_out = ctx->field_layout;
+    return 0;
+}
+
 int hermes_capture_begin(hermes_ctx* ctx) {
     HERMES_REQUIRE(ctx != nullptr, "null context");
     HERMES_REQUIRE(!ctx->capturing, "hermes_capture_begin: already capturing");

@@ -78,11 +78,14 @@ __global__ __launch_bounds__(FFT_THREADS) void k_fft_rows(float2* __restrict__ d
 }
 
 // Pass over strided lines.  Tile t = (outer, mid, zt): the lines at z = zt*TW .. zt*TW+TW-1 of one
-// position `mid` on the other strided axis; element e of a line at e * axis_stride (< 2^31 for
-// L <= 1024).  LDS (dynamic) holds the L twiddles and TW * L complex values, element-major.
+// position `mid` on the other strided axis; element e of a line is read at in + e * es_in + mid * ms_in and
+// written at out + e * es_out + mid * ms_out (plus the grid and z offsets; < 2^31 within a grid for
+// L <= 1024).  in == out with equal geometries is the in-place pass (a workgroup reads its whole tile before
+// it writes); otherwise the two grids do not overlap, and the write geometry may transpose the grid.
+// LDS (dynamic) holds the L twiddles and TW * L complex values, element-major.
 template <int LOG2L, int TW, int NT, int DIR>
-__global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, const float2* __restrict__ twg,
-                                                 int axis_stride, int64_t mid_stride, int n_mid,
+__global__ __launch_bounds__(NT) void k_fft_cols(const float2* in, float2* out, const float2* __restrict__ twg,
+                                                 int es_in, int64_t ms_in, int es_out, int64_t ms_out, int n_mid,
                                                  int64_t outer_stride, int ntile_z) {
     constexpr int L = 1 << LOG2L;
     static_assert(NT % TW == 0, "a thread keeps one z line of the tile");
@@ -101,9 +104,10 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, cons
     const int mid = (int)(b % n_mid);
     const int64_t outer = b / n_mid;
     // this thread's z line is fixed (NT is a multiple of TW); its elements are NT / TW apart
-    float2* base = data + outer * outer_stride + (int64_t)mid * mid_stride + (zt * TW + tid % TW) +
-                   (int64_t)(tid / TW) * axis_stride;
-    const int64_t estep = (int64_t)(NT / TW) * axis_stride;
+    const int64_t zoff = outer * outer_stride + (zt * TW + tid % TW);
+    const float2* src = in + zoff + (int64_t)mid * ms_in + (int64_t)(tid / TW) * es_in;
+    float2* dst = out + zoff + (int64_t)mid * ms_out + (int64_t)(tid / TW) * es_out;
+    const int64_t sstep = (int64_t)(NT / TW) * es_in, dstep = (int64_t)(NT / TW) * es_out;
     load_twiddles<LOG2L>(tw, twg, tid, NT);
     // every load of the thread is issued before the first LDS store (TW * L / NT of them), so a
     // workgroup keeps its whole tile in flight: the pass is bound by HBM latency otherwise
@@ -112,7 +116,7 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, cons
     float2 v[PER];
 #pragma unroll
     for (int k = 0; k < PER; ++k)
-        if (EVEN || tid + k * NT < TW * L) v[k] = base[k * estep];
+        if (EVEN || tid + k * NT < TW * L) v[k] = src[k * sstep];
 #pragma unroll
     for (int k = 0; k < PER; ++k)
         if (EVEN || tid + k * NT < TW * L) lds[tid + k * NT] = v[k];
@@ -120,7 +124,7 @@ __global__ __launch_bounds__(NT) void k_fft_cols(float2* __restrict__ data, cons
     F::run(lds, tw, tid);
 #pragma unroll
     for (int k = 0; k < PER; ++k)
-        if (EVEN || tid + k * NT < TW * L) base[k * estep] = lds[tid + k * NT];
+        if (EVEN || tid + k * NT < TW * L) dst[k * dstep] = lds[tid + k * NT];
 }
 
 template <int LOG2L>
@@ -143,9 +147,11 @@ struct ColVariant {
     static constexpr int NT = col_nt_of(V);
 };
 
-// One strided pass (axis 0: y lines, axis 1: x lines) of a batch of grids with tile variant V.
+// One strided pass of a batch of grids with tile variant V.  Axis 0: y lines of x-major grids (element
+// (x, y, z) at (x L + y) L + z), in place; 1: their x lines, in place; 2: y lines from x-major in to y-major
+// out (element (x, y, z) at (y L + x) L + z), out of place; 3: x lines of y-major grids, in place.
 template <int LOG2L, int V, int DIR>
-int launch_cols_variant(hermes_ctx* ctx, int batch, float2* data, const float2* twg, int axis) {
+int launch_cols_variant(hermes_ctx* ctx, int batch, const float2* in, float2* out, const float2* twg, int axis) {
     constexpr int L = 1 << LOG2L;
     constexpr int TW = ColVariant<LOG2L, V>::TW, NT = ColVariant<LOG2L, V>::NT;
     const size_t shmem = sizeof(float2) * (size_t)(L + TW * L);
@@ -153,24 +159,33 @@ int launch_cols_variant(hermes_ctx* ctx, int batch, float2* data, const float2* 
     HERMES_CHECK_HIP(hipFuncSetAttribute((const void*)kc, hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem));
     const int ntz = L / TW;
     const unsigned nblk = (unsigned)((int64_t)batch * L * ntz);
-    if (axis == 0)  // y lines: tiles (b * L + x, zt)
-        kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L, (int64_t)L * L, L, (int64_t)L * L * L, ntz);
-    else  // x lines: tiles (b, y, zt)
-        kc<<<nblk, NT, shmem, ctx->stream>>>(data, twg, L * L, L, L, (int64_t)L * L * L, ntz);
+    const int64_t LL = (int64_t)L * L, grid = LL * L;
+    switch (axis) {
+        case 0:  // tiles (b * L + x, zt)
+        case 3:  // tiles (b * L + y, zt) of y-major grids: the same strides
+            kc<<<nblk, NT, shmem, ctx->stream>>>(in, out, twg, L, LL, L, LL, L, grid, ntz);
+            break;
+        case 1:  // tiles (b, y, zt)
+            kc<<<nblk, NT, shmem, ctx->stream>>>(in, out, twg, (int)LL, L, (int)LL, L, L, grid, ntz);
+            break;
+        default:  // 2: tiles (b * L + x, zt), lines read along y, written along the y-major grid's y
+            kc<<<nblk, NT, shmem, ctx->stream>>>(in, out, twg, L, LL, (int)LL, L, L, grid, ntz);
+            break;
+    }
     HERMES_LAUNCH_CHECK();
     return 0;
 }
 
 template <int LOG2L, int DIR>
-int launch_cols(hermes_ctx* ctx, int variant, int batch, float2* data, const float2* twg, int axis) {
-    if (LOG2L < 7) return launch_cols_variant<LOG2L, 0, DIR>(ctx, batch, data, twg, axis);
+int launch_cols(hermes_ctx* ctx, int variant, int batch, const float2* in, float2* out, const float2* twg, int axis) {
+    if (LOG2L < 7) return launch_cols_variant<LOG2L, 0, DIR>(ctx, batch, in, out, twg, axis);
     switch (variant) {
-        case 1: return launch_cols_variant<LOG2L, 1, DIR>(ctx, batch, data, twg, axis);
-        case 2: return launch_cols_variant<LOG2L, 2, DIR>(ctx, batch, data, twg, axis);
-        case 3: return launch_cols_variant<LOG2L, 3, DIR>(ctx, batch, data, twg, axis);
-        case 4: return launch_cols_variant<LOG2L, 4, DIR>(ctx, batch, data, twg, axis);
-        case 5: return launch_cols_variant<LOG2L, 5, DIR>(ctx, batch, data, twg, axis);
-        default: return launch_cols_variant<LOG2L, 0, DIR>(ctx, batch, data, twg, axis);
+        case 1: return launch_cols_variant<LOG2L, 1, DIR>(ctx, batch, in, out, twg, axis);
+        case 2: return launch_cols_variant<LOG2L, 2, DIR>(ctx, batch, in, out, twg, axis);
+        case 3: return launch_cols_variant<LOG2L, 3, DIR>(ctx, batch, in, out, twg, axis);
+        case 4: return launch_cols_variant<LOG2L, 4, DIR>(ctx, batch, in, out, twg, axis);
+        case 5: return launch_cols_variant<LOG2L, 5, DIR>(ctx, batch, in, out, twg, axis);
+        default: return launch_cols_variant<LOG2L, 0, DIR>(ctx, batch, in, out, twg, axis);
     }
 }
 
@@ -221,10 +236,10 @@ int tuned_variants(hermes_ctx* ctx, const float2* twg, int out[2]) {
     for (int v = 0; v < N_COL_VARIANTS && rc == 0; ++v) {
         for (int axis = 0; axis < 2 && rc == 0; ++axis) {
             float t[3];
-            rc = launch_cols<LOG2L, DIR>(ctx, v, nf, scratch, twg, axis);  // warm-up
+            rc = launch_cols<LOG2L, DIR>(ctx, v, nf, scratch, scratch, twg, axis);  // warm-up
             for (int r = 0; r < 3 && rc == 0; ++r) {
                 if (hipEventRecord(e0, ctx->stream) != hipSuccess) rc = 1;
-                if (rc == 0) rc = launch_cols<LOG2L, DIR>(ctx, v, nf, scratch, twg, axis);
+                if (rc == 0) rc = launch_cols<LOG2L, DIR>(ctx, v, nf, scratch, scratch, twg, axis);
                 if (rc == 0 && hipEventRecord(e1, ctx->stream) != hipSuccess) rc = 1;
                 if (rc == 0 && hipEventSynchronize(e1) != hipSuccess) rc = 1;
                 if (rc == 0 && hipEventElapsedTime(&t[r], e0, e1) != hipSuccess) rc = 1;
@@ -259,9 +274,23 @@ int run_yx(hermes_ctx* ctx, int batch, float2* data, const float2* twg) {
     int v[2] = {0, 0};
     int rc = tuned_variants<LOG2L, DIR>(ctx, twg, v);
     if (rc) return rc;
-    rc = launch_cols<LOG2L, DIR>(ctx, v[0], batch, data, twg, 0);
+    rc = launch_cols<LOG2L, DIR>(ctx, v[0], batch, data, data, twg, 0);
     if (rc) return rc;
-    return launch_cols<LOG2L, DIR>(ctx, v[1], batch, data, twg, 1);
+    return launch_cols<LOG2L, DIR>(ctx, v[1], batch, data, data, twg, 1);
+}
+
+// Strided passes from x-major grids `in` (transformed along z) to y-major grids `out`: the y lines are read
+// along x-major rows 4 KB apart (L = 512) and written transposed, and the x lines of the result are then 4 KB
+// apart as well; strided writes cost the HBM far less than strided reads.  Both passes take the y lines'
+// tuned tile (the same read geometry).
+template <int LOG2L, int DIR>
+int run_yx_ymajor(hermes_ctx* ctx, int batch, const float2* in, float2* out, const float2* twg) {
+    int v[2] = {0, 0};
+    int rc = tuned_variants<LOG2L, DIR>(ctx, twg, v);
+    if (rc) return rc;
+    rc = launch_cols<LOG2L, DIR>(ctx, v[0], batch, in, out, twg, 2);
+    if (rc) return rc;
+    return launch_cols<LOG2L, DIR>(ctx, v[0], batch, out, out, twg, 3);
 }
 
 template <int LOG2L, int DIR>
@@ -331,6 +360,30 @@ int hermes_fft3_inverse_yx_f32(hermes_ctx* ctx, int J, int batch, void* data) {
     HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_fft3_inverse_yx_f32: J=%d outside 3..10", J);
     HERMES_REQUIRE(batch >= 1, "hermes_fft3_inverse_yx_f32: batch=%d", batch);
     return hermes_fft3_inverse_yx(ctx, J, batch, (float2*)data);
+}
+
+int hermes_fft3_inverse_yx_ymajor_f32(hermes_ctx* ctx, int J, int batch, const void* in, void* out) {
+    HERMES_REQUIRE(ctx && in && out, "hermes_fft3_inverse_yx_ymajor_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 10, "hermes_fft3_inverse_yx_ymajor_f32: J=%d outside 3..10", J);
+    HERMES_REQUIRE(batch >= 1, "hermes_fft3_inverse_yx_ymajor_f32: batch=%d", batch);
+    const size_t bytes = sizeof(float2) * ((size_t)batch << (3 * J));
+    const char *a = (const char*)in, *b = (const char*)out;
+    HERMES_REQUIRE(a + bytes <= b || b + bytes <= a, "hermes_fft3_inverse_yx_ymajor_f32: in and out overlap");
+    HERMES_CHECK_HIP(hipSetDevice(ctx->device));
+    const float2* twg = hermes_twiddles(ctx, J, 1);
+    HERMES_REQUIRE(twg, "%s", hermes_last_error());
+    const float2* src = (const float2*)in;
+    float2* dst = (float2*)out;
+    switch (J) {
+        case 3: return run_yx_ymajor<3, 1>(ctx, batch, src, dst, twg);
+        case 4: return run_yx_ymajor<4, 1>(ctx, batch, src, dst, twg);
+        case 5: return run_yx_ymajor<5, 1>(ctx, batch, src, dst, twg);
+        case 6: return run_yx_ymajor<6, 1>(ctx, batch, src, dst, twg);
+        case 7: return run_yx_ymajor<7, 1>(ctx, batch, src, dst, twg);
+        case 8: return run_yx_ymajor<8, 1>(ctx, batch, src, dst, twg);
+        case 9: return run_yx_ymajor<9, 1>(ctx, batch, src, dst, twg);
+        default: return run_yx_ymajor<10, 1>(ctx, batch, src, dst, twg);
+    }
 }
 
 int hermes_fft3_tile_choice(hermes_ctx* ctx, int J, int inverse, int* tw_out, int* threads_out) {

@@ -22,6 +22,8 @@ struct hermes_ctx {
     float2* twiddles = nullptr;  // e^{+-2 pi i k / 2^j} tables for the LDS FFTs (first use)
     // autotuned strided-pass tile per (log2 L, inverse, axis: 0 = y lines, 1 = x lines); 0 = not tuned
     int fft_cols_variant[11][2][2] = {};
+    // layout of the FP32 fields the Gram contractions read (hermes_set_field_layout): 0 = x-major, 1 = y-major
+    int field_layout = 0;
     // optional HIP-event timing of the Gram contraction kernel (hermes_profile_gram)
     bool time_gram = false;
     std::vector<hipEvent_t> gram_ev;  // (start, end) per timed launch, summed when queried

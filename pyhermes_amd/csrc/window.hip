@@ -1399,11 +1399,11 @@ static int mpwin_zpass(hermes_ctx* ctx, int J, const void* H, const double* P, c
     return 0;
 }
 
-int hermes_multipole_window_ifft_pack0_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
-                                           const double* radii, int n_shell, int l, double scale, void* out) {
-    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_window_ifft_pack0_f32: null argument");
-    HERMES_REQUIRE(J >= 3 && J <= 9, "hermes_multipole_window_ifft_pack0_f32: J=%d outside 3..9", J);
-    HERMES_REQUIRE(l >= 0 && l < 40, "hermes_multipole_window_ifft_pack0_f32: l=%d", l);
+int hermes_multipole_window_zpass_pack0_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
+                                            const double* radii, int n_shell, int l, double scale, void* out) {
+    HERMES_REQUIRE(ctx && H && P && radii && out, "hermes_multipole_window_zpass_pack0_f32: null argument");
+    HERMES_REQUIRE(J >= 3 && J <= 9, "hermes_multipole_window_zpass_pack0_f32: J=%d outside 3..9", J);
+    HERMES_REQUIRE(l >= 0 && l < 40, "hermes_multipole_window_zpass_pack0_f32: l=%d", l);
     ApplyArgs A;
     float* Pf = nullptr;
     int rc = make_apply_args(ctx, J, P, radii, n_shell, l, 0, scale, &A, &Pf);
@@ -1435,7 +1435,14 @@ int hermes_multipole_window_ifft_pack0_f32(hermes_ctx* ctx, int J, const void* H
     }
 #undef HERMES_MPWP
     HERMES_LAUNCH_CHECK();
-    return hermes_fft3_inverse_yx(ctx, J, n_shell, o);
+    return 0;
+}
+
+int hermes_multipole_window_ifft_pack0_f32(hermes_ctx* ctx, int J, const void* H, const double* P,
+                                           const double* radii, int n_shell, int l, double scale, void* out) {
+    const int rc = hermes_multipole_window_zpass_pack0_f32(ctx, J, H, P, radii, n_shell, l, scale, out);
+    if (rc) return rc;
+    return hermes_fft3_inverse_yx(ctx, J, n_shell, (float2*)out);
 }
 
 int hermes_multipole_window_ifft_multi_f32(hermes_ctx* ctx, int J, const void* H, const double* P,

@@ -306,6 +306,54 @@ def multipole_window_zpass_multi_f32(H, P, radii_grid, l, ms, scale, out):
     return out
 
 
+def fft3_inverse_yx_ymajor_f32(src, dst):
+    """The y and x inverse passes of the x-major batch ``src`` (z lines already transformed) into ``dst`` in
+    y-major order (dst[..., y, x, z] holds the transform at (x, y, z)); src is left transformed along y only."""
+    for t in (src, dst):
+        if t.dtype != torch.complex64 or not t.is_contiguous() or t.ndim < 3:
+            raise ValueError("fft3_inverse_yx_ymajor_f32 needs contiguous complex64 (.., L, L, L) tensors")
+    if src.shape != dst.shape:
+        raise ValueError(f"fft3_inverse_yx_ymajor_f32: shapes {tuple(src.shape)} and {tuple(dst.shape)} differ")
+    L = src.shape[-1]
+    batch = int(np.prod(src.shape[:-3])) if src.ndim > 3 else 1
+    _lib.call("hermes_fft3_inverse_yx_ymajor_f32", _lib.context(), _J_of(L), batch, ptr(src), ptr(dst))
+    return dst
+
+
+FIELD_LAYOUTS = {"xmajor": 0, "ymajor": 1}
+
+
+def get_field_layout():
+    """Layout of the fields the Gram contractions read: "xmajor" (default) or "ymajor"."""
+    v = ctypes.c_int(0)
+    _lib.call("hermes_get_field_layout", _lib.context(), ctypes.byref(v))
+    return {c: n for n, c in FIELD_LAYOUTS.items()}[v.value]
+
+
+def set_field_layout(name):
+    """Select the layout of the fields U, V read by the multipole_gram_f32* contractions."""
+    if name not in FIELD_LAYOUTS:
+        raise ValueError(f"field layout {name!r} is not one of {sorted(FIELD_LAYOUTS)}")
+    _lib.call("hermes_set_field_layout", _lib.context(), FIELD_LAYOUTS[name])
+
+
+class field_layout:
+    """Context manager: the Gram contractions read fields in layout ``name`` inside the block."""
+
+    def __init__(self, name):
+        self.name = name
+        self.prev = None
+
+    def __enter__(self):
+        self.prev = get_field_layout()
+        set_field_layout(self.name)
+        return self
+
+    def __exit__(self, *exc):
+        set_field_layout(self.prev)
+        return False
+
+
 def fft3_inverse_yx_f32(x):
     """The y and x inverse passes (unnormalised) of a contiguous complex64 (.., L, L, L) batch whose z
     lines are already transformed."""
@@ -386,6 +434,16 @@ def multipole_window_ifft_pack0_f32(H, P, radii_grid, l, scale, out):
     S = r.size
     L = _check_multipole_inputs(H, P, S, out)
     _lib.call("hermes_multipole_window_ifft_pack0_f32", _lib.context(), _J_of(L), ptr(H), ptr(P),
+              r.ctypes.data_as(_DP), S, int(l), float(scale), ptr(out))
+    return out
+
+
+def multipole_window_zpass_pack0_f32(H, P, radii_grid, l, scale, out):
+    """The z pass of multipole_window_ifft_pack0_f32 alone (out: S fields transformed along z only)."""
+    r = _np_f64(radii_grid)
+    S = r.size
+    L = _check_multipole_inputs(H, P, S, out)
+    _lib.call("hermes_multipole_window_zpass_pack0_f32", _lib.context(), _J_of(L), ptr(H), ptr(P),
               r.ctypes.data_as(_DP), S, int(l), float(scale), ptr(out))
     return out
 

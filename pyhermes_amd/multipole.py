@@ -148,11 +148,15 @@ def _same_edges(a, b):
 
 
 def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min, l_max, timer=None,
-                              schedule="serial", m_values=None):
+                              schedule="serial", m_values=None, field_order="ymajor"):
     """All (r12, r13) shell pairs: returns (n12, n13, n_l) float64.  d*: (L,L,L) float32 device tensors.
     ``schedule`` picks the stream schedule of the auto case with at most 16 shells (see
     _ddd_auto_pipelined): "serial", "split" or "overlap".  A dict passed as ``m_values`` receives
-    Re v_{l,m} (n12 x n13, m >= 0) per (l, m), the terms combine_multipole_m_terms sums."""
+    Re v_{l,m} (n12 x n13, m >= 0) per (l, m), the terms combine_multipole_m_terms sums.  ``field_order``: the
+    packed auto path transforms its shell fields into y-major order ("ymajor", the default: the strided
+    inverse-FFT passes then read along 4 KB strides only) or in place in x-major order ("xmajor")."""
+    if field_order not in ("ymajor", "xmajor"):
+        raise ValueError(f"field_order {field_order!r} is neither 'ymajor' nor 'xmajor'")
     L, J = geo.L, geo.J
     tm = timer if timer is not None else _NoTimer()
     N = L ** 3
@@ -170,7 +174,8 @@ def ddd_multipole_batched_f32(geo, d1, d2, d3, radii12_grid, radii13_grid, l_min
     result = np.zeros((n12, n13, n_l), dtype=np.float64)
     if auto and n12 <= 16:
         if not profiled and packs_m0(geo.L, n12, schedule):
-            return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm, m_values, overlap=schedule == "overlap")
+            return _ddd_auto_packed(geo, d1, H2, r12, l_min, l_max, tm, m_values, overlap=schedule == "overlap",
+                                    ymajor=field_order == "ymajor")
         return _ddd_auto_pipelined(geo, d1, H2, r12, l_min, l_max, tm, schedule="serial" if profiled else schedule,
                                    m_values=m_values, win=win12)
     blocks12 = _shell_blocks(n12)
@@ -244,7 +249,7 @@ def _packed_units(l_min, l_max):
     return units
 
 
-def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm, m_values=None, overlap=False):
+def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm, m_values=None, overlap=False, ymajor=True):
     """Auto case with 9..16 shells, the m = 0 terms packed in pairs: the windows of (l, 0) and (l + 1, 0)
     are real with opposite parity under k -> -k, so one transform yields the even degree's fields as real
     parts and the odd degree's as imaginary parts, and one Gram with the real and imaginary products summed
@@ -252,13 +257,17 @@ def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm, m_values=None, overlap=Fal
     as in _ddd_auto_pipelined, one Gram per term.
     overlap: two field buffers; the windows and transforms of unit k + 1 run on the caller's stream while
     the Grams of unit k run on a second stream (the memory-bound strided FFT passes and the ALU-bound Gram
-    share the CUs: a Gram workgroup takes at most 66 KB of LDS and half of a SIMD's registers)."""
+    share the CUs: a Gram workgroup takes at most 66 KB of LDS and half of a SIMD's registers).
+    ymajor: the windows' z pass writes x-major fields into a staging buffer and the strided passes move them
+    into y-major order in the buffer the Grams read (hermes_fft3_inverse_yx_ymajor_f32); otherwise the
+    strided passes run in place."""
     L = geo.L
     N = L ** 3
     S = r.size
     units = _packed_units(l_min, l_max)
     bufs = [torch.empty((2 * S, L, L, L), dtype=torch.complex64, device=d1.device)
             for _ in range(2 if overlap and len(units) > 1 else 1)]
+    stage = torch.empty((2 * S, L, L, L), dtype=torch.complex64, device=d1.device) if ymajor else None
     n_blocks = sum(1 if kind == "pack0" else len(ms) for kind, _, ms in units)
     blocks = torch.zeros((n_blocks, 512), dtype=torch.float64, device=d1.device)
     main = torch.cuda.current_stream(d1.device)
@@ -271,15 +280,22 @@ def _ddd_auto_packed(geo, d1, H, r, l_min, l_max, tm, m_values=None, overlap=Fal
         if overlap and k >= len(bufs):
             main.wait_event(gram_done[k - len(bufs)])  # the buffer's previous Grams are done
         t = tm.start("window_ifft")
-        if kind == "pack0":
+        n = S if kind == "pack0" else len(ms) * S
+        if ymajor:
+            if kind == "pack0":
+                kernels.multipole_window_zpass_pack0_f32(H, geo.P, r, l, 1.0 / N, stage[:S])
+            else:
+                kernels.multipole_window_zpass_multi_f32(H, geo.P, r, l, ms, 1.0 / N, stage[:n])
+            kernels.fft3_inverse_yx_ymajor_f32(stage[:n], U[:n])
+        elif kind == "pack0":
             kernels.multipole_window_ifft_pack0_f32(H, geo.P, r, l, 1.0 / N, U[:S])
         else:
-            kernels.multipole_window_ifft_multi_f32(H, geo.P, r, l, ms, 1.0 / N, U[:len(ms) * S])
+            kernels.multipole_window_ifft_multi_f32(H, geo.P, r, l, ms, 1.0 / N, U[:n])
         tm.stop(t)
         if overlap:
             win_done = torch.cuda.Event()
             win_done.record(main)
-        with torch.cuda.stream(side):
+        with torch.cuda.stream(side), kernels.field_layout("ymajor" if ymajor else "xmajor"):
             if overlap:
                 side.wait_event(win_done)
             t = tm.start("gram")

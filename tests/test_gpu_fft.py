@@ -139,3 +139,23 @@ def test_window_ifft_order_pairs_match_single_orders(phi, J):
             err = float((out[i * S:(i + 1) * S] - one).abs().max() / one.abs().max())
             print(f"  m={m}: max rel diff {err:.2e}", flush=True)
             assert err <= 1e-6, (l, m, err)
+
+
+@pytest.mark.parametrize("J", [5, 7])
+def test_inverse_yx_into_ymajor_order(J):
+    """The strided passes from x-major grids into y-major order give the in-place x-major transform,
+    transposed in (x, y); overlapping grids are refused."""
+    import torch
+    from pyhermes_amd import kernels
+    L = 1 << J
+    print(f"inverse yx into y-major order J={J}", flush=True)
+    x = kernels.as_device(_rand_complex((3, L, L, L), 500 + J), torch.complex64)
+    ref = x.clone()
+    kernels.fft3_inverse_yx_f32(ref)
+    out = torch.empty_like(x)
+    kernels.fft3_inverse_yx_ymajor_f32(x, out)
+    want = ref.transpose(1, 2)  # [b, y, x, z]
+    scale = float(ref.abs().max())
+    assert float((out - want).abs().max()) <= 2e-6 * scale
+    with pytest.raises(RuntimeError):
+        kernels.fft3_inverse_yx_ymajor_f32(x[:2], x[1:])

@@ -262,3 +262,38 @@ def test_headline_path_every_set_matches_oracle(phi):
             worst = max(worst, rel)
             assert rel <= 1e-4, (i, j, rel)
     print(f"headline path: worst relative L2 error over 225 ordered sets {worst:.2e}", flush=True)
+
+
+def test_gram_reads_ymajor_fields(phi):
+    """Auto, cross and split Gram sums over y-major fields (hermes_set_field_layout(1)) equal those over the
+    same fields in x-major order, bit for bit; and the packed auto path in y-major order (the default)
+    agrees with its in-place x-major order to FP32 rounding."""
+    import torch
+    from pyhermes_amd import kernels, multipole
+    L = 64
+    rng = np.random.default_rng(41)
+    d = kernels.as_device(rng.normal(size=(L, L, L)), torch.float32)
+    U = kernels.as_device(rng.normal(size=(5, L, L, L)) + 1j * rng.normal(size=(5, L, L, L)), torch.complex64)
+    V = kernels.as_device(rng.normal(size=(4, L, L, L)) + 1j * rng.normal(size=(4, L, L, L)), torch.complex64)
+    Ut, Vt = U.transpose(1, 2).contiguous(), V.transpose(1, 2).contiguous()
+    lam, Q = np.linalg.eigh(scaling.cal_gamma(phi, 3, 1024))
+    blk_x = torch.zeros(512, dtype=torch.float64, device=U.device)
+    blk_y = torch.zeros_like(blk_x)
+    auto_x = kernels.multipole_gram_f32(d, U, Q, lam)
+    cross_x = kernels.multipole_gram_f32(d, U, Q, lam, V=V, v_conj=True)
+    kernels.multipole_gram_f32_dev_ri(d, U, Q, lam, blk_x)
+    with kernels.field_layout("ymajor"):
+        auto_y = kernels.multipole_gram_f32(d, Ut, Q, lam)
+        cross_y = kernels.multipole_gram_f32(d, Ut, Q, lam, V=Vt, v_conj=True)
+        kernels.multipole_gram_f32_dev_ri(d, Ut, Q, lam, blk_y)
+    assert kernels.get_field_layout() == "xmajor"
+    assert np.array_equal(auto_x, auto_y) and np.array_equal(cross_x, cross_y)
+    assert torch.equal(blk_x, blk_y)
+    geo = multipole.MultipoleGeometry(phi, L)
+    delta = rng.normal(size=(L, L, L))
+    d32 = kernels.as_device(delta - delta.mean(), torch.float32)
+    radii = np.linspace(1.5, 12.0, 10)
+    assert multipole.packs_m0(L, radii.size)
+    ym = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, 6)
+    xm = multipole.ddd_multipole_batched_f32(geo, d32, d32, d32, radii, radii, 0, 6, field_order="xmajor")
+    assert np.linalg.norm(ym - xm) <= 1e-6 * np.linalg.norm(xm)
