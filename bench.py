@@ -617,6 +617,8 @@ def main():
                     help="stream schedule of the multipole pipeline")
     ap.add_argument("--skip-fp64", action="store_true")
     ap.add_argument("--skip-cpu", action="store_true")
+    ap.add_argument("--skip-pass-probe", action="store_true",
+                    help="skip timing the strided inverse passes against a device copy of the same bytes")
     ap.add_argument("--copy-probe-mib", type=int, default=0,
                     help="after the run, copy a buffer of this many MiB once (HBM counter calibration)")
     ap.add_argument("--mc-J", type=int, default=8)
@@ -756,6 +758,30 @@ def main():
     # algorithmic bytes of the stage: the half spectrum read once per call and every output field written
     # once (the intermediate passes' traffic is the FFT's, not the window's)
     win_alg_bytes = win_calls * n_modes * 8 + n_fields * S * N * 8
+    # the two strided inverse passes alone on one call's worth of fields (S of them, x-major in, y-major
+    # out), against a device copy of the same bytes: both move every byte once per read and once per write
+    passes = None
+    if not args.skip_pass_probe:
+        src = torch.zeros((S, L, L, L), dtype=torch.complex64, device="cuda")
+        dst = torch.empty_like(src)
+
+        def med_ms(fn):
+            ts = []
+            for _ in range(4):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record()
+                fn()
+                e1.record()
+                torch.cuda.synchronize()
+                ts.append(e0.elapsed_time(e1))
+            return float(np.median(ts[1:]))
+        t_pass = med_ms(lambda: kernels.fft3_inverse_yx_ymajor_f32(src, dst))
+        t_copy = med_ms(lambda: dst.copy_(src))
+        moved = 2 * S * N * 8  # one read and one write of S complex64 fields
+        passes = {"fields": S, "ms_both_passes": t_pass, "tbs_per_pass": 2 * moved / (t_pass * 1e-3) / 1e12,
+                  "device_copy_ms": t_copy, "device_copy_tbs": moved / (t_copy * 1e-3) / 1e12}
+        del src, dst
+        log(f"strided passes: {t_pass:.2f} ms for {S} fields; device copy of one pass's bytes {t_copy:.2f} ms")
     if args.copy_probe_mib > 0:
         n_el = args.copy_probe_mib * (1 << 20) // 4
         src = torch.ones(n_el, dtype=torch.float32, device="cuda")
@@ -794,7 +820,8 @@ def main():
                         "bytes_per_step": win_bytes,
                         "hbm_frac": win_bytes / (win_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
                         "algorithmic_bytes_per_step": win_alg_bytes,
-                        "hbm_frac_algorithmic": win_alg_bytes / (win_ms * 1e-3) / 1e9 / HBM_PEAK_GBS},
+                        "hbm_frac_algorithmic": win_alg_bytes / (win_ms * 1e-3) / 1e9 / HBM_PEAK_GBS,
+                        "strided_passes": passes},
     }
     line.update(out)
     print(json.dumps(line))

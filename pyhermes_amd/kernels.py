@@ -308,7 +308,7 @@ def multipole_window_zpass_multi_f32(H, P, radii_grid, l, ms, scale, out):
 
 def fft3_inverse_yx_ymajor_f32(src, dst):
     """The y and x inverse passes of the x-major batch ``src`` (z lines already transformed) into ``dst`` in
-    y-major order (dst[..., y, x, z] holds the transform at (x, y, z)); src is left transformed along y only."""
+    y-major order (dst[..., y, x, z] holds the transform at (x, y, z)); src is only read."""
     for t in (src, dst):
         if t.dtype != torch.complex64 or not t.is_contiguous() or t.ndim < 3:
             raise ValueError("fft3_inverse_yx_ymajor_f32 needs contiguous complex64 (.., L, L, L) tensors")
