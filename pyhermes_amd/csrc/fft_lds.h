@@ -29,6 +29,18 @@ __device__ __forceinline__ void lds_barrier() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
 }
 
+// Loads and stores with the nontemporal hint, for data that a kernel reads or writes once and that no later
+// access finds in the caches (passes over grids of hundreds of MB).
+typedef float v2f_t __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 stream_load(const float2* p) {
+    const v2f_t v = __builtin_nontemporal_load((const v2f_t*)p);
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void stream_store(float2* p, float2 a) {
+    const v2f_t v = {a.x, a.y};
+    __builtin_nontemporal_store(v, (v2f_t*)p);
+}
+
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {

@@ -778,7 +778,12 @@ __global__ __launch_bounds__(mpwin_threads<LOG2L>()) void k_mpwin_zfft_multi(
                 if (L % NT == 0 || tid + q * NT < L) v[q] = lds[F::addr(line, tid + q * NT)];
 #pragma unroll
             for (int q = 0; q < EPT; ++q)
-                if (L % NT == 0 || tid + q * NT < L) o[tid + q * NT] = v[q];
+                if (L % NT == 0 || tid + q * NT < L) {
+                    if (LOG2L >= 8)  // the fields are written once and read by the next pass from HBM
+                        hermes_fft::stream_store(o + tid + q * NT, v[q]);
+                    else
+                        o[tid + q * NT] = v[q];
+                }
         }
         hermes_fft::lds_barrier();
     }
