@@ -762,7 +762,8 @@ def main():
     # out), against a device copy of the same bytes: both move every byte once per read and once per write
     passes = None
     if not args.skip_pass_probe:
-        src = torch.zeros((S, L, L, L), dtype=torch.complex64, device="cuda")
+        n_probe = max(1, min(S, (16 << 30) // (N * 8)))  # at most 16 GiB per buffer
+        src = torch.zeros((n_probe, L, L, L), dtype=torch.complex64, device="cuda")
         dst = torch.empty_like(src)
 
         def med_ms(fn):
@@ -777,11 +778,11 @@ def main():
             return float(np.median(ts[1:]))
         t_pass = med_ms(lambda: kernels.fft3_inverse_yx_ymajor_f32(src, dst))
         t_copy = med_ms(lambda: dst.copy_(src))
-        moved = 2 * S * N * 8  # one read and one write of S complex64 fields
-        passes = {"fields": S, "ms_both_passes": t_pass, "tbs_per_pass": 2 * moved / (t_pass * 1e-3) / 1e12,
+        moved = 2 * n_probe * N * 8  # one read and one write of the fields
+        passes = {"fields": n_probe, "ms_both_passes": t_pass, "tbs_per_pass": 2 * moved / (t_pass * 1e-3) / 1e12,
                   "device_copy_ms": t_copy, "device_copy_tbs": moved / (t_copy * 1e-3) / 1e12}
         del src, dst
-        log(f"strided passes: {t_pass:.2f} ms for {S} fields; device copy of one pass's bytes {t_copy:.2f} ms")
+        log(f"strided passes: {t_pass:.2f} ms for {n_probe} fields; device copy of one pass's bytes {t_copy:.2f} ms")
     if args.copy_probe_mib > 0:
         n_el = args.copy_probe_mib * (1 << 20) // 4
         src = torch.ones(n_el, dtype=torch.float32, device="cuda")
