@@ -141,7 +141,23 @@ def test_window_ifft_order_pairs_match_single_orders(phi, J):
             assert err <= 1e-6, (l, m, err)
 
 
-@pytest.mark.parametrize("J", [5, 7])
+def test_inverse_yx_streaming_passes_match_numpy():
+    """From 256^3 up the strided passes use streaming loads and stores: the in-place y and x inverse passes at
+    J=8 against numpy's unnormalised 2-D inverse transform over (x, y)."""
+    import torch
+    from pyhermes_amd import kernels
+    J = 8
+    L = 1 << J
+    print(f"inverse yx passes vs numpy J={J}", flush=True)
+    h = _rand_complex((1, L, L, L), 808)
+    x = kernels.as_device(h, torch.complex64)
+    kernels.fft3_inverse_yx_f32(x)
+    want = np.fft.ifft2(h.astype(np.complex128), axes=(1, 2)) * (L * L)
+    got = x.cpu().numpy()
+    assert np.abs(got - want).max() <= 1e-5 * np.abs(want).max()
+
+
+@pytest.mark.parametrize("J", [5, 7, 8])
 def test_inverse_yx_into_ymajor_order(J):
     """The strided passes from x-major grids into y-major order give the in-place x-major transform,
     transposed in (x, y); overlapping grids are refused."""
