@@ -28,6 +28,10 @@ hipError_t hermes_fill_u32(hipStream_t st, void* p, size_t n, uint32_t v) {
 }
 
 void* hermes_workspace(hermes_ctx* ctx, int slot, size_t bytes) {
+    if (slot < 0 || slot >= HERMES_WS_SLOTS) {
+        hermes_set_error("workspace slot %d outside 0..%d", slot, HERMES_WS_SLOTS - 1);
+        return nullptr;
+    }
     if (bytes == 0) bytes = 256;
     if (ctx->ws_size[slot] >= bytes) return ctx->ws[slot];
     if (ctx->capturing) {

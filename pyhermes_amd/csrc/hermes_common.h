@@ -11,13 +11,27 @@
 
 struct hermes_fft_cache;
 
+// Workspace slots (each slot is an independent grow-only buffer).
+enum hermes_ws_slot {
+    WS_PROJ_TILE = 0,   // projection: per-tile counters / offsets
+    WS_PROJ_REC = 1,    // projection: bucketed particle records
+    WS_REDUCE = 2,      // reductions: per-block partial sums
+    WS_SPEC = 3,        // window / FFT staging
+    WS_MISC = 4,
+    WS_MC = 5,
+    WS_MLP = 6,
+    WS_CONTRACT = 7,
+    WS_PROFILE = 8,     // radial profiles of the fused window kernel
+};
+constexpr int HERMES_WS_SLOTS = 9;
+
 // One library context per device: stream, grow-only workspaces and FFT plans.
 struct hermes_ctx {
     int device = 0;
     hipStream_t stream = nullptr;  // default stream unless the caller installs one
     // grow-only scratch buffers owned by the context
-    void* ws[9] = {nullptr};
-    size_t ws_size[9] = {0};
+    void* ws[HERMES_WS_SLOTS] = {nullptr};
+    size_t ws_size[HERMES_WS_SLOTS] = {0};
     hermes_fft_cache* fft = nullptr;
     float2* twiddles = nullptr;  // e^{+-2 pi i k / 2^j} tables for the LDS FFTs (first use)
     // autotuned strided-pass tile per (log2 L, inverse, axis: 0 = y lines, 1 = x lines); 0 = not tuned
@@ -37,19 +51,6 @@ struct hermes_ctx {
     int* host_word = nullptr;  // pinned host word for hermes_read_word (allocated on first use)
 };
 
-// Workspace slots (each slot is an independent grow-only buffer).
-enum hermes_ws_slot {
-    WS_PROJ_TILE = 0,   // projection: per-tile counters / offsets
-    WS_PROJ_REC = 1,    // projection: bucketed particle records
-    WS_REDUCE = 2,      // reductions: per-block partial sums
-    WS_SPEC = 3,        // window / FFT staging
-    WS_MISC = 4,
-    WS_MC = 5,
-    WS_MLP = 6,
-    WS_CONTRACT = 7,
-    WS_PROFILE = 8,     // radial profiles of the fused window kernel
-};
-constexpr int HERMES_WS_SLOTS = 9;
 
 void hermes_set_error(const char* fmt, ...);
 // Fill n 32-bit words with v on the stream (a kernel, so it can be captured into graphs).
