@@ -93,10 +93,10 @@ class ProposalMLP:
         it = 0
         for ep in range(epochs):
             perm = torch.randperm(n, device=x.device, generator=gen)
+            xp, yp = x[perm], y[perm]  # the epoch's order once; the batches are then contiguous row ranges
             tot = 0.0
             for s0 in range(0, n, batch):
-                idx = perm[s0:s0 + batch]
-                xb, yb = x[idx].contiguous(), y[idx].contiguous()
+                xb, yb = xp[s0:s0 + batch], yp[s0:s0 + batch]
                 kernels.mlp_grad(self.wb, xb, yb, clip, self.grad)
                 self.step += 1
                 rate = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * it / max(total - 1, 1)))
@@ -134,21 +134,31 @@ class LearnedProposal:
                                                                                torch.float64)
         self.mlp = ProposalMLP(seed=seed)
         Ls = self.L * self.sub
-        # bucket edges: quantiles of the field at a random sample of sub-cell centres
+        # bucket edges: quantiles (linear interpolation between order statistics) of the field at a random
+        # sample of sub-cell centres, all on the device from the sorted sample
+        dev = grid.device
         rng = np.random.default_rng(seed)
-        idx = rng.integers(0, Ls ** 3, size=min(int(n_quantile), Ls ** 3))
-        centres = (np.stack([idx // (Ls * Ls), (idx // Ls) % Ls, idx % Ls], axis=1) + 0.5) / self.sub
-        v = kernels.interpolate(grid, kernels.as_device(centres, torch.float64), self.phi).cpu().numpy()
-        self.edges = np.quantile(v, np.linspace(0.0, 1.0, self.nbucket + 1)[1:-1])
+        idx = torch.from_numpy(rng.integers(0, Ls ** 3, size=min(int(n_quantile), Ls ** 3))).to(dev)
+        centres = (torch.stack([idx // (Ls * Ls), (idx // Ls) % Ls, idx % Ls], dim=1).double() + 0.5) / self.sub
+        vs = torch.sort(kernels.interpolate(grid, centres.contiguous(), self.phi)).values
+        n = int(vs.numel())
+        pos = torch.linspace(0.0, 1.0, self.nbucket + 1, dtype=torch.float64, device=dev)[1:-1] * (n - 1)
+        lo = pos.floor().long()
+        hi = torch.clamp(lo + 1, max=n - 1)
+        edges_t = vs[lo] + (vs[hi] - vs[lo]) * (pos - lo.double())
+        self.edges = edges_t.cpu().numpy()
         self.cell_bucket, self.bucket_cells, self.bucket_off, counts = kernels.mc_bucketize(
             grid, self.phi, self.sub, self.edges)
         self.n_k = counts.astype(np.float64)
-        # representative feature of each bucket: mean of asinh(v / scale) over its sampled sub-cells
-        self.rho_scale = float(v.std()) or 1.0
-        t = np.arcsinh(v / self.rho_scale)
-        kv = np.searchsorted(self.edges, v, side="right")
-        cnt = np.bincount(kv, minlength=self.nbucket).astype(np.float64)
-        t_k = np.bincount(kv, weights=t, minlength=self.nbucket) / np.maximum(cnt, 1.0)
+        # representative feature of each bucket: mean of asinh(v / scale) over its sampled sub-cells; bucket k
+        # (edges[k - 1] <= v < edges[k]) is a run of the sorted sample, summed by prefix sums
+        self.rho_scale = float(vs.std(correction=0)) or 1.0
+        csum = torch.cat([torch.zeros(1, dtype=torch.float64, device=dev),
+                          torch.cumsum(torch.asinh(vs / self.rho_scale), 0)])
+        bnd = torch.cat([torch.zeros(1, dtype=torch.long, device=dev), torch.searchsorted(vs, edges_t),
+                         torch.full((1,), n, dtype=torch.long, device=dev)])
+        cnt = (bnd[1:] - bnd[:-1]).double().cpu().numpy()
+        t_k = ((csum[bnd[1:]] - csum[bnd[:-1]]).cpu().numpy()) / np.maximum(cnt, 1.0)
         te = np.arcsinh(np.asarray(self.edges) / self.rho_scale)
         for k in np.nonzero(cnt == 0)[0]:  # unsampled bucket: its edge interval
             t_k[k] = te[min(k, te.size - 1)] if te.size else 0.0
