@@ -421,6 +421,12 @@ int hermes_mlp_grad(hermes_ctx *ctx, const void *wb, const float *x, const float
 /* One Adam step on the FP32 weights w with moments m, v; refreshes the bf16 copy wb. */
 int hermes_mlp_adam(hermes_ctx *ctx, float *w, float *m, float *v, const float *grad, void *wb, float lr,
                     float beta1, float beta2, float eps, int step);
+/* n_steps (hermes_mlp_grad, hermes_mlp_adam) pairs over consecutive row ranges of x (n, 4) and y (n):
+ * step s uses rows s*batch .. min((s+1)*batch, n)-1, step size lr_host[s] (host array of n_steps) and Adam
+ * counter step0 + s + 1.  Identical to the per-step calls, without host round trips between steps. */
+int hermes_mlp_train_steps(hermes_ctx *ctx, float *w, float *m, float *v, float *grad, void *wb, const float *x,
+                           const float *y, int64_t n, int batch, float clip, const float *lr_host, int n_steps,
+                           int step0, float beta1, float beta2, float eps);
 
 #ifdef __cplusplus
 }

@@ -114,6 +114,9 @@ _SIGNATURES.update({
     "hermes_mlp_grad": [c_void_p, c_void_p, c_void_p, c_void_p, c_int, ctypes.c_float, c_void_p],
     "hermes_mlp_adam": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, ctypes.c_float, ctypes.c_float,
                         ctypes.c_float, ctypes.c_float, c_int],
+    "hermes_mlp_train_steps": [c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_void_p, c_i64,
+                               c_int, ctypes.c_float, ctypes.POINTER(ctypes.c_float), c_int, c_int, ctypes.c_float,
+                               ctypes.c_float, ctypes.c_float],
 })
 
 

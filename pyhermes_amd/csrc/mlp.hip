@@ -253,4 +253,24 @@ int hermes_mlp_adam(hermes_ctx* ctx, float* w, float* m, float* v, const float* 
     return 0;
 }
 
+// n_steps Adam steps over consecutive row ranges of (x, y): step s takes rows s * batch .. min((s + 1) * batch,
+// n) - 1, step size lr_host[s] and Adam step counter step0 + s + 1; the same launches as hermes_mlp_grad then
+// hermes_mlp_adam per step, without a host round trip between steps.
+int hermes_mlp_train_steps(hermes_ctx* ctx, float* w, float* m, float* v, float* grad, void* wb, const float* x,
+                           const float* y, int64_t n, int batch, float clip, const float* lr_host, int n_steps,
+                           int step0, float beta1, float beta2, float eps) {
+    HERMES_REQUIRE(ctx && w && m && v && grad && wb && x && y && lr_host && n >= 1 && batch >= 1 &&
+                       batch <= (1 << 18) && n_steps >= 0 && step0 >= 0,
+                   "hermes_mlp_train_steps: invalid argument");
+    HERMES_REQUIRE((int64_t)n_steps * batch < n + batch, "hermes_mlp_train_steps: %d steps of %d rows exceed %lld rows",
+                   n_steps, batch, (long long)n);
+    for (int s = 0; s < n_steps; ++s) {
+        const int64_t off = (int64_t)s * batch;
+        const int nb = (int)(n - off < batch ? n - off : batch);
+        if (int rc = hermes_mlp_grad(ctx, wb, x + off * IN, y + off, nb, clip, grad)) return rc;
+        if (int rc = hermes_mlp_adam(ctx, w, m, v, grad, wb, lr_host[s], beta1, beta2, eps, step0 + s + 1)) return rc;
+    }
+    return 0;
+}
+
 }  // extern "C"

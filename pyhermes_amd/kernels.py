@@ -650,6 +650,20 @@ def mlp_grad(wb, x, y, clip, grad_out):
     return grad_out
 
 
+def mlp_train_steps(w, m, v, grad, wb, x, y, batch, clip, lrs, step0, beta1=0.9, beta2=0.999, eps=1e-8):
+    """len(lrs) Adam steps over consecutive row ranges of batch rows of the contiguous (n, 4) x and (n,) y."""
+    for t in (x, y):
+        if t.dtype != torch.float32 or not t.is_contiguous():
+            raise ValueError("mlp_train_steps needs contiguous float32 x and y")
+    n = int(x.shape[0])
+    if x.ndim != 2 or x.shape[1] != 4 or y.shape != (n,):
+        raise ValueError("mlp_train_steps: x must be (n, 4) and y (n,)")
+    lr = np.ascontiguousarray(lrs, dtype=np.float32)
+    _lib.call("hermes_mlp_train_steps", _lib.context(), ptr(w), ptr(m), ptr(v), ptr(grad), ptr(wb), ptr(x), ptr(y), n,
+              int(batch), float(clip), lr.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), int(lr.size), int(step0),
+              float(beta1), float(beta2), float(eps))
+
+
 def mlp_adam(w, m, v, grad, wb, lr, step, beta1=0.9, beta2=0.999, eps=1e-8):
     _lib.call("hermes_mlp_adam", _lib.context(), ptr(w), ptr(m), ptr(v), ptr(grad), ptr(wb), float(lr), float(beta1),
               float(beta2), float(eps), int(step))

@@ -94,14 +94,13 @@ class ProposalMLP:
         for ep in range(epochs):
             perm = torch.randperm(n, device=x.device, generator=gen)
             xp, yp = x[perm], y[perm]  # the epoch's order once; the batches are then contiguous row ranges
-            tot = 0.0
-            for s0 in range(0, n, batch):
-                xb, yb = xp[s0:s0 + batch], yp[s0:s0 + batch]
-                kernels.mlp_grad(self.wb, xb, yb, clip, self.grad)
-                self.step += 1
-                rate = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * it / max(total - 1, 1)))
-                it += 1
-                kernels.mlp_adam(self.w, self.m, self.v, self.grad, self.wb, float(rate), self.step)
+            steps = (n + batch - 1) // batch
+            its = np.arange(it, it + steps, dtype=np.float64)
+            rates = lr_final + 0.5 * (lr - lr_final) * (1.0 + np.cos(np.pi * its / max(total - 1, 1)))
+            kernels.mlp_train_steps(self.w, self.m, self.v, self.grad, self.wb, xp, yp, batch, clip, rates,
+                                    self.step)
+            self.step += steps
+            it += steps
             # epoch loss on a fixed subsample
             sub = min(n, 65536)
             kernels.mlp_grad(self.wb, x[:sub].contiguous(), y[:sub].contiguous(), clip, self.grad)

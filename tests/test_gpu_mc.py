@@ -167,6 +167,29 @@ def test_mlp_gradient_matches_oracle():
     np.testing.assert_allclose(s_got, s_ref, rtol=1e-5, atol=1e-5)
 
 
+def test_mlp_train_steps_equal_per_step_calls():
+    """The device-side training loop (hermes_mlp_train_steps) gives bit-identical weights, moments and bf16
+    copy to the same steps issued one hermes_mlp_grad / hermes_mlp_adam pair at a time, with a short last
+    batch."""
+    import torch
+    from pyhermes_amd import kernels, mcsample
+    rng = np.random.default_rng(8)
+    n, batch = 5000, 1024
+    x = kernels.as_device(rng.uniform(-1, 1, size=(n, 4)).astype(np.float32), torch.float32)
+    y = kernels.as_device(rng.exponential(1.0, size=n).astype(np.float32), torch.float32)
+    steps = (n + batch - 1) // batch
+    lrs = np.linspace(3e-3, 1e-3, steps).astype(np.float32)
+    a, b = mcsample.ProposalMLP(seed=5), mcsample.ProposalMLP(seed=5)
+    for s in range(steps):
+        kernels.mlp_grad(a.wb, x[s * batch:(s + 1) * batch], y[s * batch:(s + 1) * batch], 50.0, a.grad)
+        kernels.mlp_adam(a.w, a.m, a.v, a.grad, a.wb, float(lrs[s]), 7 + s + 1)
+    kernels.mlp_train_steps(b.w, b.m, b.v, b.grad, b.wb, x, y, batch, 50.0, lrs, 7)
+    for u, v in ((a.w, b.w), (a.m, b.m), (a.v, b.v), (a.wb, b.wb)):
+        assert torch.equal(u, v)
+    with pytest.raises(RuntimeError):
+        kernels.mlp_train_steps(b.w, b.m, b.v, b.grad, b.wb, x, y, batch, 50.0, np.zeros(steps + 1), 0)
+
+
 def test_learned_proposal_is_unbiased_and_reduces_variance(phi):
     import torch
     from pyhermes_amd import kernels, mcsample
