@@ -108,6 +108,9 @@ class CatalogueBatch:
         self.dc = torch.empty(Lm ** 3, dtype=torch.complex64, device=dev)
         self.H = torch.empty(Lm * Lm * (Lm // 2 + 1), dtype=torch.complex64, device=dev)
         self.U = torch.empty(self.pack * S * Lm ** 3, dtype=torch.complex64, device=dev)
+        # the window z passes write x-major fields here; the strided passes move them into U in y-major order
+        # (both passes then read lines L elements apart), which the Grams read
+        self.stage = torch.empty_like(self.U)
         self.pp_out = torch.zeros((len(self.cats), self.s_bins.size), dtype=torch.float64, device=dev)
         self.gram_out = torch.zeros((len(self.cats), len(self.groups), 512), dtype=torch.float64, device=dev)
         self._warm = False
@@ -132,20 +135,24 @@ class CatalogueBatch:
         dv["plan"].reduce(H, H, self.P[J], self.pp_out[i])
         S = self.shells.size
         for k, (kind, group) in enumerate(self.groups):
+            n = len(group) * S
             if kind == "pack0":
                 for pos, l in enumerate(group):
-                    Uj = self.U[pos * S * N:(pos + 1) * S * N].view(S, L, L, L)
-                    kernels.multipole_window_ifft_pack0_f32(H, self.P[J], dv["radii"], l, 1.0 / N, Uj)
-                U = self.U[:len(group) * S * N].view(len(group) * S, L, L, L)
-                kernels.multipole_gram_f32_dev_ri(d, U, self.Q, self.lam, self.gram_out[i, k])
-                continue
-            pos = 0
-            for l, ms in _window_runs(group):  # orders of one l share the radial factors
-                Uj = self.U[pos * S * N:(pos + len(ms)) * S * N].view(len(ms) * S, L, L, L)
-                kernels.multipole_window_ifft_multi_f32(H, self.P[J], dv["radii"], l, ms, 1.0 / N, Uj)
-                pos += len(ms)
-            U = self.U[:len(group) * S * N].view(len(group) * S, L, L, L)
-            kernels.multipole_gram_f32_dev(d, U, self.Q, self.lam, self.gram_out[i, k, :256])
+                    Sj = self.stage[pos * S * N:(pos + 1) * S * N].view(S, L, L, L)
+                    kernels.multipole_window_zpass_pack0_f32(H, self.P[J], dv["radii"], l, 1.0 / N, Sj)
+            else:
+                pos = 0
+                for l, ms in _window_runs(group):  # orders of one l share the radial factors
+                    Sj = self.stage[pos * S * N:(pos + len(ms)) * S * N].view(len(ms) * S, L, L, L)
+                    kernels.multipole_window_zpass_multi_f32(H, self.P[J], dv["radii"], l, ms, 1.0 / N, Sj)
+                    pos += len(ms)
+            U = self.U[:n * N].view(n, L, L, L)
+            kernels.fft3_inverse_yx_ymajor_f32(self.stage[:n * N].view(n, L, L, L), U)
+            with kernels.field_layout("ymajor"):
+                if kind == "pack0":
+                    kernels.multipole_gram_f32_dev_ri(d, U, self.Q, self.lam, self.gram_out[i, k])
+                else:
+                    kernels.multipole_gram_f32_dev(d, U, self.Q, self.lam, self.gram_out[i, k, :256])
 
     def warmup(self):
         """One eager catalogue per level (FFT tables, tuned tiles, workspaces) and the projection
